@@ -1,0 +1,145 @@
+// dfrg/device.hh -- device building blocks shared by the C-ABI kernels (diffrg_amd/csrc)
+// and the drop-in integrator templates (include/DiFfRG/...).  gfx950 (CDNA4), wave64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dfrg
+{
+  namespace dev
+  {
+    constexpr int wave = 64;
+    constexpr double pi = 3.14159265358979323846;
+
+    // ------------------------------------------------------------------------------------------
+    // wave64 sum: DPP within each 16-lane row (quad_perm xor1, xor2, row_half_mirror,
+    // row_mirror), then two cross-row swaps.  Partners always add the same two operands,
+    // so every lane ends with the bitwise-identical total (a fixed, data-independent tree).
+    // ------------------------------------------------------------------------------------------
+    template <int ctrl> __device__ __forceinline__ double dpp_f64(double v)
+    {
+      const uint64_t u = __double_as_longlong(v);
+      const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, ctrl, 0xF, 0xF, false);
+      const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), ctrl, 0xF, 0xF, false);
+      return __longlong_as_double(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+    template <int ctrl> __device__ __forceinline__ float dpp_f32(float v)
+    {
+      return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+    }
+
+    __device__ __forceinline__ double wave_sum(double v)
+    {
+      v += dpp_f64<0xB1>(v);  // quad_perm(1,0,3,2): lane ^ 1
+      v += dpp_f64<0x4E>(v);  // quad_perm(2,3,0,1): lane ^ 2
+      v += dpp_f64<0x141>(v); // row_half_mirror: pairs the two quads of each 8-lane half
+      v += dpp_f64<0x140>(v); // row_mirror: pairs the two halves of each 16-lane row
+      v += __shfl_xor(v, 16, wave);
+      v += __shfl_xor(v, 32, wave);
+      return v;
+    }
+
+    // Block sum in a fixed order: wave sums to LDS, then wave 0 adds them in index order.
+    // `smem` needs blockDim.x / 64 doubles.  Returns the total to every thread of the block.
+    __device__ __forceinline__ double block_sum(double v, double *smem)
+    {
+      const int lane = threadIdx.x & (wave - 1);
+      const int wid = threadIdx.x / wave;
+      const int nw = (blockDim.x + wave - 1) / wave;
+      v = wave_sum(v);
+      __syncthreads();
+      if (lane == 0) smem[wid] = v;
+      __syncthreads();
+      double t = 0.0;
+      for (int i = 0; i < nw; ++i) t += smem[i];
+      __syncthreads();
+      return t;
+    }
+
+    // Compensated (Neumaier) running sum in FP64.
+    struct KahanSum {
+      double s = 0.0, c = 0.0;
+      __device__ __host__ __forceinline__ void add(double v)
+      {
+        const double t = s + v;
+        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+        s = t;
+      }
+      __device__ __host__ __forceinline__ double value() const { return s + c; }
+    };
+
+    // ------------------------------------------------------------------------------------------
+    // Deterministic cross-block reduction.  Each block writes its partial, publishes it with a
+    // release fence and takes a ticket; the block that takes the last ticket of a group sums all
+    // partials of that group in index order (compensated) and resets the group's counter so the
+    // next launch (or graph replay) starts from zero.  Returns true in the finishing block, with
+    // the group total in *total (valid in thread 0).
+    // ------------------------------------------------------------------------------------------
+    __device__ __forceinline__ bool finish_group(double partial, double *partials, int slot, int first_slot,
+                                                 int n_slots, unsigned *counter, double *total)
+    {
+      __shared__ bool last;
+      if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[slot], partial, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned ticket = atomicAdd(counter, 1u);
+        last = (ticket == (unsigned)(n_slots - 1));
+      }
+      __syncthreads();
+      if (!last) return false;
+      if (threadIdx.x == 0) {
+        __threadfence();
+        KahanSum s;
+        for (int i = 0; i < n_slots; ++i)
+          s.add(__hip_atomic_load(&partials[first_slot + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        *total = s.value();
+        atomicExch(counter, 0u);
+      }
+      return true;
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Exponential regulator R = k^2 exp(-(q^2/k^2)^b), b = 1 specialisation used by the
+    // built-in integrands (regulators.hh:144-191 form).
+    // ------------------------------------------------------------------------------------------
+    __device__ __forceinline__ double exp_RB_b1(double k2, double q2) { return k2 * exp(-(q2 / k2)); }
+    __device__ __forceinline__ double exp_RBdot_b1(double k2, double q2)
+    {
+      const double xb = q2 / k2;
+      return 2.0 * exp(-xb) * k2 * (1.0 + xb);
+    }
+
+    // Log-coordinate lerp: idx = log1p((y - c) * inv_b) * scale, clamped to [0, n-1]
+    // (coordinates.hh:182-186 backward, linear_interpolation_1D.hh:75-92 lerp).
+    __device__ __forceinline__ double lerp_log_f64(const double *tab, int n, double c, double inv_b, double scale,
+                                                   double y)
+    {
+      double idx = log1p((y - c) * inv_b) * scale;
+      idx = fmax(0.0, fmin(idx, (double)(n - 1)));
+      const double fl = floor(idx);
+      const int i0 = (int)fl;
+      const int i1 = (int)ceil(idx);
+      return tab[i0] * (1.0 - idx + fl) + tab[i1] * (idx - fl);
+    }
+
+    // FP32 variant: hardware log2 / floor, table in FP32.
+    __device__ __forceinline__ float lerp_log_f32(const float *tab, int n, float c, float inv_b, float scale_ln2,
+                                                  float y)
+    {
+      // log1p(t) * scale = log2(1 + t) * (ln2 * scale)
+      float idx = __builtin_amdgcn_logf(1.0f + (y - c) * inv_b) * scale_ln2;
+      idx = fmaxf(0.0f, fminf(idx, (float)(n - 1))); // NaN-safe clamp
+      const float fl = __builtin_floorf(idx);
+      const float fr = idx - fl;
+      const int i0 = (int)fl;
+      const int i1 = i0 + (fr > 0.0f ? 1 : 0);
+      return tab[i0] + fr * (tab[i1] - tab[i0]);
+    }
+
+    __device__ __forceinline__ float fast_exp2f(float v) { return __builtin_amdgcn_exp2f(v); }
+    __device__ __forceinline__ float fast_rcpf(float v) { return __builtin_amdgcn_rcpf(v); }
+    __device__ __forceinline__ float fast_sqrtf(float v) { return __builtin_amdgcn_sqrtf(v); }
+  } // namespace dev
+} // namespace dfrg

@@ -1,0 +1,248 @@
+"""Batched integrals on the GPU against the oracle on the same seeded inputs and nodes.
+
+Tolerances: FP64 (plain) relative error <= 1e-12; mixed precision (FP32 integrand, FP64
+nodes/denominators/accumulation) relative error <= 1e-8.  Determinism: a result does not
+depend on the batch it runs in, on the launch shape's grid, or on graph replay.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import diffrg_amd.workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+TOL_MIXED = 1e-8
+
+
+def rel_err(a, b):
+    return np.abs(a - b) / np.abs(b)
+
+
+# ------------------------------------------------------------------------------ threshold (C0)
+
+def test_threshold_fp64_and_mixed(gpu, qp):
+    c = W.threshold()
+    x, w = qp.get_points(64), qp.get_weights(64)
+    ref = O.c0_threshold(c["m2"], x, w, c["x_extent"], c["k"], 1.0, c["d"])
+    d_m2 = gpu.DeviceArray.from_host(c["m2"])
+    d_x, d_w = gpu.DeviceArray.from_host(x), gpu.DeviceArray.from_host(w)
+    out = gpu.DeviceArray(len(c["m2"]))
+    for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+        print(f"[case] threshold 256x64 mode={mode}", flush=True)
+        gpu.threshold_batch(d_m2, d_x, d_w, out, c["x_extent"], c["k"], mode, c["d"])
+        got = out.to_host()
+        err = rel_err(got, ref)
+        print(f"  max rel err {err.max():.3e}", flush=True)
+        assert err.max() <= tol
+
+
+def test_threshold_near_pole_mixed(gpu, qp):
+    # m2 -> -k^2: the FP64 denominator keeps the mixed path accurate
+    m2 = np.array([-0.99, -0.995, -0.999, -0.9999])
+    x, w = qp.get_points(64), qp.get_weights(64)
+    ref = O.c0_threshold(m2, x, w, 20.0, 1.0, 1.0, 4)
+    d_m2, d_x, d_w = (gpu.DeviceArray.from_host(a) for a in (m2, x, w))
+    out = gpu.DeviceArray(len(m2))
+    print("[case] threshold near pole mixed", flush=True)
+    gpu.threshold_batch(d_m2, d_x, d_w, out, 20.0, 1.0, gpu.MIXED, 4)
+    err = rel_err(out.to_host(), ref)
+    print(f"  max rel err {err.max():.3e}", flush=True)
+    assert err.max() <= TOL_MIXED
+
+
+# ------------------------------------------------------------------------------ self-energy (C1)
+
+def selfenergy_batch(gpu, qp, c):
+    n = len(c["p"])
+    return gpu.SEBatch(qp, c["p"], c["nq"], c["na"], [c["Z"]] * n, *[[v] * n for v in c["coord"]],
+                       x_extent=c["x_extent"], k=c["k"])
+
+
+def selfenergy_oracle(qp, c):
+    n = len(c["p"])
+    xq = (qp.get_points(c["nq"]), qp.get_weights(c["nq"]))
+    xa = (qp.get_points(c["na"]), qp.get_weights(c["na"]))
+    coord = O.logcoord(c["n_tab"], *c["coord"])
+    return O.c1_selfenergy(c["p"], [c["Z"]] * n, [coord] * n, [xq] * n, [xa] * n, c["x_extent"], c["k"])
+
+
+def test_selfenergy_fp64_and_mixed(gpu, qp):
+    c = W.selfenergy()
+    ref = selfenergy_oracle(qp, c)
+    b = selfenergy_batch(gpu, qp, c)
+    assert b.evals == 64 * 64 * 32 * 32
+    out = gpu.DeviceArray(len(c["p"]))
+    for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+        print(f"[case] selfenergy 64x(64,32,32) mode={mode} shape={b.shape()}", flush=True)
+        b.run(out, mode)
+        err = rel_err(out.to_host(), ref)
+        print(f"  max rel err {err.max():.3e}", flush=True)
+        assert err.max() <= tol
+
+
+@pytest.mark.parametrize("block,tile,grid", [(64, 1024, 0), (128, 4096, 0), (512, 32768, 0), (256, 8192, 512)])
+def test_selfenergy_shapes(gpu, qp, block, tile, grid):
+    c = W.selfenergy(n_p=16)
+    ref = selfenergy_oracle(qp, c)
+    b = selfenergy_batch(gpu, qp, c)
+    b.set_shape(block, tile, grid)
+    out = gpu.DeviceArray(len(c["p"]))
+    print(f"[case] selfenergy shape block={block} tile={tile} grid={grid}", flush=True)
+    b.run(out, gpu.FP64)
+    assert rel_err(out.to_host(), ref).max() <= TOL_FP64
+
+
+# ------------------------------------------------------------------------------ finite T (C2)
+
+def finite_T_device(gpu, c, qp):
+    x, w = qp.get_points(c["nx"]), qp.get_weights(c["nx"])
+    temps = sorted(c["rules"])
+    mx = np.concatenate([c["rules"][t][0] for t in temps])
+    mw = np.concatenate([c["rules"][t][1] for t in temps])
+    off = {t: i * len(c["rules"][t][0]) for i, t in enumerate(temps)}
+    m_off = np.array([off[t] for t in c["T"]], np.int32)
+    m_n = np.array([len(c["rules"][t][0]) for t in c["T"]], np.int32)
+    D = gpu.DeviceArray.from_host
+    dev = {"T": D(c["T"]), "p0": D(c["p0"]), "pabs": D(c["pabs"]), "m1": D(c["m1"]), "m2": D(c["m2"]),
+           "mx": D(mx), "mw": D(mw), "m_off": D(m_off), "m_n": D(m_n), "x": D(x), "w": D(w)}
+    return dev, x, w
+
+
+def test_finite_T_fp64_and_mixed(gpu, qp):
+    c = W.finite_T()
+    dev, x, w = finite_T_device(gpu, c, qp)
+    mats = [c["rules"][t] for t in c["T"]]
+    ref = O.c2_finiteT(c["T"], c["p0"], c["pabs"], c["m1"], c["m2"], mats, x, w, c["q_max"], c["k"], 1.0, c["d"],
+                       c["radial_mode"])
+    out = gpu.DeviceArray(len(c["T"]))
+    for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+        print(f"[case] finite-T 512x48x129 mode={mode}", flush=True)
+        gpu.ft_batch(dev, out, c["q_max"], c["k"], mode, c["d"], c["radial_mode"])
+        got = out.to_host()
+        err = rel_err(got, ref)
+        for t in W.TEMPS:
+            sel = c["T"] == t
+            print(f"  T={t}: max rel err {err[sel].max():.3e}", flush=True)
+        assert err.max() <= tol
+
+
+def test_finite_T_reference_mapping(gpu, qp):
+    # radial_mode 0: the reference's x = q^2/k^2 mapping with a Monien rule from the provider
+    c = W.finite_T(n=32)
+    T = c["T"]
+    rules = {}
+    for t in W.TEMPS:
+        rules[t] = (qp.get_matsubara_points(t, 1.0), qp.get_matsubara_weights(t, 1.0))
+    c["rules"] = rules
+    c["radial_mode"] = 0
+    dev, x, w = finite_T_device(gpu, c, qp)
+    mats = [rules[t] for t in T]
+    ref = O.c2_finiteT(T, c["p0"], c["pabs"], c["m1"], c["m2"], mats, x, w, 20.0, 1.0, 1.0, 4, 0)
+    out = gpu.DeviceArray(len(T))
+    print("[case] finite-T reference mapping", flush=True)
+    gpu.ft_batch(dev, out, 20.0, 1.0, gpu.FP64, 4, 0)
+    assert rel_err(out.to_host(), ref).max() <= TOL_FP64
+
+
+# ------------------------------------------------------------------------------ uneven batch (C3)
+
+def uneven_subset(n_sub):
+    c = W.uneven()
+    return {"p": c["p"][:n_sub], "nq": c["nq"][:n_sub], "na": c["na"][:n_sub], "tables": c["tables"][:n_sub],
+            "coord": c["coord"], "x_extent": c["x_extent"], "k": c["k"]}
+
+
+def uneven_oracle(qp, c):
+    n = len(c["p"])
+    xq = [(qp.get_points(int(v)), qp.get_weights(int(v))) for v in c["nq"]]
+    xa = [(qp.get_points(int(v)), qp.get_weights(int(v))) for v in c["na"]]
+    coords = [O.logcoord(len(t), *c["coord"]) for t in c["tables"]]
+    return O.c1_selfenergy(c["p"], c["tables"], coords, xq, xa, c["x_extent"], c["k"])
+
+
+def make_uneven(gpu, qp, c):
+    n = len(c["p"])
+    return gpu.SEBatch(qp, c["p"], c["nq"], c["na"], c["tables"], *[[v] * n for v in c["coord"]],
+                       x_extent=c["x_extent"], k=c["k"])
+
+
+def test_uneven_batch_parity(gpu, qp):
+    c = uneven_subset(384)
+    ref = uneven_oracle(qp, c)
+    b = make_uneven(gpu, qp, c)
+    out = gpu.DeviceArray(len(c["p"]))
+    for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+        for grid in (0, 1024):
+            b.set_shape(256, 8192, grid)
+            print(f"[case] uneven 384 mode={mode} grid={grid}", flush=True)
+            b.run(out, mode)
+            err = rel_err(out.to_host(), ref)
+            print(f"  max rel err {err.max():.3e}", flush=True)
+            assert err.max() <= tol
+
+
+def test_uneven_alone_equals_in_mix(gpu, qp):
+    c = uneven_subset(256)
+    b = make_uneven(gpu, qp, c)
+    out = gpu.DeviceArray(len(c["p"]))
+    for mode in (gpu.FP64, gpu.MIXED):
+        print(f"[case] alone vs mix mode={mode}", flush=True)
+        b.run(out, mode)
+        mix = out.to_host()
+        b.run(out, mode)
+        assert np.array_equal(out.to_host(), mix)  # run to run
+        for i in (0, 7, 100, 255):
+            one = {"p": c["p"][i:i + 1], "nq": c["nq"][i:i + 1], "na": c["na"][i:i + 1],
+                   "tables": c["tables"][i:i + 1], "coord": c["coord"], "x_extent": 20.0, "k": 1.0}
+            b1 = make_uneven(gpu, qp, one)
+            o1 = gpu.DeviceArray(1)
+            b1.run(o1, mode)
+            assert o1.to_host()[0] == mix[i]  # bitwise
+
+
+def test_uneven_graph_replay(gpu, qp):
+    c = uneven_subset(128)
+    b = make_uneven(gpu, qp, c)
+    out = gpu.DeviceArray(len(c["p"]))
+    for grid in (0, 512):
+        b.set_shape(256, 4096, grid)
+        b.run(out, gpu.MIXED)
+        eager = out.to_host()
+        for rep in range(3):
+            print(f"[case] graph replay grid={grid} rep={rep}", flush=True)
+            out2 = gpu.DeviceArray(len(c["p"]))
+            b.graph_launch(out, gpu.MIXED, launches=2)
+            gpu.stream_sync()
+            assert np.array_equal(out.to_host(), eager)
+            b.run(out2, gpu.MIXED)  # eager work between replays
+            assert np.array_equal(out2.to_host(), eager)
+
+
+def test_uneven_autotune_keeps_results(gpu, qp, tmp_path):
+    c = uneven_subset(64)
+    ref = uneven_oracle(qp, c)
+    b = make_uneven(gpu, qp, c)
+    cache = str(tmp_path / "tune.txt")
+    print("[case] autotune", flush=True)
+    r1 = b.autotune(gpu.FP64, reps=1, cache_path=cache)
+    assert r1["block"] in (128, 256, 512) and r1["ms"] > 0
+    r2 = b.autotune(gpu.FP64, reps=1, cache_path=cache)  # cache hit
+    assert (r2["block"], r2["tile_evals"], r2["grid"]) == (r1["block"], r1["tile_evals"], r1["grid"])
+    out = gpu.DeviceArray(len(c["p"]))
+    b.run(out, gpu.FP64)
+    assert rel_err(out.to_host(), ref).max() <= TOL_FP64
+
+
+def test_invalid_arguments(gpu, qp):
+    with pytest.raises(gpu.DfrgInvalidArgument):
+        gpu.SEBatch(qp, [1.0], [64], [24], [np.ones(8)], [1e-3], [40.0], [6.0])  # angular order not 2^k
+    with pytest.raises(gpu.DfrgInvalidArgument):
+        gpu.SEBatch(qp, [1.0], [64], [32], [np.ones(300)], [1e-3], [40.0], [6.0])  # table too large
+    c = W.selfenergy(n_p=2)
+    b = selfenergy_batch(gpu, qp, c)
+    with pytest.raises(gpu.DfrgInvalidArgument):
+        b.set_shape(100, 1024, 0)
