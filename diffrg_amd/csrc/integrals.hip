@@ -77,11 +77,14 @@ namespace
         const double den = q2 + k2 * e + m;
         acc += el * (2.0 * e * k2 * (1.0 + xr)) / (den * den);
       } else {
-        const float e = dev::fast_exp2f((float)(-xr * kLog2e));
-        const double den = q2 + k2 * (double)e + m; // denominator in FP64
-        const float g = dev::fast_rcpf((float)den);
-        const float f = 2.0f * e * (float)k2 * (1.0f + (float)xr) * g * g;
-        acc += el * (double)f;
+        // the regulator enters the pole denominator q^2 + R + m2, which cancels as m2 -> -k^2:
+        // R and the denominator stay FP64; reciprocal and products run in FP32
+        // 64-node integrals cannot average FP32 roundings below 1e-8: the reciprocal is an FP32
+        // hardware seed refined by one FP64 Newton step, everything else stays FP64
+        const double e = exp(-xr);
+        const double den = q2 + k2 * e + m;
+        const double g = dev::rcp_seeded(den);
+        acc += el * (2.0 * e * k2 * (1.0 + xr)) * (g * g);
       }
     }
     acc = dev::wave_sum(acc);
@@ -141,26 +144,30 @@ namespace
       const double wn = mx[n];
       const double a1 = s1[j], a2 = s2[j];
       const double dp = wn + p0, dm = -wn + p0;
-      double val;
+      // denominators in FP64 in both modes; mixed replaces the FP64 division sequences by an FP32
+      // hardware reciprocal refined with one FP64 Newton step
+      const double d1 = wn * wn + a1, d2p = dp * dp + a2, d2m = dm * dm + a2;
+      double g1, gp, gm;
       if (MODE == DFRG_FP64) {
-        const double g1 = 1.0 / (wn * wn + a1);
-        const double gp = 1.0 / (dp * dp + a2), gm = 1.0 / (dm * dm + a2);
-        val = mw[n] * (rdot[j] * g1 * g1 * gp + rdot[j] * g1 * g1 * gm);
-        if (T > 0 && n == 0) {
-          const double g10 = 1.0 / a1, g20 = 1.0 / (p0 * p0 + a2);
-          val += T * (rdot[j] * g10 * g10 * g20);
-        }
+        g1 = 1.0 / d1;
+        gp = 1.0 / d2p;
+        gm = 1.0 / d2m;
       } else {
-        // denominators in FP64, reciprocals and products in FP32
-        const float g1 = dev::fast_rcpf((float)(wn * wn + a1));
-        const float gp = dev::fast_rcpf((float)(dp * dp + a2));
-        const float gm = dev::fast_rcpf((float)(dm * dm + a2));
-        const float g11 = g1 * g1;
-        val = mw[n] * (rdot[j] * (double)(g11 * (gp + gm)));
-        if (T > 0 && n == 0) {
-          const float g10 = dev::fast_rcpf((float)a1), g20 = dev::fast_rcpf((float)(p0 * p0 + a2));
-          val += T * (rdot[j] * (double)(g10 * g10 * g20));
+        g1 = dev::rcp_seeded(d1);
+        gp = dev::rcp_seeded(d2p);
+        gm = dev::rcp_seeded(d2m);
+      }
+      double val = mw[n] * (rdot[j] * g1 * g1 * gp + rdot[j] * g1 * g1 * gm);
+      if (T > 0 && n == 0) {
+        double g10, g20;
+        if (MODE == DFRG_FP64) {
+          g10 = 1.0 / a1;
+          g20 = 1.0 / (p0 * p0 + a2);
+        } else {
+          g10 = dev::rcp_seeded(a1);
+          g20 = dev::rcp_seeded(p0 * p0 + a2);
         }
+        val += T * (rdot[j] * g10 * g10 * g20);
       }
       acc += elw[j] * val;
     }
@@ -175,6 +182,11 @@ namespace
     int integral, q0, nq, slot;
   };
 
+  struct SECellZ {
+    double z;
+    float dz, pad;
+  };
+
   struct SEArgs {
     int n_tiles;
     const SETile *tiles;
@@ -184,7 +196,8 @@ namespace
     const double *nodes;
     const int *tab_off, *tab_n;
     const double *tab64;  // FP64 tables
-    const float2 *tab32;  // FP32 (value, slope to next) tables
+    const double2 *tabA;  // mixed-path cell data: (y_i, 1/(b + y_i - c))
+    const SECellZ *tabB;  // mixed-path cell data: (Z_i, Z_{i+1} - Z_i)
     const double *coord;  // per integral: c, inv_b, scale (3 doubles)
     const int *first_slot, *n_slots;
     double x_extent, k, pref; // pref = 2pi/(2pi)^4
@@ -202,13 +215,17 @@ namespace
   template <int MODE, int BLOCK> __global__ __launch_bounds__(BLOCK) void se_kernel(SEArgs A)
   {
     __shared__ double s_tab64[MODE == DFRG_FP64 ? kMaxTab : 1];
-    __shared__ float2 s_tab32[MODE == DFRG_FP64 ? 1 : kMaxTab];
+    __shared__ double2 s_tabA[MODE == DFRG_FP64 ? 1 : kMaxTab];
+    __shared__ SECellZ s_tabB[MODE == DFRG_FP64 ? 1 : kMaxTab];
+    __shared__ double s_exp2[MODE == DFRG_FP64 ? 1 : 64]; // 2^(j/64), j = 0..63
     __shared__ double s_cos1[kMaxAng], s_aw1[kMaxAng], s_w2[kMaxAng];
     __shared__ double s_qt[kMaxQ], s_qs[kMaxQ], s_qc[kMaxQ];
     __shared__ double s_red[BLOCK / 64];
     __shared__ int s_tile, s_cur;
 
     if (threadIdx.x == 0) s_cur = -1;
+    if (MODE != DFRG_FP64)
+      for (int j = threadIdx.x; j < 64; j += BLOCK) s_exp2[j] = exp2(j * (1.0 / 64.0));
     const double k = A.k, k2 = k * k;
     int iter = 0;
     for (;;) {
@@ -236,7 +253,10 @@ namespace
         if (MODE == DFRG_FP64) {
           for (int j = threadIdx.x; j < ntab; j += BLOCK) s_tab64[j] = A.tab64[A.tab_off[i] + j];
         } else {
-          for (int j = threadIdx.x; j < ntab; j += BLOCK) s_tab32[j] = A.tab32[A.tab_off[i] + j];
+          for (int j = threadIdx.x; j < ntab; j += BLOCK) {
+            s_tabA[j] = A.tabA[A.tab_off[i] + j];
+            s_tabB[j] = A.tabB[A.tab_off[i] + j];
+          }
         }
         for (int j = threadIdx.x; j < na; j += BLOCK) {
           const double c1 = 2.0 * (xa[j] - 0.5);
@@ -290,26 +310,48 @@ namespace
           acc += aw * sq;
         }
       } else {
-        const float cf = (float)c, ibf = (float)inv_b, scl = (float)(scale * kLn2);
-        const float ek = (float)(-kLog2e / k2), k2f = (float)k2;
-        const float tmax = (float)(ntab - 1);
+        // Mixed: FP32 hardware seeds replace the FP64 library sequences; FP64 is kept where the
+        // error budget needs it.  |p-q|^2 in FP64 (cancellation); sqrt from an FP32 seed and one
+        // FP64 Newton step; Z cell found with the FP32 log2, the in-cell position s*log1p(u)
+        // (|u| <~ 0.2) in FP32 with the rounding of 1+u restored, Z assembled in FP64 from the
+        // FP64 node value; R = k^2 2^t with t split on a 1/64 grid (table) and a degree-4
+        // polynomial remainder in FP64; the denominator in FP64; the reciprocal from an FP32 seed
+        // refined in FP64.  Per-evaluation error ~1e-9 relative.
+        const float cf = (float)c, ibf = (float)inv_b, scl = (float)(scale * kLn2), sf = (float)scale;
+        const double ek64 = -64.0 * kLog2e / k2; // t * 64
+        const int imax = ntab - 2;
         for (int pr = pr0; pr < npairs; pr += pstride) {
           const int i1 = pr >> l2, i2 = pr & (na - 1);
           const double c1 = s_cos1[i1];
           const double aw = s_aw1[i1] * s_w2[i2];
           double sq = 0.0;
           for (int j = qs0; j < nqt; j += qlanes) {
-            const double pq2d = fma(s_qc[j], c1, s_qs[j]); // |p-q|^2 in FP64 (cancellation)
-            const float pq2 = (float)pq2d;
-            const float y = dev::fast_sqrtf(pq2);
-            float idx = __builtin_amdgcn_logf(fmaf(y - cf, ibf, 1.0f)) * scl;
-            idx = fmaxf(0.0f, fminf(idx, tmax)); // NaN-safe clamp: a NaN index maps to tmax
-            const float fl = __builtin_floorf(idx);
-            const float2 zz = s_tab32[(int)fl];
-            const float Z = fmaf(idx - fl, zz.y, zz.x);
-            const float R = k2f * dev::fast_exp2f(pq2 * ek);
-            const float G = dev::fast_rcpf(fmaf(Z, pq2, R));
-            sq = fma(s_qt[j], (double)G, sq);
+            const double pq2d = fma(s_qc[j], c1, s_qs[j]);
+            const float ys = __builtin_amdgcn_sqrtf((float)pq2d);
+            const double ysd = (double)ys;
+            const double yd = fma(fma(-ysd, ysd, pq2d), (double)(0.5f * __builtin_amdgcn_rcpf(ys)), ysd);
+            const float coarse = __builtin_amdgcn_logf(fmaf(ys - cf, ibf, 1.0f)) * scl;
+            const int i0 = min(max((int)__builtin_floorf(coarse), 0), imax);
+            const double2 ca = s_tabA[i0];
+            const SECellZ cz = s_tabB[i0];
+            const float u = (float)((yd - ca.x) * ca.y);
+            const float w1 = 1.0f + u;
+            const float ru = u - (w1 - 1.0f); // rounding error of 1 + u
+            float fr = fmaf(__builtin_amdgcn_logf(w1), scl, ru * sf);
+            fr = fmaxf(0.0f, fminf(fr, 1.0f)); // NaN-safe; realises the [0, n-1] clamp
+            const double Z = cz.z + (double)(fr * cz.dz);
+            // R = k^2 2^(t), t = -pq2/k^2 log2(e) <= 0
+            const double t64 = pq2d * ek64;
+            double R = 0.0;
+            if (t64 > -64.0 * 1000.0) {
+              const double tn = rint(t64);
+              const int jn = (int)tn;
+              const double r = (t64 - tn) * (kLn2 / 64.0);
+              const double pe = fma(fma(fma(fma(r, 1.0 / 24.0, 1.0 / 6.0), r, 0.5), r, 1.0), r, 1.0);
+              R = k2 * ldexp(pe * s_exp2[jn & 63], jn >> 6);
+            }
+            const double G = dev::rcp_seeded(fma(Z, pq2d, R));
+            sq = fma(s_qt[j], G, sq);
           }
           acc += aw * sq;
         }
@@ -364,7 +406,8 @@ struct dfrg_se_batch {
   unsigned long long evals = 0;
   // device data
   double *d_p = nullptr, *d_nodes = nullptr, *d_tab64 = nullptr, *d_coord = nullptr;
-  float2 *d_tab32 = nullptr;
+  double2 *d_tabA = nullptr;
+  SECellZ *d_tabB = nullptr;
   int *d_ints = nullptr; // nq, na, log2na, xq_off, xa_off, tab_off, tab_n, first_slot, n_slots
   SETile *d_tiles = nullptr;
   double *d_partials = nullptr;
@@ -439,7 +482,8 @@ namespace
     a.n_slots = b->d_ints + 8 * n;
     a.nodes = b->d_nodes;
     a.tab64 = b->d_tab64;
-    a.tab32 = b->d_tab32;
+    a.tabA = b->d_tabA;
+    a.tabB = b->d_tabB;
     a.coord = b->d_coord;
     a.x_extent = b->x_extent;
     a.k = b->k;
@@ -590,13 +634,19 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
     coord[3 * i + 2] = (tab_n[i] - 1.0) / bias[i];
     b->evals += (unsigned long long)nq[i] * na[i] * na[i];
   }
-  // FP32 tables as (value, slope to the next point); the last point has slope 0
-  std::vector<float2> tab32(tables_len);
+  // cell data of the mixed path (coordinates.hh:141-196 LogarithmicCoordinates1D): node
+  // y_i = b expm1(a i/(n-1)) + c, 1/(b + y_i - c), Z_i and Z_{i+1} - Z_i
+  std::vector<double2> tabA(tables_len);
+  std::vector<SECellZ> tabB(tables_len);
   for (int i = 0; i < n_int; ++i) {
-    for (int j = 0; j < tab_n[i]; ++j) {
+    const double a = bias[i], bb = (stop[i] - start[i]) / std::expm1(a), cc = start[i];
+    const int n = tab_n[i];
+    for (int j = 0; j < n; ++j) {
+      const double yj = bb * std::expm1(a * j * (1.0 / (n - 1.0))) + cc;
       const double v = tables[tab_off[i] + j];
-      const double nv = (j + 1 < tab_n[i]) ? tables[tab_off[i] + j + 1] : v;
-      tab32[tab_off[i] + j] = make_float2((float)v, (float)(nv - v));
+      const double nv = (j + 1 < n) ? tables[tab_off[i] + j + 1] : v;
+      tabA[tab_off[i] + j] = make_double2(yj, 1.0 / (bb + yj - cc));
+      tabB[tab_off[i] + j] = SECellZ{v, (float)(nv - v), 0.0f};
     }
   }
   auto fail = [&](int rc) {
@@ -614,7 +664,8 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
   DFRG_HIP_B(hipMalloc((void **)&b->d_p, sizeof(double) * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_nodes, sizeof(double) * nodes.size()));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tab64, sizeof(double) * tables_len));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_tab32, sizeof(float2) * tables_len));
+  DFRG_HIP_B(hipMalloc((void **)&b->d_tabA, sizeof(double2) * tables_len));
+  DFRG_HIP_B(hipMalloc((void **)&b->d_tabB, sizeof(SECellZ) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_coord, sizeof(double) * 3 * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_ints, sizeof(int) * 9 * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_counters, sizeof(unsigned) * n_int));
@@ -622,7 +673,8 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
   DFRG_HIP_B(hipMemcpy(b->d_p, p, sizeof(double) * n_int, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tab64, tables, sizeof(double) * tables_len, hipMemcpyHostToDevice));
-  DFRG_HIP_B(hipMemcpy(b->d_tab32, tab32.data(), sizeof(float2) * tables_len, hipMemcpyHostToDevice));
+  DFRG_HIP_B(hipMemcpy(b->d_tabA, tabA.data(), sizeof(double2) * tables_len, hipMemcpyHostToDevice));
+  DFRG_HIP_B(hipMemcpy(b->d_tabB, tabB.data(), sizeof(SECellZ) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_coord, coord.data(), sizeof(double) * 3 * n_int, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_ints, ints.data(), sizeof(int) * 9 * n_int, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemset(b->d_counters, 0, sizeof(unsigned) * n_int));
@@ -654,7 +706,8 @@ int dfrg_se_batch_destroy(dfrg_se_batch *b)
   hipFree(b->d_p);
   hipFree(b->d_nodes);
   hipFree(b->d_tab64);
-  hipFree(b->d_tab32);
+  hipFree(b->d_tabA);
+  hipFree(b->d_tabB);
   hipFree(b->d_coord);
   hipFree(b->d_ints);
   hipFree(b->d_tiles);

@@ -13,6 +13,10 @@
 #include "internal.hh"
 #include "dfrg/device.hh"
 
+// The double-double arithmetic below relies on error-free transformations; fusing a product
+// into a later sum (FP contraction, on by default for device code) would break them.
+#pragma clang fp contract(off)
+
 #include <cmath>
 #include <map>
 #include <vector>

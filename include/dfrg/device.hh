@@ -139,7 +139,22 @@ namespace dfrg
     }
 
     __device__ __forceinline__ float fast_exp2f(float v) { return __builtin_amdgcn_exp2f(v); }
+    // Reciprocal with one Newton step: the hardware estimate is biased by about -0.06 ulp;
+    // after the step the error is an unbiased rounding (<= 1 ulp).
+    __device__ __forceinline__ float rcp_refined(float d)
+    {
+      float r = __builtin_amdgcn_rcpf(d);
+      const float e = fmaf(-d, r, 1.0f);
+      return fmaf(r, e, r);
+    }
     __device__ __forceinline__ float fast_rcpf(float v) { return __builtin_amdgcn_rcpf(v); }
+    // FP64 reciprocal from an FP32 hardware seed and one FP64 Newton step (relative error about
+    // (2^-23)^2, i.e. a few FP64 ulp; d must be within the FP32 range).
+    __device__ __forceinline__ double rcp_seeded(double d)
+    {
+      const double r = (double)__builtin_amdgcn_rcpf((float)d);
+      return fma(r, fma(-d, r, 1.0), r);
+    }
     __device__ __forceinline__ float fast_sqrtf(float v) { return __builtin_amdgcn_sqrtf(v); }
   } // namespace dev
 } // namespace dfrg

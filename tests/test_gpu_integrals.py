@@ -103,7 +103,8 @@ def finite_T_device(gpu, c, qp):
     temps = sorted(c["rules"])
     mx = np.concatenate([c["rules"][t][0] for t in temps])
     mw = np.concatenate([c["rules"][t][1] for t in temps])
-    off = {t: i * len(c["rules"][t][0]) for i, t in enumerate(temps)}
+    starts = np.concatenate([[0], np.cumsum([len(c["rules"][t][0]) for t in temps])])
+    off = {t: int(starts[i]) for i, t in enumerate(temps)}
     m_off = np.array([off[t] for t in c["T"]], np.int32)
     m_n = np.array([len(c["rules"][t][0]) for t in c["T"]], np.int32)
     D = gpu.DeviceArray.from_host
