@@ -57,6 +57,26 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+KAT_SRC = os.path.join(ROOT, "tests", "native", "kat_kernels.hip")
+KAT_LIB = os.path.join(ROOT, "tests", "native", "libdfrg_kat.so")
+
+
+def build_kat(verbose: bool = True) -> str:
+    """Compile the test library that instantiates the C++ integrator templates (tests/native)."""
+    tmp = KAT_LIB + ".tmp"
+    cmd = [HIPCC, *COMMON_FLAGS, "-fopenmp", "-shared", KAT_SRC, "-o", tmp, f"-L{PKG}", "-ldfrg",
+           "-Wl,-rpath,$ORIGIN/../../diffrg_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {KAT_SRC}:\n{msg}")
+    os.replace(tmp, KAT_LIB)
+    if verbose:
+        print(f"built {KAT_LIB}")
+    return KAT_LIB
+
+
 if __name__ == "__main__":
     build()
+    build_kat()
     sys.exit(0)

@@ -1,0 +1,21 @@
+// DiFfRG/common/types.hh -- get_type::ctype<NT>: the real type of a number type
+// (reference DiFfRG/include/DiFfRG/common/types.hh:130-167).
+#pragma once
+
+#include <DiFfRG/common/complex_math.hh>
+
+namespace DiFfRG
+{
+  namespace get_type
+  {
+    namespace internal
+    {
+      template <typename CT> struct _ctype;
+      template <> struct _ctype<float> { using value = float; };
+      template <> struct _ctype<double> { using value = double; };
+      template <> struct _ctype<complex<float>> { using value = float; };
+      template <> struct _ctype<complex<double>> { using value = double; };
+    } // namespace internal
+    template <typename CT> using ctype = typename internal::_ctype<CT>::value;
+  } // namespace get_type
+} // namespace DiFfRG

@@ -1,0 +1,71 @@
+// DiFfRG/physics/integration/integrator_4D_gpu.hh -- Integrator4DGPU<NT, KERNEL>
+// (reference integrator_4D_gpu.hh:62-352):
+//   int dq dcos1 dcos2 dphi (2pi)^-4 q^2/2 k^2 sqrt(1 - cos1^2) KERNEL::kernel(q, cos1, cos2, phi, k, t...) + constant
+// with q^2 in [0, x_extent k^2].  One launch per call; the three angular orders must be equal.
+#pragma once
+
+#include <array>
+#include <future>
+#include <stdexcept>
+
+#include <DiFfRG/common/json.hh>
+#include <DiFfRG/common/math.hh>
+#include <DiFfRG/common/quadrature/quadrature_provider.hh>
+#include <DiFfRG/common/types.hh>
+
+#include "dfrg/integrate.hh"
+
+namespace DiFfRG
+{
+  template <typename NT, typename KERNEL> class Integrator4DGPU
+  {
+  public:
+    using ctype = typename get_type::ctype<NT>;
+
+    Integrator4DGPU(QuadratureProvider &quadrature_provider, const std::array<uint, 4> grid_sizes, const ctype x_extent,
+                    const JSONValue &json)
+        : Integrator4DGPU(quadrature_provider, grid_sizes, x_extent,
+                          json.get_uint("/integration/cudathreadsperblock", 256))
+    {
+    }
+    Integrator4DGPU(QuadratureProvider &quadrature_provider, std::array<uint, 4> grid_sizes, const ctype x_extent,
+                    const uint max_block_size = 256)
+        : grid_sizes(grid_sizes), x_extent(x_extent)
+    {
+      (void)max_block_size;
+      if (grid_sizes[2] != grid_sizes[1] || grid_sizes[3] != grid_sizes[1])
+        throw std::runtime_error("Grid sizes must be currently equal for all angular dimensions!");
+      ptr_x_quadrature_p = quadrature_provider.get_device_points<ctype>(grid_sizes[0]);
+      ptr_x_quadrature_w = quadrature_provider.get_device_weights<ctype>(grid_sizes[0]);
+      ptr_ang_quadrature_p = quadrature_provider.get_device_points<ctype>(grid_sizes[1]);
+      ptr_ang_quadrature_w = quadrature_provider.get_device_weights<ctype>(grid_sizes[1]);
+    }
+
+    template <typename... T> NT get(const ctype k, const T &...t) const { return launch(k, t...).wait(); }
+
+    template <typename... T> std::future<NT> request(const ctype k, const T &...t) const
+    {
+      auto p = launch(k, t...);
+      return std::async(std::launch::deferred, [p]() mutable { return p.wait(); });
+    }
+
+  private:
+    template <typename... T> dfrg::tmpl::Pending<NT> launch(const ctype k, const T &...t) const
+    {
+      auto p = dfrg::tmpl::Pending<NT>::make();
+      p.constant = KERNEL::constant(k, t...);
+      const long long na = grid_sizes[1];
+      const long long nodes = (long long)grid_sizes[0] * na * na * na;
+      hipLaunchKernelGGL((dfrg::tmpl::angle3_kernel<ctype, NT, KERNEL, T...>), dim3(dfrg::tmpl::grid_for(nodes)),
+                         dim3(dfrg::tmpl::kBlock), 0, p.stream, ptr_x_quadrature_p, ptr_x_quadrature_w,
+                         ptr_ang_quadrature_p, ptr_ang_quadrature_w, (int)grid_sizes[0], (int)grid_sizes[1], x_extent,
+                         k, p.partials(), p.counter(), p.result(), t...);
+      p.enqueue_result();
+      return p;
+    }
+
+    std::array<uint, 4> grid_sizes;
+    ctype x_extent;
+    const ctype *ptr_x_quadrature_p, *ptr_x_quadrature_w, *ptr_ang_quadrature_p, *ptr_ang_quadrature_w;
+  };
+} // namespace DiFfRG

@@ -1,0 +1,256 @@
+// dfrg/integrate.hh -- generic gfx950 kernels behind the DiFfRG-compatible integrator templates.
+//
+// A user KERNEL (static __host__ __device__ kernel()/constant()) is device code of the caller's
+// translation unit, so these kernels are templates compiled there by hipcc.  Each call of get()
+// or request() is one launch on the calling thread's stream: every lane evaluates a strided set
+// of quadrature nodes (element * weight * KERNEL::kernel), lanes are reduced with a wave64 DPP
+// tree, waves in LDS, and blocks through per-block partials whose last-arriving block sums them
+// in block order.  The result (one NT) is copied into pinned host memory on the same stream.
+// Per-call scratch comes from libdfrg.so's caching pools, so no call allocates after warm-up.
+//
+// Summation order is fixed by the grid (a function of the node count only), so a result is
+// bitwise reproducible run to run.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <future>
+#include <type_traits>
+
+#include <DiFfRG/common/complex_math.hh>
+#include <DiFfRG/common/hip_prefix.hh>
+#include <DiFfRG/common/math.hh>
+
+#include "dfrg/device.hh"
+
+namespace dfrg
+{
+  namespace tmpl
+  {
+    constexpr int kBlock = 256;
+    constexpr int kMaxBlocks = 1024;
+    constexpr int kNodesPerLane = 4;
+
+    // ------------------------------------------------------------------------------------------
+    // reductions over the number type NT (float, double, DiFfRG::complex<float|double>)
+    // ------------------------------------------------------------------------------------------
+    __device__ __forceinline__ float wave_sum_f32(float v)
+    {
+      v += dev::dpp_f32<0xB1>(v);
+      v += dev::dpp_f32<0x4E>(v);
+      v += dev::dpp_f32<0x141>(v);
+      v += dev::dpp_f32<0x140>(v);
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      return v;
+    }
+    template <class NT> __device__ __forceinline__ NT wave_reduce(NT v)
+    {
+      if constexpr (std::is_same_v<NT, double>)
+        return dev::wave_sum(v);
+      else if constexpr (std::is_same_v<NT, float>)
+        return wave_sum_f32(v);
+      else
+        return NT(wave_reduce(v.re), wave_reduce(v.im));
+    }
+
+    template <class NT> __device__ __forceinline__ NT block_reduce(NT v)
+    {
+      // raw storage: NT may have a user-provided constructor (complex)
+      __shared__ __attribute__((aligned(16))) unsigned char smem_raw[sizeof(NT) * (kBlock / 64)];
+      NT *smem = reinterpret_cast<NT *>(smem_raw);
+      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+      v = wave_reduce(v);
+      if (lane == 0) smem[wid] = v;
+      __syncthreads();
+      NT t = smem[0];
+      for (int i = 1; i < kBlock / 64; ++i) t += smem[i];
+      return t;
+    }
+
+    // Publish this block's partial; the last block adds all partials in block order and writes
+    // *result (then resets the counter for the next use of the scratch block).
+    template <class NT> __device__ __forceinline__ void finish_blocks(NT part, NT *partials, unsigned *counter,
+                                                                      NT *result)
+    {
+      __shared__ bool last;
+      if (threadIdx.x == 0) {
+        partials[blockIdx.x] = part;
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+      }
+      __syncthreads();
+      if (last && threadIdx.x == 0) {
+        __threadfence();
+        // the acquire fence above orders these loads after every block's published partial
+        NT s = partials[0];
+        for (unsigned i = 1; i < gridDim.x; ++i) s += partials[i];
+        *result = s;
+        atomicExch(counter, 0u);
+      }
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // node kernels
+    // ------------------------------------------------------------------------------------------
+    // IntegratorGPU<d> (integrator_gpu.hh:34-60)
+    template <int d, class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void radial_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                            int nx, ctype x_extent, ctype k, NT *partials,
+                                                            unsigned *counter, NT *result, T... t)
+    {
+      NT acc = NT(0);
+      constexpr ctype S_d = DiFfRG::S_d_prec<ctype>(d);
+      for (int i = blockIdx.x * kBlock + threadIdx.x; i < nx; i += gridDim.x * kBlock) {
+        const ctype weight = wq[i] * x_extent;
+        const ctype q = k * sqrt(xq[i] * x_extent);
+        const ctype int_element = S_d * (DiFfRG::powr<d - 2>(q) / 2 * DiFfRG::powr<2>(k)) /
+                                  DiFfRG::powr<d>(2 * (ctype)M_PI);
+        const NT res = KERNEL::kernel(q, k, t...);
+        acc += int_element * res * weight;
+      }
+      finish_blocks(block_reduce(acc), partials, counter, result);
+    }
+
+    // Integrator4D2AngGPU (integrator_4D_2ang_gpu.hh:20-45): node (q, cos1, cos2), cos2 fastest
+    template <class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void angle2_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
+                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
+                                                            unsigned *counter, NT *result, T... t)
+    {
+      NT acc = NT(0);
+      const long long W = (long long)nx * na * na;
+      for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < W; i += (long long)gridDim.x * kBlock) {
+        const int i2 = (int)(i % na);
+        const long long r = i / na;
+        const int i1 = (int)(r % na);
+        const int ix = (int)(r / na);
+        const ctype q = k * sqrt(xq[ix] * x_extent);
+        const ctype cos1 = 2 * (xa[i1] - (ctype)0.5);
+        const ctype int_element = (DiFfRG::powr<2>(q) * (ctype)0.5 * DiFfRG::powr<2>(k)) // x = p^2 / k^2 integral
+                                  * sqrt(1. - DiFfRG::powr<2>(cos1))                      // cos1 Jacobian
+                                  * 2 * (ctype)M_PI                                       // phi integral
+                                  / DiFfRG::powr<4>(2 * (ctype)M_PI);                     // Fourier factor
+        const ctype weight = 2 * wa[i1] * wq[ix] * x_extent * 2 * wa[i2];
+        const ctype cos2 = 2 * (xa[i2] - (ctype)0.5);
+        acc += KERNEL::kernel(q, cos1, cos2, k, t...) * int_element * weight;
+      }
+      finish_blocks(block_reduce(acc), partials, counter, result);
+    }
+
+    // Integrator4DGPU (integrator_4D_gpu.hh:20-49): node (q, cos1, phi, cos2), cos2 fastest
+    template <class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void angle3_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
+                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
+                                                            unsigned *counter, NT *result, T... t)
+    {
+      NT acc = NT(0);
+      const long long W = (long long)nx * na * na * na;
+      for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < W; i += (long long)gridDim.x * kBlock) {
+        const int i2 = (int)(i % na); // cos2
+        long long r = i / na;
+        const int i3 = (int)(r % na); // phi
+        r /= na;
+        const int i1 = (int)(r % na); // cos1
+        const int ix = (int)(r / na);
+        const ctype q = k * sqrt(xq[ix] * x_extent);
+        const ctype cos1 = 2 * (xa[i1] - (ctype)0.5);
+        const ctype phi = 2 * (ctype)M_PI * xa[i3];
+        const ctype int_element = (DiFfRG::powr<2>(q) * (ctype)0.5 * DiFfRG::powr<2>(k)) *
+                                  sqrt(1. - DiFfRG::powr<2>(cos1)) / DiFfRG::powr<4>(2 * (ctype)M_PI);
+        const ctype weight = 2 * (ctype)M_PI * wa[i3] * 2 * wa[i1] * wq[ix] * x_extent;
+        const ctype cos2 = 2 * (xa[i2] - (ctype)0.5);
+        acc += KERNEL::kernel(q, cos1, cos2, phi, k, t...) * int_element * weight * 2 * wa[i2];
+      }
+      finish_blocks(block_reduce(acc), partials, counter, result);
+    }
+
+    // IntegratorFiniteTq0GPU<d> (integrator_finiteTq0_gpu.hh:20-44): node (q, Matsubara n)
+    template <int d, class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void finiteTq0_kernel(const ctype *__restrict__ xq,
+                                                               const ctype *__restrict__ wq,
+                                                               const ctype *__restrict__ mx,
+                                                               const ctype *__restrict__ mw, int nx, int nm,
+                                                               ctype x_extent, ctype m_T, ctype k, NT *partials,
+                                                               unsigned *counter, NT *result, T... t)
+    {
+      NT acc = NT(0);
+      constexpr ctype S_dm1 = DiFfRG::S_d_prec<ctype>(d - 1);
+      const long long W = (long long)nx * nm;
+      for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < W; i += (long long)gridDim.x * kBlock) {
+        const int iy = (int)(i % nm);
+        const int ix = (int)(i / nm);
+        const ctype q = k * sqrt(xq[ix] * x_extent);
+        const ctype x_weight = wq[ix] * x_extent;
+        const ctype int_element = S_dm1 * (DiFfRG::powr<d - 3>(q) / (ctype)2 * DiFfRG::powr<2>(k)) /
+                                  DiFfRG::powr<d - 1>(2 * (ctype)M_PI);
+        const ctype q0 = mx[iy];
+        const ctype weight = x_weight * mw[iy];
+        NT res = int_element * weight * (KERNEL::kernel(q, q0, k, t...) + KERNEL::kernel(q, -q0, k, t...));
+        if (m_T > 0 && iy == 0) res += int_element * x_weight * m_T * KERNEL::kernel(q, (ctype)0, k, t...);
+        acc += res;
+      }
+      finish_blocks(block_reduce(acc), partials, counter, result);
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // host side: per-call scratch, launch and result
+    // ------------------------------------------------------------------------------------------
+    inline int grid_for(long long nodes)
+    {
+      const long long per_block = (long long)kBlock * kNodesPerLane;
+      long long g = (nodes + per_block - 1) / per_block;
+      if (g < 1) g = 1;
+      if (g > kMaxBlocks) g = kMaxBlocks;
+      return (int)g;
+    }
+
+    // One in-flight evaluation: scratch (partials | counter | result) in HBM and a pinned result
+    // slot, enqueued on `stream`.
+    template <class NT> struct Pending {
+      hipStream_t stream = nullptr;
+      void *d_scratch = nullptr;
+      void *h_result = nullptr;
+      NT constant = NT(0);
+
+      static constexpr size_t counter_offset = ((kMaxBlocks * sizeof(NT) + 255) / 256) * 256;
+      static constexpr size_t result_offset = counter_offset + 256;
+      static constexpr size_t bytes = result_offset + 256;
+
+      NT *partials() const { return (NT *)d_scratch; }
+      unsigned *counter() const { return (unsigned *)((char *)d_scratch + counter_offset); }
+      NT *result() const { return (NT *)((char *)d_scratch + result_offset); }
+
+      static Pending make()
+      {
+        Pending p;
+        void *s = nullptr;
+        DiFfRG::dfrg_check(dfrg_thread_stream(&s), "thread stream");
+        p.stream = (hipStream_t)s;
+        DiFfRG::dfrg_check(dfrg_pool_alloc(&p.d_scratch, bytes), "scratch");
+        DiFfRG::dfrg_check(dfrg_pool_host_alloc(&p.h_result, sizeof(NT)), "result slot");
+        return p;
+      }
+      // after the kernel launch: copy the result back on the same stream
+      void enqueue_result()
+      {
+        DiFfRG::check_hip("integrator kernel launch");
+        if (hipMemcpyAsync(h_result, result(), sizeof(NT), hipMemcpyDeviceToHost, stream) != hipSuccess)
+          throw std::runtime_error("integrator: result copy failed");
+      }
+      NT wait()
+      {
+        if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("integrator: stream error");
+        NT v;
+        std::memcpy((void *)&v, h_result, sizeof(NT));
+        dfrg_pool_free(d_scratch);
+        dfrg_pool_host_free(h_result);
+        return constant + v;
+      }
+    };
+  } // namespace tmpl
+} // namespace dfrg

@@ -1,0 +1,353 @@
+// kat_kernels.hip -- the DiFfRG-compatible integrator templates instantiated with user KERNEL
+// structs, exported through extern "C" for the GPU tests: the polynomial integrands of the
+// reference tests (tests/physics/integration/integrator_*_cpu.cc, integration_finiteT/
+// integrator_finiteTq0_cpu.cc) and the three built-in integrands written as ordinary KERNELs
+// with regulators and LinearInterpolator1D.  Each function returns 0, or -1 (invalid argument) /
+// -2 (runtime error) with the exception message in kat_last_error().
+#include <DiFfRG/common/math.hh>
+#include <DiFfRG/common/quadrature/quadrature_provider.hh>
+#include <DiFfRG/discretization/grid/coordinates.hh>
+#include <DiFfRG/physics/integration/integrator_4D_2ang_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_4D_2ang_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_4D_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_4D_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_gpu.hh>
+#include <DiFfRG/physics/interpolation/linear_interpolation_1D.hh>
+#include <DiFfRG/physics/regulators.hh>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace DiFfRG;
+
+namespace
+{
+  std::string g_err;
+  QuadratureProvider &provider()
+  {
+    static std::unique_ptr<QuadratureProvider> qp;
+    if (!qp) qp = std::make_unique<QuadratureProvider>();
+    return *qp;
+  }
+  template <class F> int guard(F &&f)
+  {
+    try {
+      f();
+      return 0;
+    } catch (const std::invalid_argument &e) {
+      g_err = e.what();
+      return -1;
+    } catch (const std::exception &e) {
+      g_err = e.what();
+      return -2;
+    }
+  }
+
+  // integrator_cpu.cc:15-32
+  struct PolyRadial {
+    static __forceinline__ __host__ __device__ auto kernel(const double q, const double, const double, const double x0,
+                                                           const double x1, const double x2, const double x3,
+                                                           const double x4, const double x5)
+    {
+      return x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q);
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+  // complex-valued variant: i * poly(q) + poly(q)
+  struct PolyRadialC {
+    static __forceinline__ __host__ __device__ complex<double> kernel(const double q, const double k, const double c,
+                                                                      const double x0, const double x1, const double x2,
+                                                                      const double x3, const double x4,
+                                                                      const double x5)
+    {
+      const double v = PolyRadial::kernel(q, k, c, x0, x1, x2, x3, x4, x5);
+      return complex<double>(v, 2.0 * v);
+    }
+    static __forceinline__ __host__ __device__ complex<double> constant(const double, const double c, const double,
+                                                                        const double, const double, const double,
+                                                                        const double, const double)
+    {
+      return complex<double>(c, 0.0);
+    }
+  };
+  // integrator_4D_2ang_cpu.cc:14-36
+  struct Poly2Ang {
+    static __forceinline__ __host__ __device__ auto kernel(const double q, const double cos1, const double cos2,
+                                                           const double, const double, const double x0, const double x1,
+                                                           const double x2, const double x3, const double x4,
+                                                           const double x5, const double a0, const double a1,
+                                                           const double a2, const double a3, const double b0,
+                                                           const double b1, const double b2, const double b3)
+    {
+      return (x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q)) *
+             (a0 + a1 * powr<1>(cos1) + a2 * powr<2>(cos1) + a3 * powr<3>(cos1)) *
+             (b0 + b1 * powr<1>(cos2) + b2 * powr<2>(cos2) + b3 * powr<3>(cos2));
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+  // integrator_4D_cpu.cc:14-39
+  struct Poly3Ang {
+    static __forceinline__ __host__ __device__ auto
+    kernel(const double q, const double cos1, const double cos2, const double phi, const double, const double,
+           const double x0, const double x1, const double x2, const double x3, const double x4, const double x5,
+           const double a0, const double a1, const double a2, const double a3, const double b0, const double b1,
+           const double b2, const double b3, const double p0, const double p1, const double p2, const double p3)
+    {
+      return (x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q)) *
+             (a0 + a1 * powr<1>(cos1) + a2 * powr<2>(cos1) + a3 * powr<3>(cos1)) *
+             (b0 + b1 * powr<1>(cos2) + b2 * powr<2>(cos2) + b3 * powr<3>(cos2)) *
+             (p0 + p1 * powr<1>(phi) + p2 * powr<2>(phi) + p3 * powr<3>(phi));
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+  // integrator_finiteTq0_cpu.cc:14-34
+  struct PolyFiniteT {
+    static __forceinline__ __host__ __device__ auto kernel(const double q, const double q0, const double,
+                                                           const double, const double x0, const double x1,
+                                                           const double x2, const double x3, const double x4,
+                                                           const double x5, const double d0, const double d1,
+                                                           const double d2, const double d3)
+    {
+      return (x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q)) /
+             (d0 + d1 * powr<1>(q0) + d2 * powr<2>(q0) + d3 * powr<3>(q0));
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+
+  struct B1 {
+    static constexpr double b = 1.;
+  };
+  using Reg = ExponentialRegulator<B1>;
+  using ZInterp = LinearInterpolator1D<double, LogarithmicCoordinates1D<double>>;
+
+  // LPA threshold integrand RBdot / (q^2 + RB + m2)^2
+  struct ThresholdKernel {
+    static __forceinline__ __host__ __device__ double kernel(const double q, const double k, const double m2)
+    {
+      const double k2 = powr<2>(k), q2 = powr<2>(q);
+      return Reg::RBdot(k2, q2) / powr<2>(q2 + Reg::RB(k2, q2) + m2);
+    }
+    static __forceinline__ __host__ __device__ double constant(const double, const double) { return 0.; }
+  };
+  // self-energy RBdot(q) G(q)^2 G(|p-q|), G(y) = 1/(Z(y) y^2 + RB(y^2))
+  struct SelfEnergyKernel {
+    static __forceinline__ __host__ __device__ double G(const double k2, const double y2, const ZInterp &Z)
+    {
+      return 1.0 / (Z(sqrt(y2)) * y2 + Reg::RB(k2, y2));
+    }
+    static __forceinline__ __host__ __device__ double kernel(const double q, const double cos1, const double,
+                                                             const double k, const double p, const ZInterp &Z)
+    {
+      const double k2 = powr<2>(k), q2 = powr<2>(q);
+      const double Gq = G(k2, q2, Z);
+      return Reg::RBdot(k2, q2) * Gq * Gq * G(k2, powr<2>(p) + q2 - 2. * p * q * cos1, Z);
+    }
+    static __forceinline__ __host__ __device__ double constant(const double, const double, const ZInterp &) { return 0.; }
+  };
+  // finite-T two-propagator integrand
+  struct FiniteTKernel {
+    static __forceinline__ __host__ __device__ double G(const double k2, const double w, const double s, const double m)
+    {
+      return 1.0 / (powr<2>(w) + s + Reg::RB(k2, s) + m);
+    }
+    static __forceinline__ __host__ __device__ double kernel(const double q, const double q0, const double k,
+                                                             const double p0, const double pabs, const double m1,
+                                                             const double m2)
+    {
+      const double k2 = powr<2>(k), q2 = powr<2>(q);
+      const double g1 = G(k2, q0, q2, m1);
+      return Reg::RBdot(k2, q2) * g1 * g1 * G(k2, q0 + p0, q2 + powr<2>(pabs), m2);
+    }
+    static __forceinline__ __host__ __device__ double constant(const double, const double, const double, const double,
+                                                               const double)
+    {
+      return 0.;
+    }
+  };
+} // namespace
+
+extern "C" {
+
+const char *kat_last_error() { return g_err.c_str(); }
+
+// device = 1: GPU template, 0: TBB template
+int kat_radial(int device, int d, int nx, double x_extent, double k, const double *c, double constant, double *out)
+{
+  return guard([&] {
+    auto run = [&](auto integ) {
+      *out = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5]);
+    };
+    switch (d * 2 + device) {
+    case 2: run(IntegratorTBB<1, double, PolyRadial>(provider(), nx, x_extent)); break;
+    case 3: run(IntegratorGPU<1, double, PolyRadial>(provider(), nx, x_extent)); break;
+    case 4: run(IntegratorTBB<2, double, PolyRadial>(provider(), nx, x_extent)); break;
+    case 5: run(IntegratorGPU<2, double, PolyRadial>(provider(), nx, x_extent)); break;
+    case 6: run(IntegratorTBB<3, double, PolyRadial>(provider(), nx, x_extent)); break;
+    case 7: run(IntegratorGPU<3, double, PolyRadial>(provider(), nx, x_extent)); break;
+    case 8: run(IntegratorTBB<4, double, PolyRadial>(provider(), nx, x_extent)); break;
+    case 9: run(IntegratorGPU<4, double, PolyRadial>(provider(), nx, x_extent)); break;
+    default: throw std::invalid_argument("unsupported dimension");
+    }
+  });
+}
+
+int kat_radial_complex(int nx, double x_extent, double k, const double *c, double constant, double *out_re,
+                       double *out_im)
+{
+  return guard([&] {
+    IntegratorGPU<4, complex<double>, PolyRadialC> integ(provider(), nx, x_extent);
+    const auto v = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5]);
+    *out_re = v.re;
+    *out_im = v.im;
+  });
+}
+
+// n_req futures issued before any is waited on (request/update_data pattern)
+int kat_radial_requests(int nx, double x_extent, const double *ks, int n_req, const double *c, double *out)
+{
+  return guard([&] {
+    IntegratorGPU<4, double, PolyRadial> integ(provider(), nx, x_extent);
+    std::vector<std::future<double>> f;
+    for (int i = 0; i < n_req; ++i) f.push_back(integ.request(ks[i], 0.0, c[0], c[1], c[2], c[3], c[4], c[5]));
+    for (int i = 0; i < n_req; ++i) out[i] = f[i].get();
+  });
+}
+
+int kat_2ang(int device, int nx, int na1, int na2, double x_extent, double k, const double *c, const double *a,
+             const double *b, double constant, double *out)
+{
+  return guard([&] {
+    auto run = [&](auto integ) {
+      *out = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], b[0], b[1], b[2],
+                       b[3]);
+    };
+    const std::array<uint, 3> g{{(uint)nx, (uint)na1, (uint)na2}};
+    if (device)
+      run(Integrator4D2AngGPU<double, Poly2Ang>(provider(), g, x_extent));
+    else
+      run(Integrator4D2AngTBB<double, Poly2Ang>(provider(), g, x_extent));
+  });
+}
+
+int kat_3ang(int device, int nx, int na, double x_extent, double k, const double *c, const double *a, const double *b,
+             const double *p, double constant, double *out)
+{
+  return guard([&] {
+    auto run = [&](auto integ) {
+      *out = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], b[0], b[1], b[2],
+                       b[3], p[0], p[1], p[2], p[3]);
+    };
+    const std::array<uint, 4> g{{(uint)nx, (uint)na, (uint)na, (uint)na}};
+    if (device)
+      run(Integrator4DGPU<double, Poly3Ang>(provider(), g, x_extent));
+    else
+      run(Integrator4DTBB<double, Poly3Ang>(provider(), g, x_extent));
+  });
+}
+
+// *nm receives the Matsubara rule size used for the (last) evaluation
+int kat_finiteT(int device, int d, int nx, double x_extent, double T, double k, const double *c, const double *den,
+                double constant, double *out, int *nm)
+{
+  return guard([&] {
+    auto run = [&](auto integ) {
+      *out = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], den[0], den[1], den[2], den[3]);
+      (void)nm;
+    };
+    const std::array<uint, 1> g{{(uint)nx}};
+    if (device) {
+      switch (d) {
+      case 2: { IntegratorFiniteTq0GPU<2, double, PolyFiniteT> i(provider(), g, x_extent, T); run(i); *nm = i.matsubara_size(); break; }
+      case 3: { IntegratorFiniteTq0GPU<3, double, PolyFiniteT> i(provider(), g, x_extent, T); run(i); *nm = i.matsubara_size(); break; }
+      case 4: { IntegratorFiniteTq0GPU<4, double, PolyFiniteT> i(provider(), g, x_extent, T); run(i); *nm = i.matsubara_size(); break; }
+      default: throw std::invalid_argument("unsupported dimension");
+      }
+    } else {
+      switch (d) {
+      case 2: run(IntegratorFiniteTq0TBB<2, double, PolyFiniteT>(provider(), g, x_extent, T)); break;
+      case 3: run(IntegratorFiniteTq0TBB<3, double, PolyFiniteT>(provider(), g, x_extent, T)); break;
+      case 4: run(IntegratorFiniteTq0TBB<4, double, PolyFiniteT>(provider(), g, x_extent, T)); break;
+      default: throw std::invalid_argument("unsupported dimension");
+      }
+    }
+  });
+}
+
+// E tracking: evaluate at each k in turn; records the rule size and typical E after each call
+int kat_finiteT_track(int nx, double x_extent, double T, const double *ks, int n, int *nm, double *E)
+{
+  return guard([&] {
+    IntegratorFiniteTq0GPU<4, double, PolyFiniteT> integ(provider(), {{(uint)nx}}, x_extent, T);
+    for (int i = 0; i < n; ++i) {
+      (void)integ.get(ks[i], 0.0, 1.0, 0, 0, 0, 0, 0, T * T, 0, 1.0, 0);
+      nm[i] = integ.matsubara_size();
+      E[i] = integ.typical_E();
+    }
+  });
+}
+
+int kat_threshold(int nx, double x_extent, double k, const double *m2, int n, double *out)
+{
+  return guard([&] {
+    IntegratorGPU<4, double, ThresholdKernel> integ(provider(), nx, x_extent);
+    for (int i = 0; i < n; ++i) out[i] = integ.get(k, m2[i]);
+  });
+}
+
+int kat_selfenergy(int nq, int na, double x_extent, double k, const double *p, int n_p, const double *Z, int n_tab,
+                   double start, double stop, double bias, double *out)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<double> coords((uint)n_tab, start, stop, bias);
+    ZInterp interp(Z, coords);
+    Integrator4D2AngGPU<double, SelfEnergyKernel> integ(provider(), {{(uint)nq, (uint)na, (uint)na}}, x_extent);
+    std::vector<std::future<double>> f;
+    for (int i = 0; i < n_p; ++i) f.push_back(integ.request(k, p[i], interp));
+    for (int i = 0; i < n_p; ++i) out[i] = f[i].get();
+  });
+}
+
+// finite-T two-propagator integrand, reference x-mapping, Matsubara rule of the provider at E = k
+int kat_finiteT_twoprop(int nx, double x_extent, double k, const double *T, const double *p0, const double *pabs,
+                        const double *m1, const double *m2, int n, double *out)
+{
+  return guard([&] {
+    for (int i = 0; i < n; ++i) {
+      IntegratorFiniteTq0GPU<4, double, FiniteTKernel> integ(provider(), {{(uint)nx}}, x_extent, T[i]);
+      out[i] = integ.get(k, p0[i], pabs[i], m1[i], m2[i]);
+    }
+  });
+}
+
+int kat_unequal_angles()
+{
+  return guard([&] { Integrator4D2AngGPU<double, Poly2Ang> integ(provider(), {{16, 8, 16}}, 1.0); });
+}
+
+} // extern "C"
