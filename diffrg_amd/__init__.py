@@ -89,6 +89,13 @@ SIGNATURES = {
     "dfrg_se_batch_evals": [_vp, C.POINTER(C.c_ulonglong)],
     "dfrg_ft_batch": [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
                       C.c_double, C.c_double, C.c_int, _vp, _vp],
+    "dfrg_fs_create": [C.POINTER(_vp), _vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int],
+    "dfrg_fs_destroy": [_vp],
+    "dfrg_fs_submit": [_vp, _vp, C.POINTER(C.c_longlong)],
+    "dfrg_fs_progress": [_vp, _ip, _ip, _ip],
+    "dfrg_fs_poll": [_vp, C.c_longlong, _ip],
+    "dfrg_fs_result": [_vp, C.c_longlong, _dp, C.c_int, _dp],
+    "dfrg_fs_stats": [_vp, _ip, _dp, _dp, _dp],
 }
 
 
@@ -104,6 +111,8 @@ def lib() -> C.CDLL:
             f.argtypes = args
             f.restype = C.c_int
         L.dfrg_last_error.restype = C.c_char_p
+        L.dfrg_fs_flow_bytes.restype = C.c_size_t
+        L.dfrg_fs_flow_bytes.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -344,6 +353,72 @@ class SEBatch:
     def close(self):
         if self.ptr:
             lib().dfrg_se_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------------------- flow server
+
+class FlowDesc(C.Structure):
+    _fields_ = [("n_grid", C.c_int), ("rho_max", C.c_double), ("T", C.c_double), ("mu", C.c_double),
+                ("N", C.c_double), ("m2_0", C.c_double), ("lambda_", C.c_double)]
+
+
+QUEUED, RESIDENT, FINISHED, FAILED = 0, 1, 2, 3
+
+
+def flow_bytes(n_grid: int) -> int:
+    return lib().dfrg_fs_flow_bytes(n_grid)
+
+
+class FlowServer:
+    """Continuous-batching server of O(N) LPA flows (dfrg_fs_*)."""
+
+    def __init__(self, qp: QuadratureProvider, hbm_budget: int, max_resident: int = 64, nx: int = 64,
+                 n_steps: int = 2000, k_end: float = 1e-3, steps_per_launch: int = 10):
+        h = _vp()
+        check(lib().dfrg_fs_create(C.byref(h), qp.ptr, hbm_budget, max_resident, nx, n_steps, k_end,
+                                   steps_per_launch))
+        self.ptr = h.value
+        self.n_grid = {}
+
+    def submit(self, n_grid, T, mu=0.0, rho_max=4.0, N=4.0, m2_0=0.2, lam=1.0) -> int:
+        d = FlowDesc(int(n_grid), rho_max, T, mu, N, m2_0, lam)
+        fid = C.c_longlong()
+        check(lib().dfrg_fs_submit(self.ptr, C.byref(d), C.byref(fid)))
+        self.n_grid[fid.value] = int(n_grid)
+        return fid.value
+
+    def progress(self):
+        r, q, d = C.c_int(), C.c_int(), C.c_int()
+        check(lib().dfrg_fs_progress(self.ptr, C.byref(r), C.byref(q), C.byref(d)))
+        return r.value, q.value, d.value
+
+    def poll(self, fid: int) -> int:
+        s = C.c_int()
+        check(lib().dfrg_fs_poll(self.ptr, fid, C.byref(s)))
+        return s.value
+
+    def result(self, fid: int):
+        n = self.n_grid[fid]
+        u = np.empty(n)
+        lat = C.c_double()
+        check(lib().dfrg_fs_result(self.ptr, fid, u.ctypes.data_as(_dp), n, C.byref(lat)))
+        return u, lat.value
+
+    def stats(self) -> dict:
+        f, p50, p99, busy = C.c_int(), C.c_double(), C.c_double(), C.c_double()
+        check(lib().dfrg_fs_stats(self.ptr, C.byref(f), C.byref(p50), C.byref(p99), C.byref(busy)))
+        return {"finished": f.value, "p50_s": p50.value, "p99_s": p99.value, "busy_s": busy.value}
+
+    def close(self):
+        if self.ptr:
+            lib().dfrg_fs_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

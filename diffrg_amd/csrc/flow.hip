@@ -1,0 +1,486 @@
+// flow.hip -- continuous-batching server of O(N) LPA flows with an on-device RK4 stepper.
+//
+// Each flow evolves u(rho) = U'(rho) on a uniform rho grid from k = Lambda = 1 to k_end with fixed
+// RK4 steps in t = ln(Lambda/k):
+//   du/dt = 1/2 [ (N-1) u' I_T(u) + (3 u' + 2 rho u'') I_T(u + 2 rho u') ],
+//   I_T(m2) = int d^3q/(2pi)^3 RBdot(k^2,q^2) S2(E),  E^2 = q^2 + RB(k^2,q^2) + m2,
+//   S2(E) = T sum_n (w_n^2 + E^2)^-2 = coth(E/2T)/(4E^3) + csch^2(E/2T)/(8 T E^2),
+// i.e. two threshold integrals (the d = 3 radial form of IntegratorGPU<3>, integrator_gpu.hh:34-60,
+// with the closed-form bosonic Matsubara sum, cf. Examples/ONfiniteT/flows/V/V.kernel) per grid
+// point and RK stage.  u', u'' are second-order finite differences (one-sided at the ends).
+//
+// Device layout: every resident flow owns 4 n doubles in one HBM pool (u, RK accumulator, two
+// stage buffers) and a slot in SoA descriptor arrays.  A point table lists (slot, i) of every
+// resident grid point; one wave evaluates one point (one Gauss-Legendre node per lane, wave64 DPP
+// sums), so all resident flows advance together in each launch.  One RK step is five launches
+// (four stages and an advance kernel that counts steps and flags finished / non-finite flows);
+// steps_per_launch steps are captured into one hipGraph.  Between graph launches the host retires
+// finished flows and admits queued ones while their HBM footprint fits the budget.
+#include "internal.hh"
+#include "dfrg/device.hh"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <vector>
+
+using namespace dfrg;
+
+namespace
+{
+  constexpr int kFlowBlock = 256;
+  constexpr int kFlowGrid = 1024; // waves = kFlowGrid * kFlowBlock / 64, grid-stride over points
+  constexpr double kPi = 3.14159265358979323846;
+
+  struct FlowDev {
+    int *n, *off, *step, *status, *bad; // per slot; status 0 idle, 1 running, 2 done, 3 failed
+    double *h, *T, *N;                  // grid spacing, temperature, O(N)
+    int2 *points;                       // (slot, i) of every resident grid point
+    int *n_points;                      // device copy of the point count
+    double *pool;                       // state: u | acc | y0 | y1 per flow
+    const double *x, *w;                // GL nodes on [0,1]
+    int nx, n_steps;
+    double x_extent, dt;
+  };
+
+  // S2(E) with e = exp(-E/T): coth = (1+e)/(1-e), csch^2 = 4e/(1-e)^2, 1-e = -expm1(-E/T)
+  __device__ __forceinline__ double bose_S2(double E, double T)
+  {
+    const double e = exp(-E / T);
+    const double ome = -expm1(-E / T);
+    const double coth = (1.0 + e) / ome;
+    const double csch2 = 4.0 * e / (ome * ome);
+    return coth / (4.0 * E * E * E) + csch2 / (8.0 * T * E * E);
+  }
+
+  __global__ __launch_bounds__(kFlowBlock) void flow_stage_kernel(FlowDev F, int stage)
+  {
+    const int lane = threadIdx.x & 63;
+    const int wave_id = (blockIdx.x * kFlowBlock + threadIdx.x) >> 6;
+    const int n_waves = (gridDim.x * kFlowBlock) >> 6;
+    const int P = *F.n_points;
+    for (int w = wave_id; w < P; w += n_waves) {
+      const int2 pt = F.points[w];
+      const int s = pt.x, i = pt.y;
+      if (F.status[s] != 1) continue; // wave-uniform
+      const int n = F.n[s];
+      double *u = F.pool + F.off[s];
+      double *acc = u + n, *y0 = u + 2 * n, *y1 = u + 3 * n;
+      const double *in = stage == 1 ? u : (stage == 3 ? y1 : y0);
+      const double h = F.h[s], T = F.T[s], Nn = F.N[s];
+      const double t = F.step[s] * F.dt;
+      const double k = exp(-(stage == 1 ? t : (stage == 4 ? t + F.dt : t + 0.5 * F.dt)));
+      double up, upp;
+      if (i == 0) {
+        up = (-3.0 * in[0] + 4.0 * in[1] - in[2]) / (2.0 * h);
+        upp = (in[0] - 2.0 * in[1] + in[2]) / (h * h);
+      } else if (i == n - 1) {
+        up = (3.0 * in[n - 1] - 4.0 * in[n - 2] + in[n - 3]) / (2.0 * h);
+        upp = (in[n - 1] - 2.0 * in[n - 2] + in[n - 3]) / (h * h);
+      } else {
+        up = (in[i + 1] - in[i - 1]) / (2.0 * h);
+        upp = (in[i + 1] - 2.0 * in[i] + in[i - 1]) / (h * h);
+      }
+      const double rho = i * h;
+      const double m2pi = in[i], m2sig = in[i] + 2.0 * rho * up;
+      // two threshold integrals, lane j evaluates node j (d = 3 radial element)
+      const double k2 = k * k;
+      double ipi = 0.0, isg = 0.0;
+      for (int j = lane; j < F.nx; j += 64) {
+        const double q = k * sqrt(F.x[j] * F.x_extent);
+        const double el = 4.0 * kPi * (q / 2.0 * k2) / (8.0 * kPi * kPi * kPi);
+        const double wgt = F.w[j] * F.x_extent;
+        const double q2 = q * q;
+        const double xr = q2 / k2;
+        const double ex = exp(-xr);
+        const double R = k2 * ex;
+        const double rdot = 2.0 * ex * k2 * (1.0 + xr);
+        const double Epi = sqrt(q2 + R + m2pi), Esg = sqrt(q2 + R + m2sig);
+        ipi += el * wgt * (rdot * bose_S2(Epi, T));
+        isg += el * wgt * (rdot * bose_S2(Esg, T));
+      }
+      ipi = dev::wave_sum(ipi);
+      isg = dev::wave_sum(isg);
+      if (lane == 0) {
+        const double f = 0.5 * ((Nn - 1.0) * up * ipi + (3.0 * up + 2.0 * rho * upp) * isg);
+        const double dt = F.dt;
+        if (stage == 1) {
+          acc[i] = f;
+          y0[i] = u[i] + 0.5 * dt * f;
+        } else if (stage == 2) {
+          acc[i] += 2.0 * f;
+          y1[i] = u[i] + 0.5 * dt * f;
+        } else if (stage == 3) {
+          acc[i] += 2.0 * f;
+          y0[i] = u[i] + dt * f;
+        } else {
+          const double un = u[i] + dt / 6.0 * (acc[i] + f);
+          u[i] = un;
+          if (!isfinite(un)) atomicOr(&F.bad[s], 1);
+        }
+      }
+    }
+  }
+
+  // one lane per slot: count the step, flag finished and non-finite flows
+  __global__ void flow_advance_kernel(FlowDev F, int n_slots)
+  {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots || F.status[s] != 1) return;
+    const int st = F.step[s] + 1;
+    F.step[s] = st;
+    if (F.bad[s])
+      F.status[s] = 3;
+    else if (st >= F.n_steps)
+      F.status[s] = 2;
+  }
+
+  double now_s()
+  {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  }
+} // namespace
+
+struct FlowRecord {
+  dfrg_flow_desc desc;
+  int state = 0; // 0 queued, 1 resident, 2 finished, 3 failed
+  double t_submit = 0, t_done = 0;
+  int slot = -1, off = -1;
+  std::vector<double> u;
+};
+
+struct dfrg_fs {
+  size_t budget = 0;
+  int max_resident = 0, nx = 64, n_steps = 2000, steps_per_launch = 10;
+  double k_end = 1e-3, x_extent = 20.0;
+  FlowDev dev{};
+  size_t pool_doubles = 0;
+  std::vector<std::pair<int, int>> free_ranges; // (offset, length) in doubles, sorted by offset
+  std::vector<long long> slot_flow;             // flow id per slot, -1 idle
+  std::map<long long, FlowRecord> flows;
+  std::deque<long long> queue;
+  long long next_id = 1;
+  bool points_dirty = true;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<double> latencies;
+  double busy_s = 0;
+  int finished = 0;
+};
+
+namespace
+{
+  int fs_alloc_range(dfrg_fs *fs, int len)
+  {
+    for (size_t r = 0; r < fs->free_ranges.size(); ++r) {
+      auto &fr = fs->free_ranges[r];
+      if (fr.second >= len) {
+        const int off = fr.first;
+        fr.first += len;
+        fr.second -= len;
+        if (fr.second == 0) fs->free_ranges.erase(fs->free_ranges.begin() + r);
+        return off;
+      }
+    }
+    return -1;
+  }
+
+  void fs_free_range(dfrg_fs *fs, int off, int len)
+  {
+    fs->free_ranges.push_back({off, len});
+    std::sort(fs->free_ranges.begin(), fs->free_ranges.end());
+    std::vector<std::pair<int, int>> merged;
+    for (auto &r : fs->free_ranges) {
+      if (!merged.empty() && merged.back().first + merged.back().second == r.first)
+        merged.back().second += r.second;
+      else
+        merged.push_back(r);
+    }
+    fs->free_ranges.swap(merged);
+  }
+
+  int fs_capture(dfrg_fs *fs)
+  {
+    if (fs->graph_exec) return DFRG_OK;
+    hipStream_t cs;
+    DFRG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    DFRG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int st = 0; st < fs->steps_per_launch; ++st) {
+      for (int stage = 1; stage <= 4; ++stage)
+        hipLaunchKernelGGL(flow_stage_kernel, dim3(kFlowGrid), dim3(kFlowBlock), 0, cs, fs->dev, stage);
+      hipLaunchKernelGGL(flow_advance_kernel, dim3((fs->max_resident + 63) / 64), dim3(64), 0, cs, fs->dev,
+                         fs->max_resident);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+      set_error("flow graph capture failed: %s", hipGetErrorString(e));
+      return DFRG_ERR_HIP;
+    }
+    fs->graph = g;
+    DFRG_HIP(hipGraphInstantiate(&fs->graph_exec, g, nullptr, nullptr, 0));
+    return DFRG_OK;
+  }
+
+  // retire finished / failed flows: copy u back, free their slot and HBM range
+  int fs_retire(dfrg_fs *fs)
+  {
+    std::vector<int> status(fs->max_resident);
+    DFRG_HIP(hipMemcpy(status.data(), fs->dev.status, sizeof(int) * fs->max_resident, hipMemcpyDeviceToHost));
+    const double t = now_s();
+    for (int s = 0; s < fs->max_resident; ++s) {
+      if (fs->slot_flow[s] < 0 || (status[s] != 2 && status[s] != 3)) continue;
+      FlowRecord &r = fs->flows[fs->slot_flow[s]];
+      const int n = r.desc.n_grid;
+      r.u.resize(n);
+      DFRG_HIP(hipMemcpy(r.u.data(), fs->dev.pool + r.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      r.state = status[s];
+      r.t_done = t;
+      if (r.state == 2) {
+        fs->latencies.push_back(r.t_done - r.t_submit);
+        ++fs->finished;
+      }
+      fs_free_range(fs, r.off, 4 * n);
+      const int zero = 0;
+      DFRG_HIP(hipMemcpy(fs->dev.status + s, &zero, sizeof(int), hipMemcpyHostToDevice));
+      fs->slot_flow[s] = -1;
+      r.slot = -1;
+      fs->points_dirty = true;
+    }
+    return DFRG_OK;
+  }
+
+  int fs_admit(dfrg_fs *fs)
+  {
+    while (!fs->queue.empty()) {
+      const long long id = fs->queue.front();
+      FlowRecord &r = fs->flows[id];
+      const int n = r.desc.n_grid;
+      int slot = -1;
+      for (int s = 0; s < fs->max_resident; ++s)
+        if (fs->slot_flow[s] < 0) {
+          slot = s;
+          break;
+        }
+      if (slot < 0) break;
+      const int off = fs_alloc_range(fs, 4 * n);
+      if (off < 0) break; // does not fit the HBM budget now: wait for retirements (FIFO)
+      fs->queue.pop_front();
+      std::vector<double> init(4 * n, 0.0);
+      const double h = r.desc.rho_max / (n - 1);
+      for (int i = 0; i < n; ++i) init[i] = r.desc.m2_0 + r.desc.lambda / 2.0 * (i * h);
+      DFRG_HIP(hipMemcpy(fs->dev.pool + off, init.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+      const int one = 1, zero = 0;
+      DFRG_HIP(hipMemcpy(fs->dev.n + slot, &n, sizeof(int), hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.off + slot, &off, sizeof(int), hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.step + slot, &zero, sizeof(int), hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.bad + slot, &zero, sizeof(int), hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.h + slot, &h, sizeof(double), hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.T + slot, &r.desc.T, sizeof(double), hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.N + slot, &r.desc.N, sizeof(double), hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.status + slot, &one, sizeof(int), hipMemcpyHostToDevice));
+      fs->slot_flow[slot] = id;
+      r.slot = slot;
+      r.off = off;
+      r.state = 1;
+      fs->points_dirty = true;
+    }
+    if (fs->points_dirty) {
+      std::vector<int2> pts;
+      for (int s = 0; s < fs->max_resident; ++s) {
+        if (fs->slot_flow[s] < 0) continue;
+        const int n = fs->flows[fs->slot_flow[s]].desc.n_grid;
+        for (int i = 0; i < n; ++i) pts.push_back(make_int2(s, i));
+      }
+      const int P = (int)pts.size();
+      if (P) DFRG_HIP(hipMemcpy(fs->dev.points, pts.data(), sizeof(int2) * P, hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.n_points, &P, sizeof(int), hipMemcpyHostToDevice));
+      fs->points_dirty = false;
+    }
+    return DFRG_OK;
+  }
+
+  int resident_count(const dfrg_fs *fs)
+  {
+    int c = 0;
+    for (long long id : fs->slot_flow) c += id >= 0;
+    return c;
+  }
+} // namespace
+
+extern "C" {
+
+size_t dfrg_fs_flow_bytes(int n_grid) { return (size_t)4 * n_grid * sizeof(double); }
+
+int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_resident, int nx, int n_steps,
+                   double k_end, int steps_per_launch)
+{
+  DFRG_TRY(require_init());
+  DFRG_CHECK_ARG(out && qp, "NULL handle");
+  DFRG_CHECK_ARG(max_resident >= 1 && max_resident <= 4096, "max_resident %d out of range [1,4096]", max_resident);
+  DFRG_CHECK_ARG(nx >= 1 && nx <= 1024, "nx %d out of range", nx);
+  DFRG_CHECK_ARG(n_steps >= 1 && k_end > 0 && k_end < 1, "invalid n_steps / k_end");
+  DFRG_CHECK_ARG(steps_per_launch >= 1 && steps_per_launch <= 1000, "steps_per_launch out of range");
+  const size_t max_points = (size_t)max_resident * 4096;
+  DFRG_CHECK_ARG(hbm_budget >= dfrg_fs_flow_bytes(4) && hbm_budget / sizeof(double) < (size_t)1 << 31,
+                 "hbm_budget out of range");
+  dfrg_fs *fs = new dfrg_fs;
+  fs->budget = hbm_budget;
+  fs->max_resident = max_resident;
+  fs->nx = nx;
+  fs->n_steps = n_steps;
+  fs->k_end = k_end;
+  fs->steps_per_launch = steps_per_launch;
+  fs->pool_doubles = hbm_budget / sizeof(double);
+  fs->free_ranges.push_back({0, (int)fs->pool_doubles});
+  fs->slot_flow.assign(max_resident, -1);
+  FlowDev &d = fs->dev;
+  d.nx = nx;
+  d.n_steps = n_steps;
+  d.x_extent = fs->x_extent;
+  d.dt = std::log(1.0 / k_end) / n_steps;
+  const void *dx = nullptr, *dw = nullptr;
+  int rc = dfrg_qp_gl(qp, nx, 0, &dx, &dw, nullptr, nullptr);
+  if (rc != DFRG_OK) {
+    delete fs;
+    return rc;
+  }
+  d.x = (const double *)dx;
+  d.w = (const double *)dw;
+  const size_t pts_bytes = sizeof(int2) * std::min(max_points, fs->pool_doubles / 4 + 1);
+#define FS_ALLOC(ptr, bytes)                                                                                   \
+  do {                                                                                                         \
+    if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess || hipMemset((ptr), 0, (bytes)) != hipSuccess) {     \
+      set_error("flow server: device allocation of %zu bytes failed", (size_t)(bytes));                        \
+      dfrg_fs_destroy(fs);                                                                                     \
+      return DFRG_ERR_HIP;                                                                                     \
+    }                                                                                                          \
+  } while (0)
+  FS_ALLOC(d.n, sizeof(int) * max_resident);
+  FS_ALLOC(d.off, sizeof(int) * max_resident);
+  FS_ALLOC(d.step, sizeof(int) * max_resident);
+  FS_ALLOC(d.status, sizeof(int) * max_resident);
+  FS_ALLOC(d.bad, sizeof(int) * max_resident);
+  FS_ALLOC(d.h, sizeof(double) * max_resident);
+  FS_ALLOC(d.T, sizeof(double) * max_resident);
+  FS_ALLOC(d.N, sizeof(double) * max_resident);
+  FS_ALLOC(d.points, pts_bytes);
+  FS_ALLOC(d.n_points, sizeof(int));
+  FS_ALLOC(d.pool, sizeof(double) * fs->pool_doubles);
+#undef FS_ALLOC
+  if (hipStreamCreateWithFlags(&fs->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("flow server: stream creation failed");
+    dfrg_fs_destroy(fs);
+    return DFRG_ERR_HIP;
+  }
+  DFRG_HIP(hipDeviceSynchronize());
+  *out = fs;
+  return DFRG_OK;
+}
+
+int dfrg_fs_destroy(dfrg_fs *fs)
+{
+  if (!fs) return DFRG_OK;
+  hipDeviceSynchronize();
+  if (fs->graph_exec) hipGraphExecDestroy(fs->graph_exec);
+  if (fs->graph) hipGraphDestroy(fs->graph);
+  if (fs->stream) hipStreamDestroy(fs->stream);
+  FlowDev &d = fs->dev;
+  for (void *p : {(void *)d.n, (void *)d.off, (void *)d.step, (void *)d.status, (void *)d.bad, (void *)d.h,
+                  (void *)d.T, (void *)d.N, (void *)d.points, (void *)d.n_points, (void *)d.pool})
+    if (p) hipFree(p);
+  delete fs;
+  return DFRG_OK;
+}
+
+int dfrg_fs_submit(dfrg_fs *fs, const dfrg_flow_desc *desc, long long *id)
+{
+  DFRG_CHECK_ARG(fs && desc && id, "NULL argument");
+  DFRG_CHECK_ARG(desc->n_grid >= 4 && desc->n_grid <= 4096, "n_grid %d out of range [4,4096]", desc->n_grid);
+  DFRG_CHECK_ARG(desc->rho_max > 0 && desc->T > 0 && desc->N >= 1, "invalid flow parameters");
+  DFRG_CHECK_ARG(std::isfinite(desc->m2_0) && std::isfinite(desc->lambda) && std::isfinite(desc->mu),
+                 "invalid flow parameters");
+  DFRG_CHECK_ARG(dfrg_fs_flow_bytes(desc->n_grid) <= fs->budget, "flow of %d points exceeds the HBM budget",
+                 desc->n_grid);
+  FlowRecord r;
+  r.desc = *desc;
+  r.t_submit = now_s();
+  const long long fid = fs->next_id++;
+  fs->flows.emplace(fid, std::move(r));
+  fs->queue.push_back(fid);
+  *id = fid;
+  return DFRG_OK;
+}
+
+int dfrg_fs_progress(dfrg_fs *fs, int *resident, int *queued, int *done)
+{
+  DFRG_TRY(require_init());
+  DFRG_CHECK_ARG(fs != nullptr, "NULL server");
+  DFRG_TRY(fs_admit(fs));
+  if (resident_count(fs) > 0) {
+    DFRG_TRY(fs_capture(fs));
+    const double t0 = now_s();
+    DFRG_HIP(hipGraphLaunch(fs->graph_exec, fs->stream));
+    DFRG_HIP(hipStreamSynchronize(fs->stream));
+    fs->busy_s += now_s() - t0;
+    DFRG_TRY(fs_retire(fs));
+    DFRG_TRY(fs_admit(fs));
+  }
+  if (resident) *resident = resident_count(fs);
+  if (queued) *queued = (int)fs->queue.size();
+  if (done) *done = fs->finished;
+  return DFRG_OK;
+}
+
+int dfrg_fs_poll(dfrg_fs *fs, long long id, int *state)
+{
+  DFRG_CHECK_ARG(fs && state, "NULL argument");
+  auto it = fs->flows.find(id);
+  DFRG_CHECK_ARG(it != fs->flows.end(), "unknown flow id %lld", id);
+  *state = it->second.state;
+  return DFRG_OK;
+}
+
+int dfrg_fs_result(dfrg_fs *fs, long long id, double *u_out, int n, double *latency_s)
+{
+  DFRG_CHECK_ARG(fs && u_out, "NULL argument");
+  auto it = fs->flows.find(id);
+  DFRG_CHECK_ARG(it != fs->flows.end(), "unknown flow id %lld", id);
+  FlowRecord &r = it->second;
+  DFRG_CHECK_ARG(r.state == 2 || r.state == 3, "flow %lld has not finished", id);
+  DFRG_CHECK_ARG(n >= r.desc.n_grid, "output holds %d < %d points", n, r.desc.n_grid);
+  std::copy(r.u.begin(), r.u.end(), u_out);
+  if (latency_s) *latency_s = r.t_done - r.t_submit;
+  const int failed = r.state == 3;
+  fs->flows.erase(it);
+  if (failed) {
+    set_error("flow %lld became non-finite", id);
+    return DFRG_ERR_RUNTIME;
+  }
+  return DFRG_OK;
+}
+
+int dfrg_fs_stats(dfrg_fs *fs, int *finished, double *p50_s, double *p99_s, double *busy_s)
+{
+  DFRG_CHECK_ARG(fs != nullptr, "NULL server");
+  std::vector<double> l = fs->latencies;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double p) {
+    if (l.empty()) return 0.0;
+    const double pos = p * (l.size() - 1);
+    const size_t i0 = (size_t)pos;
+    const size_t i1 = std::min(i0 + 1, l.size() - 1);
+    return l[i0] + (pos - i0) * (l[i1] - l[i0]);
+  };
+  if (finished) *finished = fs->finished;
+  if (p50_s) *p50_s = pct(0.50);
+  if (p99_s) *p99_s = pct(0.99);
+  if (busy_s) *busy_s = fs->busy_s;
+  return DFRG_OK;
+}
+
+} // extern "C"

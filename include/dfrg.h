@@ -163,6 +163,44 @@ int dfrg_ft_batch(int d, int radial_mode, int n_int, const double *d_T, const do
                   const int *d_m_n, const double *d_x, const double *d_w, int nx, double extent, double k, int mode,
                   double *d_out, void *stream);
 
+/* ------------------------------------------------------------------------------------------ */
+/* LPA flow server: continuous batching of O(N) LPA flows with an on-device RK4 stepper.       */
+/* No reference counterpart: the reference runs LPA flows as FE systems under SUNDIALS IDA     */
+/* (Examples/ONfiniteT/CG.cc:18-51) with one IntegratorTBB<3> call per quadrature point; its   */
+/* threshold integral is the V.kernel form (Examples/ONfiniteT/flows/V/V.kernel:4-15).         */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+  int n_grid;       /* uniform rho grid points (>= 4) */
+  double rho_max;   /* rho in [0, rho_max] */
+  double T;         /* temperature (> 0) */
+  double mu;        /* chemical potential (recorded; the bosonic O(N) kernel has no mu coupling) */
+  double N;         /* O(N) */
+  double m2_0;      /* u(rho) = m2_0 + lambda/2 rho at k = Lambda = 1 */
+  double lambda;
+} dfrg_flow_desc;
+
+typedef struct dfrg_fs dfrg_fs;
+/* hbm_budget: bytes of HBM the resident flows may hold (admission control); max_resident: slot
+ * limit; nx: GL nodes of each threshold integral (x_extent 20, exponential regulator b=1);
+ * n_steps RK4 steps from k = 1 to k_end; steps_per_launch: RK4 steps per graph launch. */
+int dfrg_fs_create(dfrg_fs **fs, dfrg_qp *qp, size_t hbm_budget, int max_resident, int nx, int n_steps,
+                   double k_end, int steps_per_launch);
+int dfrg_fs_destroy(dfrg_fs *fs);
+/* Queue a flow; *id receives its handle.  Timestamped on arrival. */
+int dfrg_fs_submit(dfrg_fs *fs, const dfrg_flow_desc *desc, long long *id);
+/* Admit queued flows that fit, advance every resident flow by one launch of steps, retire the
+ * finished ones.  *resident / *queued / *done (may be NULL) report the state afterwards. */
+int dfrg_fs_progress(dfrg_fs *fs, int *resident, int *queued, int *done);
+/* state: 0 queued, 1 resident, 2 finished, 3 failed (non-finite state; slot freed). */
+int dfrg_fs_poll(dfrg_fs *fs, long long id, int *state);
+/* Final u(rho) of a finished flow (n_grid doubles) and its latency in seconds (arrival to
+ * completion).  Releases the host copy. */
+int dfrg_fs_result(dfrg_fs *fs, long long id, double *u_out, int n, double *latency_s);
+/* Aggregate statistics of retired flows: count, p50 and p99 latency (s), busy seconds. */
+int dfrg_fs_stats(dfrg_fs *fs, int *finished, double *p50_s, double *p99_s, double *busy_s);
+/* Bytes of HBM one flow of n_grid points occupies while resident. */
+size_t dfrg_fs_flow_bytes(int n_grid);
+
 #ifdef __cplusplus
 }
 #endif
