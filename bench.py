@@ -1,17 +1,23 @@
 """Benchmark of the momentum-loop integrators on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload selfenergy|threshold|finite_T|uneven]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only-headline] [--no-cpu-baseline]
 
-A step is one evaluation of the workload's whole call set (for the default workload, the 64
-two-angle self-energy integrals {64 q, 32 cos1, 32 cos2} of the 64 external momenta), with inputs
-resident in HBM.  Both precisions run in the same process: FP64 (plain) and mixed (FP32 integrand,
-FP64 nodes/denominators/accumulation); `value` is the mixed path's integrals/s, accepted only if
-its relative error against the FP64 oracle is <= 1e-8.  Prints one JSON line.
+Headline (the JSON line's metric/value): the two-angle self-energy call set -- 64 external
+momenta on a log grid in [1e-3, 40], nodes {64 q^2, 32 cos1, 32 cos2}, 128-point dressing table --
+evaluated in mixed precision (FP32 hardware seeds, FP64 nodes/denominators/corrections/
+accumulation), accepted only when its relative error against the FP64 oracle is <= 1e-8.  A step
+is one evaluation of the whole call set with inputs resident in HBM; K steps are timed between
+two HIP events on the stream the kernels run on.  The plain FP64 path is timed in the same run.
+
+The same run also measures the other workloads (reported under "workloads"): LPA threshold
+integrals, finite-T integrals, the uneven 4096-integral persistent batch (hipGraph-captured) and
+the flow server under Poisson arrivals (p50/p99 latency), each checked against the oracle.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -22,9 +28,23 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
-PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
+# MI355X published FP64 vector peak (FLOP/s).  Packed FP32 is disabled in this build, so plain FP32
+# and FP64 VALU instructions issue at the same rate and this peak bounds both precisions.
+PEAK_FP64_VALU = 78.6e12
+# one wave64 VALU instruction per clock per CU (256 CUs x 4 SIMD16 x 2.4 GHz / 4 clocks)
+PEAK_VALU_WAVE_INSTR = 256 * 2.4e9
 MIXED_TOL = 1e-8
+
+# Per integrand evaluation of the innermost loop of se_kernel (FP64, MIXED): FP operations
+# (FMA = 2), VALU instructions and quarter-rate transcendentals, counted from the gfx950
+# assembly of csrc/integrals.hip by `python -m diffrg_amd.isa_count`.
+SE_FLOP = {0: 224, 1: 52}
+SE_VALU = {0: 220, 1: 60}
+
+
+def valu_issue_frac(mode, evals, seconds):
+    """Fraction of the chip's VALU issue slots the innermost loop alone fills at this rate."""
+    return SE_VALU[mode] * evals / 64 / seconds / PEAK_VALU_WAVE_INSTR
 
 
 def parse():
@@ -32,15 +52,49 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
-    ap.add_argument("--workload", default="selfenergy", choices=["selfenergy", "threshold", "finite_T", "uneven"])
+    ap.add_argument("--only-headline", action="store_true", help="skip the other workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
+    ap.add_argument("--flows", type=int, default=48, help="flows in the serving run")
+    ap.add_argument("--shape", action="append", default=[], metavar="MODE:BLOCK,TILE,GRID",
+                    help="fixed launch shape instead of the autotuner for the headline (MODE fp64|mixed); "
+                         "for counter runs, where serialised dispatches distort the tuner's timings")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (calibrates the HBM byte counters)")
     return ap.parse_args()
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "se_kernel_selfenergy_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per se_kernel<MIXED> launch from the committed counter summary, or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """Copies a 256 MiB buffer 5 times with torch's copy kernel: 256 MiB read + 256 MiB written."""
+    import torch
+    src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "copies": 5}))
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def time_steps(D, fn, steps, warmup):
-    """Run fn() warmup times, then time `steps` calls between two HIP events on the library's
-    stream of this thread (the stream every launch below is enqueued on)."""
+    """Warm up, then time `steps` calls of fn() between two HIP events on this thread's stream."""
     for _ in range(warmup):
         fn()
     D.stream_sync()
@@ -55,101 +109,267 @@ def time_steps(D, fn, steps, warmup):
     return e0.elapsed_ms(e1) / steps, wall
 
 
-# Floating-point operations per integrand evaluation of the self-energy kernels, counted from
-# the gfx950 ISA of the inner loop ("FLOP accounting").
-FLOP_PER_EVAL = {"selfenergy": {0: None, 1: None}}
+def rel_err(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.abs(np.asarray(b))))
 
 
-def run_selfenergy(D, args, out_json):
-    import diffrg_amd.workloads as W
-    c = W.selfenergy()
-    qp = D.QuadratureProvider()
-    n = len(c["p"])
-    b = D.SEBatch(qp, c["p"], c["nq"], c["na"], [c["Z"]] * n, *[[v] * n for v in c["coord"]],
-                  x_extent=c["x_extent"], k=c["k"])
-    evals = b.evals
-    out = D.DeviceArray(n)
-    res = {}
-    for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
-        tune = b.autotune(mode, reps=5)
-        ms, wall = time_steps(D, lambda: b.run(out, mode), args.steps, args.warmup)
-        vals = out.to_host()
-        res[name] = {"ms": ms, "wall_s": wall, "shape": tune, "out": vals}
-    # parity of the mixed path against the FP64 oracle on the same nodes (and the FP64 GPU path)
-    from oracle import oracle as O
-    xq = (qp.get_points(c["nq"]), qp.get_weights(c["nq"]))
-    xa = (qp.get_points(c["na"]), qp.get_weights(c["na"]))
-    coord = O.logcoord(c["n_tab"], *c["coord"])
-    ref = O.c1_selfenergy(c["p"], [c["Z"]] * n, [coord] * n, [xq] * n, [xa] * n, c["x_extent"], c["k"])
-    err_fp64 = float(np.max(np.abs(res["fp64"]["out"] - ref) / np.abs(ref)))
-    err_mixed = float(np.max(np.abs(res["mixed"]["out"] - ref) / np.abs(ref)))
-    ms_mixed, ms_fp64 = res["mixed"]["ms"], res["fp64"]["ms"]
-    out_json.update({
-        "metric": "integrals/sec at rel.err<=1e-8 vs FP64 (mixed vs plain)",
-        "value": n / (ms_mixed * 1e-3) if err_mixed <= MIXED_TOL else 0.0,
-        "unit": "integrals/s",
-        "ms_per_step": ms_mixed,
-        "dtype": "fp32-eval/fp64-acc",
-        "config": {"workload": "selfenergy: 64 p on log grid [1e-3,40], 2-angle {64,32,32}, 128-pt Z table",
-                   "integrals_per_step": n, "evals_per_step": evals},
-        "plain_fp64": {"integrals_per_s": n / (ms_fp64 * 1e-3), "ms_per_step": ms_fp64,
-                       "evals_per_s": evals / (ms_fp64 * 1e-3), "max_rel_err_vs_oracle": err_fp64,
-                       "shape": res["fp64"]["shape"]},
-        "mixed": {"integrals_per_s": n / (ms_mixed * 1e-3), "evals_per_s": evals / (ms_mixed * 1e-3),
-                  "max_rel_err_vs_oracle": err_mixed, "shape": res["mixed"]["shape"]},
-        "mixed_over_plain": ms_fp64 / ms_mixed,
-    })
-    return {"evals": evals, "n": n, "ms_mixed": ms_mixed, "ms_fp64": ms_fp64, "ref_fn": lambda: O.c1_selfenergy(
-        c["p"], [c["Z"]] * n, [coord] * n, [xq] * n, [xa] * n, c["x_extent"], c["k"])}
-
-
-def cpu_baseline(info, seconds):
-    from oracle import oracle as O
-    fn = info["ref_fn"]
-    fn()  # warm-up
+def cpu_timed(fn, seconds, min_runs=3, max_runs=50):
+    fn()
     times = []
     t_start = time.perf_counter()
-    while len(times) < 5 or (time.perf_counter() - t_start < seconds and len(times) < 50):
+    while len(times) < min_runs or (time.perf_counter() - t_start < seconds and len(times) < max_runs):
         t0 = time.perf_counter()
         fn()
         times.append(time.perf_counter() - t0)
         if time.perf_counter() - t_start > 4 * seconds:
             break
-    med = statistics.median(times)
-    cpu_model = ""
+    return statistics.median(times), len(times)
+
+
+def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": info["n"] / med, "unit": "integrals/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"{len(times)} runs of the full call set ({info['n']} integrals, {info['evals']} evals) "
-                      f"on the C restatement of the reference TBB loop, median {med * 1e3:.1f} ms",
-            "evals_per_s": info["evals"] / med, "cpu_model": cpu_model}
+    return ""
+
+
+# --------------------------------------------------------------------------------- workloads
+
+def selfenergy(D, qp, args):
+    import diffrg_amd.workloads as W
+    from oracle import oracle as O
+    c = W.selfenergy()
+    n = len(c["p"])
+    b = D.SEBatch(qp, c["p"], c["nq"], c["na"], [c["Z"]] * n, *[[v] * n for v in c["coord"]],
+                  x_extent=c["x_extent"], k=c["k"])
+    out = D.DeviceArray(n)
+    xq = (qp.get_points(c["nq"]), qp.get_weights(c["nq"]))
+    xa = (qp.get_points(c["na"]), qp.get_weights(c["na"]))
+    coord = O.logcoord(c["n_tab"], *c["coord"])
+    ref_fn = lambda: O.c1_selfenergy(c["p"], [c["Z"]] * n, [coord] * n, [xq] * n, [xa] * n, c["x_extent"], c["k"])
+    ref = ref_fn()
+    res = {}
+    fixed = {m.split(":")[0]: [int(v) for v in m.split(":")[1].split(",")] for m in args.shape}
+    for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
+        if name in fixed:
+            b.set_shape(*fixed[name])
+            shape = b.shape()
+        else:
+            shape = b.autotune(mode, reps=5)
+        ms, wall = time_steps(D, lambda: b.run(out, mode), args.steps, args.warmup)
+        res[name] = {"ms": ms, "shape": shape, "err": rel_err(out.to_host(), ref)}
+        log(f"selfenergy {name}: {ms * 1e3:.1f} us/step, err {res[name]['err']:.2e}, shape {shape}")
+    return {"n": n, "evals": b.evals, "res": res, "ref_fn": ref_fn}
+
+
+def threshold(D, qp, args):
+    import diffrg_amd.workloads as W
+    from oracle import oracle as O
+    c = W.threshold()
+    x, w = qp.get_points(64), qp.get_weights(64)
+    ref_fn = lambda: O.c0_threshold(c["m2"], x, w, c["x_extent"], c["k"], 1.0, c["d"])
+    ref = ref_fn()
+    d_m2, d_x, d_w = (D.DeviceArray.from_host(a) for a in (c["m2"], x, w))
+    out = D.DeviceArray(len(c["m2"]))
+    r = {}
+    for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
+        ms, _ = time_steps(D, lambda: D.threshold_batch(d_m2, d_x, d_w, out, c["x_extent"], c["k"], mode, c["d"]),
+                           args.steps, args.warmup)
+        r[name] = {"integrals_per_s": len(c["m2"]) / (ms * 1e-3), "us_per_step": ms * 1e3,
+                   "max_rel_err": rel_err(out.to_host(), ref)}
+    t_cpu, runs = cpu_timed(ref_fn, 2.0)
+    r["cpu_integrals_per_s"] = len(c["m2"]) / t_cpu
+    r["config"] = "256 m2 ~ U[-0.8,4], 64 GL nodes in q^2 over [0,20], d=4"
+    return r
+
+
+def finite_T(D, qp, args):
+    import diffrg_amd.workloads as W
+    from oracle import oracle as O
+    c = W.finite_T()
+    x, w = qp.get_points(c["nx"]), qp.get_weights(c["nx"])
+    temps = sorted(c["rules"])
+    starts = np.concatenate([[0], np.cumsum([len(c["rules"][t][0]) for t in temps])])
+    off = {t: int(starts[i]) for i, t in enumerate(temps)}
+    Dh = D.DeviceArray.from_host
+    dev = {"T": Dh(c["T"]), "p0": Dh(c["p0"]), "pabs": Dh(c["pabs"]), "m1": Dh(c["m1"]), "m2": Dh(c["m2"]),
+           "mx": Dh(np.concatenate([c["rules"][t][0] for t in temps])),
+           "mw": Dh(np.concatenate([c["rules"][t][1] for t in temps])),
+           "m_off": Dh(np.array([off[t] for t in c["T"]], np.int32)),
+           "m_n": Dh(np.array([len(c["rules"][t][0]) for t in c["T"]], np.int32)), "x": Dh(x), "w": Dh(w)}
+    mats = [c["rules"][t] for t in c["T"]]
+    ref_fn = lambda: O.c2_finiteT(c["T"], c["p0"], c["pabs"], c["m1"], c["m2"], mats, x, w, c["q_max"], c["k"], 1.0,
+                                  c["d"], c["radial_mode"])
+    ref = ref_fn()
+    out = D.DeviceArray(len(c["T"]))
+    evals = int(sum(c["nx"] * (2 * len(c["rules"][t][0]) + 1) for t in c["T"]))
+    r = {"evals_per_step": evals}
+    for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
+        ms, _ = time_steps(D, lambda: D.ft_batch(dev, out, c["q_max"], c["k"], mode, c["d"], c["radial_mode"]),
+                           args.steps, args.warmup)
+        got = out.to_host()
+        err = np.abs(got - ref) / np.abs(ref)
+        r[name] = {"integrals_per_s": len(c["T"]) / (ms * 1e-3), "evals_per_s": evals / (ms * 1e-3),
+                   "us_per_step": ms * 1e3, "max_rel_err": float(err.max()),
+                   "max_rel_err_per_T": {str(t): float(err[c["T"] == t].max()) for t in W.TEMPS}}
+    r["mixed_over_plain"] = r["mixed"]["integrals_per_s"] / r["fp64"]["integrals_per_s"]
+    t_cpu, _ = cpu_timed(ref_fn, 2.0)
+    r["cpu_integrals_per_s"] = len(c["T"]) / t_cpu
+    r["config"] = "512 externals, T in {0.01..0.3}, explicit Matsubara n in [-64,64], 48 GL in |q| over [0,6]"
+    return r
+
+
+def uneven(D, qp, args):
+    import diffrg_amd.workloads as W
+    from oracle import oracle as O
+    c = W.uneven()
+    n = len(c["p"])
+    b = D.SEBatch(qp, c["p"], c["nq"], c["na"], c["tables"], *[[v] * n for v in c["coord"]],
+                  x_extent=c["x_extent"], k=c["k"])
+    xq = [(qp.get_points(int(v)), qp.get_weights(int(v))) for v in c["nq"]]
+    xa = [(qp.get_points(int(v)), qp.get_weights(int(v))) for v in c["na"]]
+    coords = [O.logcoord(len(t), *c["coord"]) for t in c["tables"]]
+    ref_fn = lambda: O.c1_selfenergy(c["p"], c["tables"], coords, xq, xa, c["x_extent"], c["k"])
+    t0 = time.perf_counter()
+    ref = ref_fn()
+    t_cpu = time.perf_counter() - t0
+    out = D.DeviceArray(n)
+    r = {"evals_per_step": b.evals}
+    per_graph = 10
+    steps = max(1, args.steps // (4 * per_graph))
+    for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
+        shape = b.autotune(mode, reps=2, cache_path=None)
+        ms, _ = time_steps(D, lambda: b.graph_launch(out, mode, launches=per_graph), steps, 2)
+        ms /= per_graph
+        r[name] = {"integrals_per_s": n / (ms * 1e-3), "evals_per_s": b.evals / (ms * 1e-3), "ms_per_batch": ms,
+                   "max_rel_err": rel_err(out.to_host(), ref), "shape": shape,
+                   "valu_issue_frac": valu_issue_frac(mode, b.evals, ms * 1e-3)}
+        log(f"uneven {name}: {ms:.3f} ms/batch, err {r[name]['max_rel_err']:.2e}, shape {shape}")
+    r["mixed_over_plain"] = r["mixed"]["integrals_per_s"] / r["fp64"]["integrals_per_s"]
+    r["cpu_integrals_per_s"] = n / t_cpu
+    r["config"] = ("4096 two-angle integrals, radial order in {16,32,64,128}, angular in {8,16,32,64}, own 32-256 "
+                   "point Z table and p; one persistent launch per batch, hipGraph of 10 batches")
+    return r
+
+
+def flows(D, qp, args):
+    """Flow serving: saturation throughput, then Poisson arrivals at 0.7x saturation."""
+    import diffrg_amd.workloads as W
+    from oracle import oracle as O
+    n_flows = args.flows
+    budget = 64 << 20
+    # saturation: all flows submitted at t=0
+    fs = D.FlowServer(qp, hbm_budget=budget, max_resident=64, steps_per_launch=25)
+    reqs = W.flows(n_flows, rate=1.0)
+    t0 = time.perf_counter()
+    ids = [fs.submit(q["n_grid"], q["T"], q["mu"], q["rho_max"], q["N"], q["m2_0"], q["lambda"]) for q in reqs]
+    done = 0
+    while done < n_flows:
+        _, _, done = fs.progress()
+    sat = n_flows / (time.perf_counter() - t0)
+    results = {fid: fs.result(fid)[0] for fid in ids}
+    fs.close()
+    # Poisson arrivals at 0.7 x saturation
+    rate = 0.7 * sat
+    reqs = W.flows(n_flows, rate=rate)
+    fs = D.FlowServer(qp, hbm_budget=budget, max_resident=64, steps_per_launch=25)
+    pending = list(reqs)
+    ids2 = []
+    t0 = time.perf_counter()
+    finished = 0
+    while finished < n_flows:
+        now = time.perf_counter() - t0
+        while pending and pending[0]["arrival"] <= now:
+            q = pending.pop(0)
+            ids2.append((fs.submit(q["n_grid"], q["T"], q["mu"], q["rho_max"], q["N"], q["m2_0"], q["lambda"]), q))
+        r, qd, finished = fs.progress()
+        if r == 0 and qd == 0 and pending:
+            time.sleep(max(0.0, min(0.002, pending[0]["arrival"] - (time.perf_counter() - t0))))
+    elapsed = time.perf_counter() - t0
+    st = fs.stats()
+    # CPU reference flows for a sample (smallest and largest grids) and the in-mix results
+    order = sorted(range(n_flows), key=lambda i: reqs[i]["n_grid"])
+    sample = sorted({order[0], order[1 % n_flows], order[-1]})
+    x, w = qp.get_points(64), qp.get_weights(64)
+    max_diff, cpu_t = 0.0, []
+    for i in sample:
+        fid, q = ids2[i]
+        u, _ = fs.result(fid)
+        fp = O.flow_params(q["n_grid"], q["rho_max"], q["T"], q["mu"], q["N"], q["m2_0"], q["lambda"])
+        t1 = time.perf_counter()
+        uref = O.flow_lpa(fp, x, w)
+        cpu_t.append((time.perf_counter() - t1, q["n_grid"]))
+        max_diff = max(max_diff, float(np.max(np.abs(u - uref))))
+    fs.close()
+    cpu_flow_s = sum(t for t, _ in cpu_t) / len(cpu_t)
+    return {"flows": n_flows, "saturation_flows_per_s": sat, "offered_flows_per_s": rate,
+            "served_flows_per_s": n_flows / elapsed, "p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"],
+            "max_abs_u_diff_vs_cpu": max_diff, "cpu_reference_flow_s_mean": cpu_flow_s,
+            "cpu_sample": [{"n_grid": n, "seconds": t} for t, n in cpu_t],
+            "config": "O(4) LPA flows, n_grid ~ U{64..1024}, T ~ U[0.01,0.3], mu ~ U[0,0.4], RK4 2000 steps "
+                      "k=1 -> 1e-3, two 64-node threshold integrals per point and stage; HBM budget 64 MiB, "
+                      "64 slots, 25 steps per graph launch"}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
-                                   "DESIGN.md and not implemented", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is described in the "
+                                   "design notes and not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import diffrg_amd as D
     D.init(0)
-    out = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "strong",
-           "vs_baseline": None, "data": "synthetic (SplitMix64 seed 0xD1FF0000+c)"}
-    if args.workload != "selfenergy":
-        raise SystemExit("only the selfenergy workload is wired into bench.py yet")
-    info = run_selfenergy(D, args, out)
-    fpe = FLOP_PER_EVAL["selfenergy"][1]
-    achieved = (fpe * info["evals"] / (info["ms_mixed"] * 1e-3)) if fpe else None
-    out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU, "unit": "FLOP/s",
-                       "frac": (achieved / PEAK_FP64_VALU) if achieved else None, "traffic": None,
-                       "kernel": "se_kernel<MIXED>"}
+    qp = D.QuadratureProvider()
+    se = selfenergy(D, qp, args)
+    n, evals = se["n"], se["evals"]
+    fp, mx = se["res"]["fp64"], se["res"]["mixed"]
+    ok = mx["err"] <= MIXED_TOL
+    out = {
+        "metric": "integrals/sec at rel.err<=1e-8 vs FP64 (mixed vs plain); p50/p99 flow latency",
+        "value": n / (mx["ms"] * 1e-3) if ok else 0.0,
+        "unit": "integrals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": mx["ms"],
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "fp32-seed/fp64-acc", "data": "synthetic (SplitMix64 seed 0xD1FF0000+c)",
+        "config": {"workload": "two-angle self-energy: 64 p on log grid [1e-3,40], nodes {64,32,32}, "
+                               "128-point Z table (4,194,304 evals per step)",
+                   "integrals_per_step": n, "evals_per_step": evals},
+        "mixed": {"integrals_per_s": n / (mx["ms"] * 1e-3), "evals_per_s": evals / (mx["ms"] * 1e-3),
+                  "max_rel_err_vs_fp64_oracle": mx["err"], "shape": mx["shape"]},
+        "plain_fp64": {"integrals_per_s": n / (fp["ms"] * 1e-3), "evals_per_s": evals / (fp["ms"] * 1e-3),
+                       "ms_per_step": fp["ms"], "max_rel_err_vs_oracle": fp["err"], "shape": fp["shape"]},
+        "mixed_over_plain": fp["ms"] / mx["ms"],
+    }
+    achieved = SE_FLOP[1] * evals / (mx["ms"] * 1e-3) / 1e12
+    peak = PEAK_FP64_VALU / 1e12
+    out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                       "frac": achieved / peak, "traffic": measured_traffic(), "kernel": "se_kernel<MIXED>",
+                       "flop_per_eval": SE_FLOP[1], "evals_per_launch": evals,
+                       "valu_issue_frac": valu_issue_frac(1, evals, mx["ms"] * 1e-3),
+                       "plain_fp64": {"flop_per_eval": SE_FLOP[0],
+                                      "achieved": SE_FLOP[0] * evals / (fp["ms"] * 1e-3) / 1e12,
+                                      "frac": SE_FLOP[0] * evals / (fp["ms"] * 1e-3) / PEAK_FP64_VALU,
+                                      "valu_issue_frac": valu_issue_frac(0, evals, fp["ms"] * 1e-3)}}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(info, args.cpu_seconds)
+        t_cpu, runs = cpu_timed(se["ref_fn"], args.cpu_seconds)
+        from oracle import oracle as O
+        out["cpu_baseline"] = {"value": n / t_cpu, "unit": "integrals/s", "cores": O.num_threads(), "kind": "port",
+                               "sample": f"{runs} runs of the full call set ({n} integrals, {evals} evals), C "
+                                         f"restatement of the reference TBB loop (OpenMP), median {t_cpu * 1e3:.1f} ms",
+                               "evals_per_s": evals / t_cpu, "cpu_model": cpu_model()}
+    if not args.only_headline:
+        work = {}
+        for name, fn in (("threshold", threshold), ("finite_T", finite_T), ("uneven", uneven), ("flows", flows)):
+            t0 = time.perf_counter()
+            work[name] = fn(D, qp, args)
+            log(f"{name} done in {time.perf_counter() - t0:.1f} s")
+        out["workloads"] = work
     out["device"] = D.device_props()["name"]
     print(json.dumps(out))
 
