@@ -91,8 +91,8 @@ int dfrg_pool_host_free(void *h_ptr);
 int dfrg_gl_generate(int order, double *d_x, double *d_w, void *stream);
 /* Matsubara rule of MatsubaraQuadrature<double>::reinit(T, E, step, min_size, max_size,
  * vacuum_quad_size, precision_factor) (matsubara.cu:33-85, 128-161), generated on the GPU:
- * Monien's Jacobi matrix, eigenvalues by parallel Sturm bisection, weights from the
- * Christoffel function.  Writes at most cap nodes; *size receives the rule size and *T_rule the
+ * Monien's Jacobi matrix, eigenvalues by parallel Sturm bisection, weights from the first
+ * eigenvector component of a twisted LDL^T/UDU^T factorisation.  Writes at most cap nodes; *size receives the rule size and *T_rule the
  * rule temperature (0 for the vacuum rule). Synchronises `stream`. */
 int dfrg_matsubara_generate(double T, double E, int step, int min_size, int max_size, int vacuum_quad_size,
                             int precision_factor, double *d_x, double *d_w, int cap, int *size, double *T_rule,
