@@ -38,8 +38,8 @@ MIXED_TOL = 1e-8
 # Per integrand evaluation of the innermost loop of se_kernel (FP64, MIXED): FP operations
 # (FMA = 2), VALU instructions and quarter-rate transcendentals, counted from the gfx950
 # assembly of csrc/integrals.hip by `python -m diffrg_amd.isa_count`.
-SE_FLOP = {0: 224, 1: 52}
-SE_VALU = {0: 220, 1: 60}
+SE_FLOP = {0: 223.5, 1: 53.0}
+SE_VALU = {0: 203, 1: 56}
 
 
 def valu_issue_frac(mode, evals, seconds):
@@ -208,13 +208,21 @@ def finite_T(D, qp, args):
     evals = int(sum(c["nx"] * (2 * len(c["rules"][t][0]) + 1) for t in c["T"]))
     r = {"evals_per_step": evals}
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
-        ms, _ = time_steps(D, lambda: D.ft_batch(dev, out, c["q_max"], c["k"], mode, c["d"], c["radial_mode"]),
-                           args.steps, args.warmup)
+        run = lambda: D.ft_batch(dev, out, c["q_max"], c["k"], mode, c["d"], c["radial_mode"])
+        best = None
+        for block in (64, 128, 256, 512):  # threads per integral, timed on this batch
+            D.ft_set_block(block)
+            t, _ = time_steps(D, run, 50, 5)
+            if best is None or t < best[0]:
+                best = (t, block)
+        D.ft_set_block(best[1])
+        ms, _ = time_steps(D, run, args.steps, args.warmup)
         got = out.to_host()
         err = np.abs(got - ref) / np.abs(ref)
         r[name] = {"integrals_per_s": len(c["T"]) / (ms * 1e-3), "evals_per_s": evals / (ms * 1e-3),
-                   "us_per_step": ms * 1e3, "max_rel_err": float(err.max()),
+                   "us_per_step": ms * 1e3, "max_rel_err": float(err.max()), "block": best[1],
                    "max_rel_err_per_T": {str(t): float(err[c["T"] == t].max()) for t in W.TEMPS}}
+    D.ft_set_block(0)
     r["mixed_over_plain"] = r["mixed"]["integrals_per_s"] / r["fp64"]["integrals_per_s"]
     t_cpu, _ = cpu_timed(ref_fn, 2.0)
     r["cpu_integrals_per_s"] = len(c["T"]) / t_cpu

@@ -87,6 +87,7 @@ SIGNATURES = {
     "dfrg_se_batch_autotune": [_vp, C.c_int, C.c_int, C.c_char_p, _ip, _ip, _ip, C.POINTER(C.c_float)],
     "dfrg_se_batch_graph_launch": [_vp, C.c_int, _vp, C.c_int, _vp],
     "dfrg_se_batch_evals": [_vp, C.POINTER(C.c_ulonglong)],
+    "dfrg_ft_set_block": [C.c_int],
     "dfrg_ft_batch": [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
                       C.c_double, C.c_double, C.c_int, _vp, _vp],
     "dfrg_fs_create": [C.POINTER(_vp), _vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int],
@@ -298,6 +299,11 @@ def ft_batch(dev: dict, d_out: DeviceArray, extent, k=1.0, mode=FP64, d=4, radia
                               dev["m1"].ptr, dev["m2"].ptr, dev["mx"].ptr, dev["mw"].ptr, dev["m_off"].ptr,
                               dev["m_n"].ptr, dev["x"].ptr, dev["w"].ptr, dev["x"].n, extent, k, mode, d_out.ptr,
                               stream))
+
+
+def ft_set_block(block: int = 0) -> None:
+    """Threads per integral of ft_batch (64/128/256/512; 0 = automatic)."""
+    check(lib().dfrg_ft_set_block(block))
 
 
 class SEBatch:

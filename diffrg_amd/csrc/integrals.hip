@@ -13,6 +13,7 @@
 #include "dfrg/device.hh"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -107,14 +108,17 @@ namespace
   {
     extern __shared__ double sm[];
     const int nx = A.nx;
-    double *elw = sm;           // element * weight
-    double *rdot = sm + nx;     // RBdot(q^2)
-    double *s1 = sm + 2 * nx;   // q^2 + R(q^2) + m1
-    double *s2 = sm + 3 * nx;   // q^2 + |p|^2 + R(q^2 + |p|^2) + m2
-    double *red = sm + 4 * nx;  // block reduction
+    double *s_elw = sm;          // element * weight (plain) or element * weight * RBdot (mixed)
+    double *s_rdot = sm + nx;    // RBdot(q^2)
+    double *s_a1 = sm + 2 * nx;  // q^2 + R(q^2) + m1
+    double *s_a2 = sm + 3 * nx;  // q^2 + |p|^2 + R(q^2 + |p|^2) + m2
+    double *s_red = sm + 4 * nx; // block reduction
     const int i = blockIdx.x;
     const double k = A.k, k2 = k * k;
     const double T = A.T[i], p0 = A.p0[i], pa = A.pabs[i], m1 = A.m1[i], m2 = A.m2[i];
+    const int off = A.m_off[i], nm = A.m_n[i];
+    double acc = 0.0;
+    // q-only terms, and the zero mode T f(q, 0) of sum = T f(0) + sum_n w_n (f(x_n) + f(-x_n))
     for (int j = threadIdx.x; j < nx; j += BLOCK) {
       double q, el;
       if (A.radial_mode == 0) {
@@ -125,40 +129,17 @@ namespace
         el = A.pref * ipow_dev(q, A.d - 2);
       }
       const double q2 = q * q;
-      elw[j] = el * (A.w[j] * A.extent);
+      const double elw = el * (A.w[j] * A.extent);
       const double xr = q2 / k2;
-      rdot[j] = 2.0 * exp(-xr) * k2 * (1.0 + xr);
-      s1[j] = q2 + k2 * exp(-xr) + m1;
+      const double rdot = 2.0 * exp(-xr) * k2 * (1.0 + xr);
+      const double a1 = q2 + k2 * exp(-xr) + m1;
       const double qp2 = q2 + pa * pa;
-      s2[j] = qp2 + k2 * exp(-(qp2 / k2)) + m2;
-    }
-    __syncthreads();
-    const double *mx = A.mx + A.m_off[i];
-    const double *mw = A.mw + A.m_off[i];
-    const int nm = A.m_n[i];
-    const int work = nx * nm;
-    double acc = 0.0;
-    for (int e = threadIdx.x; e < work; e += BLOCK) {
-      const int j = e / nm;
-      const int n = e - j * nm;
-      const double wn = mx[n];
-      const double a1 = s1[j], a2 = s2[j];
-      const double dp = wn + p0, dm = -wn + p0;
-      // denominators in FP64 in both modes; mixed replaces the FP64 division sequences by an FP32
-      // hardware reciprocal refined with one FP64 Newton step
-      const double d1 = wn * wn + a1, d2p = dp * dp + a2, d2m = dm * dm + a2;
-      double g1, gp, gm;
-      if (MODE == DFRG_FP64) {
-        g1 = 1.0 / d1;
-        gp = 1.0 / d2p;
-        gm = 1.0 / d2m;
-      } else {
-        g1 = dev::rcp_seeded(d1);
-        gp = dev::rcp_seeded(d2p);
-        gm = dev::rcp_seeded(d2m);
-      }
-      double val = mw[n] * (rdot[j] * g1 * g1 * gp + rdot[j] * g1 * g1 * gm);
-      if (T > 0 && n == 0) {
+      const double a2 = qp2 + k2 * exp(-(qp2 / k2)) + m2;
+      s_elw[j] = MODE == DFRG_FP64 ? elw : elw * rdot;
+      s_rdot[j] = rdot;
+      s_a1[j] = a1;
+      s_a2[j] = a2;
+      if (T > 0) {
         double g10, g20;
         if (MODE == DFRG_FP64) {
           g10 = 1.0 / a1;
@@ -167,19 +148,62 @@ namespace
           g10 = dev::rcp_seeded(a1);
           g20 = dev::rcp_seeded(p0 * p0 + a2);
         }
-        val += T * (rdot[j] * g10 * g10 * g20);
+        acc += elw * (T * (rdot * g10 * g10 * g20));
       }
-      acc += elw[j] * val;
     }
-    const double tot = dev::block_sum(acc, red);
+    __syncthreads();
+    // Matsubara nodes across the lanes (fixed per lane), radial nodes across the waves; the
+    // q-only terms are wave-uniform LDS reads
+    constexpr int NW = BLOCK / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int n0 = 0; n0 < nm; n0 += 64) {
+      const int n = n0 + lane;
+      if (n >= nm) break;
+      const double wn = A.mx[off + n], wgt = A.mw[off + n];
+      const double dp = wn + p0, dm = -wn + p0;
+      if (MODE == DFRG_FP64) {
+        // the reference's expression tree: three FP64 divisions per pair
+        double s = 0.0;
+#pragma unroll 2
+        for (int j = wv; j < nx; j += NW) {
+          const double a1 = s_a1[j], a2 = s_a2[j], rd = s_rdot[j];
+          const double g1 = 1.0 / (wn * wn + a1), gp = 1.0 / (dp * dp + a2), gm = 1.0 / (dm * dm + a2);
+          s += s_elw[j] * (wgt * (rd * g1 * g1 * gp + rd * g1 * g1 * gm));
+        }
+        acc += s;
+      } else {
+        // f(q,w) + f(q,-w) = RBdot (d2p + d2m) / (d1^2 d2p d2m): one reciprocal per pair, from an
+        // FP32 hardware seed refined in FP64; denominators and products in FP64
+        const double w2 = wn * wn, dp2 = dp * dp, dm2 = dm * dm;
+        double s = 0.0;
+#pragma unroll 4
+        for (int j = wv; j < nx; j += NW) {
+          const double a2 = s_a2[j];
+          const double d1 = w2 + s_a1[j];
+          const double d2p = dp2 + a2, d2m = dm2 + a2;
+          const double D2 = d2p * d2m;
+          const double r = dev::rcp_seeded(d1 * D2);
+          s = fma(s_elw[j], (d2p + d2m) * (r * (r * D2)), s);
+        }
+        acc = fma(wgt, s, acc);
+      }
+    }
+    const double tot = dev::block_sum(acc, s_red);
     if (threadIdx.x == 0) A.out[i] = tot;
   }
 
   // ============================================================================================
   // C1/C3: two-angle self-energy tile engine
   // ============================================================================================
+  // A tile is (integral, range of radial nodes).  Everything a block needs about its tile sits in
+  // one record, so the block starts with a single load; the table, angular terms and the tile's
+  // radial nodes then come in one round of independent loads into LDS.
   struct SETile {
-    int integral, q0, nq, slot;
+    double p, c, inv_b, scale; // external momentum; log coordinates: start c, 1/b, (n-1)/bias
+    int integral, q0, nqt, slot;
+    int na, l2na, xq_off, xa_off; // node offsets: x at off, w at off + order
+    int nq, tab_off, ntab, first_slot;
+    int n_slots, pad0, pad1, pad2;
   };
 
   struct SECellZ {
@@ -190,20 +214,14 @@ namespace
   struct SEArgs {
     int n_tiles;
     const SETile *tiles;
-    const double *p;
-    const int *nq, *na, *log2na;
-    const int *xq_off, *xa_off; // offsets into nodes: x at off, w at off + n
     const double *nodes;
-    const int *tab_off, *tab_n;
     const double *tab64;  // FP64 tables
     const double2 *tabA;  // mixed-path cell data: (y_i, 1/(b + y_i - c))
     const SECellZ *tabB;  // mixed-path cell data: (Z_i, Z_{i+1} - Z_i)
-    const double *coord;  // per integral: c, inv_b, scale (3 doubles)
-    const int *first_slot, *n_slots;
     double x_extent, k, pref; // pref = 2pi/(2pi)^4
     double *partials;
-    unsigned *counters;
-    unsigned *queue; // [0] next tile, [1] exited blocks (persistent mode)
+    unsigned *counters; // per integral
+    unsigned *queue;    // [0] next tile, [1] exited blocks (persistent mode)
     int persistent;
     double *out;
   };
@@ -212,174 +230,234 @@ namespace
   constexpr int kMaxAng = 64;
   constexpr int kMaxQ = 128;
 
+  // Log-coordinate lerp of the FP64 values of a table (coordinates.hh:182-186 backward,
+  // linear_interpolation_1D.hh:75-92): idx = log1p((y - c)/b) (n-1)/bias clamped to [0, n-1].
+  template <typename V> __device__ __forceinline__ double lerp_log_cells(V val, int n, double c, double inv_b,
+                                                                         double scale, double y)
+  {
+    double idx = log1p((y - c) * inv_b) * scale;
+    idx = fmax(0.0, fmin(idx, (double)(n - 1)));
+    const double fl = floor(idx);
+    const int i0 = (int)fl;
+    const int i1 = (int)ceil(idx);
+    return val(i0) * (1.0 - idx + fl) + val(i1) * (idx - fl);
+  }
+
+  // G(|p-q|) = 1/(Z(|p-q|) |p-q|^2 + R(|p-q|^2)) in FP64, the reference's expression tree.
+  __device__ __forceinline__ double se_G_fp64(const double *tab, int ntab, double c, double inv_b, double scale,
+                                              double k2, double pq2)
+  {
+    const double y = sqrt(pq2);
+    const double Z = dev::lerp_log_f64(tab, ntab, c, inv_b, scale, y);
+    return 1.0 / (Z * pq2 + k2 * exp(-(pq2 / k2)));
+  }
+
+  struct SEMixed {
+    const double2 *tabA;
+    const SECellZ *tabB;
+    const double *exp2t; // 2^(j/64), j = 0..63
+    float cf, ibf, scl, sf;
+    double ek64, k2;
+    int imax;
+  };
+
+  // G(|p-q|) in mixed precision: FP32 hardware seeds replace the FP64 library sequences, FP64 is
+  // kept where the error budget needs it.  sqrt from an FP32 seed and one FP64 Newton step; Z cell
+  // found with the FP32 log2, the in-cell position s*log1p(u) (|u| <~ 0.05) in FP32 with the
+  // rounding of 1+u restored, Z assembled in FP64 from the FP64 node value; R = k^2 2^t with t
+  // split on a 1/64 grid (table) and a degree-4 polynomial remainder in FP64 (t is clamped at
+  // -1020, where R is below 1e-300 of the denominator); the denominator in FP64; the reciprocal
+  // from an FP32 seed refined in FP64.  Per-evaluation error ~1e-9 relative.
+  __device__ __forceinline__ double se_G_mixed(const SEMixed &m, double pq2d)
+  {
+    const float ys = __builtin_amdgcn_sqrtf((float)pq2d);
+    const double ysd = (double)ys;
+    const double yd = fma(fma(-ysd, ysd, pq2d), (double)(0.5f * __builtin_amdgcn_rcpf(ys)), ysd);
+    const float coarse = __builtin_amdgcn_logf(fmaf(ys - m.cf, m.ibf, 1.0f)) * m.scl;
+    const int i0 = min(max((int)__builtin_floorf(coarse), 0), m.imax);
+    const double2 ca = m.tabA[i0];
+    const SECellZ cz = m.tabB[i0];
+    const float u = (float)((yd - ca.x) * ca.y);
+    const float w1 = 1.0f + u;
+    const float ru = u - (w1 - 1.0f); // rounding error of 1 + u
+    float fr = fmaf(__builtin_amdgcn_logf(w1), m.scl, ru * m.sf);
+    fr = fmaxf(0.0f, fminf(fr, 1.0f)); // NaN-safe; realises the [0, n-1] clamp
+    const double Z = cz.z + (double)(fr * cz.dz);
+    const double t64 = fmax(pq2d * m.ek64, -64.0 * 1020.0); // 64 t, t = -|p-q|^2/k^2 log2(e)
+    const double tn = rint(t64);
+    const int jn = (int)tn;
+    const double r = (t64 - tn) * (kLn2 / 64.0);
+    const double pe = fma(fma(fma(fma(r, 1.0 / 24.0, 1.0 / 6.0), r, 0.5), r, 1.0), r, 1.0);
+    const double R = m.k2 * ldexp(pe * m.exp2t[jn & 63], jn >> 6);
+    return dev::rcp_seeded(fma(Z, pq2d, R));
+  }
+
+  // Sum over the block's share of (cos1, cos2) pairs and radial nodes of one tile.  Lanes own
+  // pairs; with two or more pairs per lane, each radial step evaluates two pairs that share the
+  // LDS reads of the q terms; with one pair per lane, lanes split the radial nodes and each step
+  // evaluates two of them.  Two independent chains per step hide the LDS latency.
+  template <int BLOCK, typename G>
+  __device__ __forceinline__ double se_tile_sum(const G &g, int na, int l2, int nqt, const double2 *s_ang,
+                                                const double *s_w2, const double *s_qs, const double *s_qc,
+                                                const double *s_qt)
+  {
+    const int npairs = na * na;
+    const int pstride = npairs < BLOCK ? npairs : BLOCK;
+    const int qlanes = BLOCK / pstride;
+    const int pr0 = threadIdx.x & (pstride - 1);
+    const int qs0 = threadIdx.x / pstride;
+    double acc = 0.0;
+    if (2 * pstride <= npairs) {
+      for (int pr = pr0; pr < npairs; pr += 2 * pstride) {
+        const int prb = pr + pstride;
+        const double2 a = s_ang[pr >> l2], b = s_ang[prb >> l2];
+        double sa = 0.0, sb = 0.0;
+        for (int j = 0; j < nqt; ++j) {
+          const double qs = s_qs[j], qc = s_qc[j], qt = s_qt[j];
+          const double ga = g(fma(qc, a.x, qs));
+          const double gb = g(fma(qc, b.x, qs));
+          sa = fma(qt, ga, sa);
+          sb = fma(qt, gb, sb);
+        }
+        acc += a.y * s_w2[pr & (na - 1)] * sa;
+        acc += b.y * s_w2[prb & (na - 1)] * sb;
+      }
+    } else {
+      const double2 a = s_ang[pr0 >> l2];
+      double sa = 0.0, sb = 0.0;
+      int j = qs0;
+      for (; j + qlanes < nqt; j += 2 * qlanes) {
+        const int j2 = j + qlanes;
+        const double ga = g(fma(s_qc[j], a.x, s_qs[j]));
+        const double gb = g(fma(s_qc[j2], a.x, s_qs[j2]));
+        sa = fma(s_qt[j], ga, sa);
+        sb = fma(s_qt[j2], gb, sb);
+      }
+      if (j < nqt) sa = fma(s_qt[j], g(fma(s_qc[j], a.x, s_qs[j])), sa);
+      acc = a.y * s_w2[pr0 & (na - 1)] * (sa + sb);
+    }
+    return acc;
+  }
+
   template <int MODE, int BLOCK> __global__ __launch_bounds__(BLOCK) void se_kernel(SEArgs A)
   {
-    __shared__ double s_tab64[MODE == DFRG_FP64 ? kMaxTab : 1];
-    __shared__ double2 s_tabA[MODE == DFRG_FP64 ? 1 : kMaxTab];
-    __shared__ SECellZ s_tabB[MODE == DFRG_FP64 ? 1 : kMaxTab];
-    __shared__ double s_exp2[MODE == DFRG_FP64 ? 1 : 64]; // 2^(j/64), j = 0..63
-    __shared__ double s_cos1[kMaxAng], s_aw1[kMaxAng], s_w2[kMaxAng];
-    __shared__ double s_qt[kMaxQ], s_qs[kMaxQ], s_qc[kMaxQ];
+    constexpr bool F64 = MODE == DFRG_FP64;
+    __shared__ double s_tab64[F64 ? kMaxTab : 1];
+    __shared__ double2 s_tabA[F64 ? 1 : kMaxTab];
+    __shared__ SECellZ s_tabB[F64 ? 1 : kMaxTab];
+    __shared__ double s_exp2[F64 ? 1 : 64];
+    __shared__ double2 s_ang[kMaxAng]; // (cos1, 2 w1 sqrt(1 - cos1^2))
+    __shared__ double s_w2[kMaxAng];   // 2 w2
+    __shared__ double s_qs[kMaxQ], s_qc[kMaxQ], s_qt[kMaxQ];
     __shared__ double s_red[BLOCK / 64];
-    __shared__ int s_tile, s_cur;
+    __shared__ int s_tile;
 
-    if (threadIdx.x == 0) s_cur = -1;
-    if (MODE != DFRG_FP64)
+    if (!F64)
       for (int j = threadIdx.x; j < 64; j += BLOCK) s_exp2[j] = exp2(j * (1.0 / 64.0));
     const double k = A.k, k2 = k * k;
-    int iter = 0;
-    for (;;) {
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        if (A.persistent)
-          s_tile = (int)atomicAdd(&A.queue[0], 1u);
-        else
-          s_tile = (iter == 0) ? (int)blockIdx.x : A.n_tiles;
-      }
-      __syncthreads();
-      const int t = s_tile;
-      ++iter;
-      if (t >= A.n_tiles) break;
-      const SETile tile = A.tiles[t];
-      const int i = tile.integral;
-      const int na = A.na[i], l2 = A.log2na[i];
-      const double p = A.p[i];
-      const double c = A.coord[3 * i], inv_b = A.coord[3 * i + 1], scale = A.coord[3 * i + 2];
-      const int ntab = A.tab_n[i];
-      const double *xa = A.nodes + A.xa_off[i];
-      const double *wa = xa + na;
-      if (s_cur != i) {
-        // stage this integral's table and angular terms
-        if (MODE == DFRG_FP64) {
-          for (int j = threadIdx.x; j < ntab; j += BLOCK) s_tab64[j] = A.tab64[A.tab_off[i] + j];
-        } else {
-          for (int j = threadIdx.x; j < ntab; j += BLOCK) {
-            s_tabA[j] = A.tabA[A.tab_off[i] + j];
-            s_tabB[j] = A.tabB[A.tab_off[i] + j];
-          }
-        }
-        for (int j = threadIdx.x; j < na; j += BLOCK) {
-          const double c1 = 2.0 * (xa[j] - 0.5);
-          s_cos1[j] = c1;
-          s_aw1[j] = 2.0 * wa[j] * sqrt(1.0 - c1 * c1); // cos1 weight and Jacobian
-          s_w2[j] = 2.0 * wa[j];                          // cos2 weight
-        }
-      }
-      // q-only terms of this tile (FP64): el * w_q x_ext * RBdot(q) G(q)^2, p^2 + q^2, -2 p q
-      {
-        const double *xq = A.nodes + A.xq_off[i];
-        const double *wq = xq + A.nq[i];
-        for (int j = threadIdx.x; j < tile.nq; j += BLOCK) {
-          const int iq = tile.q0 + j;
-          const double q = k * sqrt(xq[iq] * A.x_extent);
-          const double q2 = q * q;
-          const double xr = q2 / k2;
-          const double ex = exp(-xr);
-          // read from HBM: the LDS copy of the table may still be in flight here
-          const double Zq = dev::lerp_log_f64(A.tab64 + A.tab_off[i], ntab, c, inv_b, scale, q);
-          const double Gq = 1.0 / (Zq * q2 + k2 * ex);
-          const double el = A.pref * (q2 * 0.5 * k2);
-          s_qt[j] = el * (wq[iq] * A.x_extent) * (2.0 * ex * k2 * (1.0 + xr)) * Gq * Gq;
-          s_qs[j] = p * p + q2;
-          s_qc[j] = -2.0 * p * q;
-        }
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) s_cur = i;
-
-      const int npairs = na * na;
-      const int pstride = npairs < BLOCK ? npairs : BLOCK;
-      const int qlanes = BLOCK / pstride;
-      const int pr0 = threadIdx.x % pstride;
-      const int qs0 = threadIdx.x / pstride;
-      const int nqt = tile.nq;
-      double acc = 0.0;
-      if (MODE == DFRG_FP64) {
-        for (int pr = pr0; pr < npairs; pr += pstride) {
-          const int i1 = pr >> l2, i2 = pr & (na - 1);
-          const double c1 = s_cos1[i1];
-          const double aw = s_aw1[i1] * s_w2[i2];
-          double sq = 0.0;
-          for (int j = qs0; j < nqt; j += qlanes) {
-            const double pq2 = fma(s_qc[j], c1, s_qs[j]);
-            const double y = sqrt(pq2);
-            const double Z = dev::lerp_log_f64(s_tab64, ntab, c, inv_b, scale, y);
-            const double G = 1.0 / (Z * pq2 + k2 * exp(-(pq2 / k2)));
-            sq += s_qt[j] * G;
-          }
-          acc += aw * sq;
-        }
+    int cur = -1; // integral whose table and angular terms are in LDS (block-uniform)
+    for (int iter = 0;; ++iter) {
+      int t;
+      if (A.persistent) {
+        __syncthreads(); // the previous tile is finished with s_tile
+        if (threadIdx.x == 0) s_tile = (int)atomicAdd(&A.queue[0], 1u);
+        __syncthreads();
+        t = __builtin_amdgcn_readfirstlane(s_tile);
       } else {
-        // Mixed: FP32 hardware seeds replace the FP64 library sequences; FP64 is kept where the
-        // error budget needs it.  |p-q|^2 in FP64 (cancellation); sqrt from an FP32 seed and one
-        // FP64 Newton step; Z cell found with the FP32 log2, the in-cell position s*log1p(u)
-        // (|u| <~ 0.2) in FP32 with the rounding of 1+u restored, Z assembled in FP64 from the
-        // FP64 node value; R = k^2 2^t with t split on a 1/64 grid (table) and a degree-4
-        // polynomial remainder in FP64; the denominator in FP64; the reciprocal from an FP32 seed
-        // refined in FP64.  Per-evaluation error ~1e-9 relative.
-        const float cf = (float)c, ibf = (float)inv_b, scl = (float)(scale * kLn2), sf = (float)scale;
-        const double ek64 = -64.0 * kLog2e / k2; // t * 64
-        const int imax = ntab - 2;
-        for (int pr = pr0; pr < npairs; pr += pstride) {
-          const int i1 = pr >> l2, i2 = pr & (na - 1);
-          const double c1 = s_cos1[i1];
-          const double aw = s_aw1[i1] * s_w2[i2];
-          double sq = 0.0;
-          for (int j = qs0; j < nqt; j += qlanes) {
-            const double pq2d = fma(s_qc[j], c1, s_qs[j]);
-            const float ys = __builtin_amdgcn_sqrtf((float)pq2d);
-            const double ysd = (double)ys;
-            const double yd = fma(fma(-ysd, ysd, pq2d), (double)(0.5f * __builtin_amdgcn_rcpf(ys)), ysd);
-            const float coarse = __builtin_amdgcn_logf(fmaf(ys - cf, ibf, 1.0f)) * scl;
-            const int i0 = min(max((int)__builtin_floorf(coarse), 0), imax);
-            const double2 ca = s_tabA[i0];
-            const SECellZ cz = s_tabB[i0];
-            const float u = (float)((yd - ca.x) * ca.y);
-            const float w1 = 1.0f + u;
-            const float ru = u - (w1 - 1.0f); // rounding error of 1 + u
-            float fr = fmaf(__builtin_amdgcn_logf(w1), scl, ru * sf);
-            fr = fmaxf(0.0f, fminf(fr, 1.0f)); // NaN-safe; realises the [0, n-1] clamp
-            const double Z = cz.z + (double)(fr * cz.dz);
-            // R = k^2 2^(t), t = -pq2/k^2 log2(e) <= 0
-            const double t64 = pq2d * ek64;
-            double R = 0.0;
-            if (t64 > -64.0 * 1000.0) {
-              const double tn = rint(t64);
-              const int jn = (int)tn;
-              const double r = (t64 - tn) * (kLn2 / 64.0);
-              const double pe = fma(fma(fma(fma(r, 1.0 / 24.0, 1.0 / 6.0), r, 0.5), r, 1.0), r, 1.0);
-              R = k2 * ldexp(pe * s_exp2[jn & 63], jn >> 6);
-            }
-            const double G = dev::rcp_seeded(fma(Z, pq2d, R));
-            sq = fma(s_qt[j], G, sq);
+        t = iter == 0 ? (int)blockIdx.x : A.n_tiles;
+      }
+      if (t >= A.n_tiles) break;
+      const SETile T = A.tiles[t];
+      const int i = T.integral;
+      // one round of independent loads: table and angular terms (when the integral changes) and
+      // the radial nodes of this tile
+      if (i != cur) {
+        if (F64) {
+          for (int j = threadIdx.x; j < T.ntab; j += BLOCK) s_tab64[j] = A.tab64[T.tab_off + j];
+        } else {
+          for (int j = threadIdx.x; j < T.ntab; j += BLOCK) {
+            s_tabA[j] = A.tabA[T.tab_off + j];
+            s_tabB[j] = A.tabB[T.tab_off + j];
           }
-          acc += aw * sq;
         }
+        const double *xa = A.nodes + T.xa_off;
+        for (int j = threadIdx.x; j < T.na; j += BLOCK) {
+          const double c1 = 2.0 * (xa[j] - 0.5);
+          const double wa = xa[T.na + j];
+          s_ang[j] = make_double2(c1, 2.0 * wa * sqrt(1.0 - c1 * c1)); // cos1 weight and Jacobian
+          s_w2[j] = 2.0 * wa;                                           // cos2 weight
+        }
+      }
+      const double *xq = A.nodes + T.xq_off;
+      for (int j = threadIdx.x; j < T.nqt; j += BLOCK) {
+        const double q = k * sqrt(xq[T.q0 + j] * A.x_extent);
+        s_qs[j] = T.p * T.p + q * q;
+        s_qc[j] = -2.0 * T.p * q;
+      }
+      __syncthreads();
+      // q-only factor in FP64 from the staged table: el * w_q x_ext * RBdot(q) G(q)^2
+      for (int j = threadIdx.x; j < T.nqt; j += BLOCK) {
+        const int iq = T.q0 + j;
+        const double q = k * sqrt(xq[iq] * A.x_extent);
+        const double q2 = q * q;
+        const double xr = q2 / k2;
+        const double ex = exp(-xr);
+        double Zq;
+        if (F64)
+          Zq = dev::lerp_log_f64(s_tab64, T.ntab, T.c, T.inv_b, T.scale, q);
+        else
+          Zq = lerp_log_cells([&](int m) { return s_tabB[m].z; }, T.ntab, T.c, T.inv_b, T.scale, q);
+        const double Gq = 1.0 / (Zq * q2 + k2 * ex);
+        const double el = A.pref * (q2 * 0.5 * k2);
+        s_qt[j] = el * (xq[T.nq + iq] * A.x_extent) * (2.0 * ex * k2 * (1.0 + xr)) * Gq * Gq;
+      }
+      cur = i;
+      __syncthreads();
+
+      double acc;
+      if (F64) {
+        const auto g = [&](double pq2) { return se_G_fp64(s_tab64, T.ntab, T.c, T.inv_b, T.scale, k2, pq2); };
+        acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
+      } else {
+        SEMixed m;
+        m.tabA = s_tabA;
+        m.tabB = s_tabB;
+        m.exp2t = s_exp2;
+        m.cf = (float)T.c;
+        m.ibf = (float)T.inv_b;
+        m.scl = (float)(T.scale * kLn2);
+        m.sf = (float)T.scale;
+        m.ek64 = -64.0 * kLog2e / k2;
+        m.k2 = k2;
+        m.imax = T.ntab - 2;
+        const auto g = [&](double pq2) { return se_G_mixed(m, pq2); };
+        acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
       }
       const double part = dev::block_sum(acc, s_red);
       double total = 0.0;
-      if (dev::finish_group(part, A.partials, tile.slot, A.first_slot[i], A.n_slots[i], &A.counters[i], &total)) {
+      if (dev::finish_group(part, A.partials, T.slot, T.first_slot, T.n_slots, &A.counters[i], &total)) {
         if (threadIdx.x == 0) A.out[i] = total;
       }
     }
     // persistent mode: the last block out resets the queue for the next launch / replay
+    // (every earlier queue access of this block is an atomic whose result has been used)
     if (A.persistent && threadIdx.x == 0) {
-      __threadfence();
-      const unsigned done = atomicAdd(&A.queue[1], 1u);
+      const unsigned done = dev::take_ticket(&A.queue[1]);
       if (done == gridDim.x - 1) {
-        atomicExch(&A.queue[0], 0u);
-        atomicExch(&A.queue[1], 0u);
+        dev::reset_counter(&A.queue[0]);
+        dev::reset_counter(&A.queue[1]);
       }
     }
   }
 
-  template <int MODE> hipError_t launch_se(int block, int grid, size_t, hipStream_t s, const SEArgs &a)
+  template <int MODE> hipError_t launch_se(int block, int grid, hipStream_t s, const SEArgs &a)
   {
     switch (block) {
     case 64: hipLaunchKernelGGL((se_kernel<MODE, 64>), dim3(grid), dim3(64), 0, s, a); break;
     case 128: hipLaunchKernelGGL((se_kernel<MODE, 128>), dim3(grid), dim3(128), 0, s, a); break;
     case 256: hipLaunchKernelGGL((se_kernel<MODE, 256>), dim3(grid), dim3(256), 0, s, a); break;
     case 512: hipLaunchKernelGGL((se_kernel<MODE, 512>), dim3(grid), dim3(512), 0, s, a); break;
+    case 1024: hipLaunchKernelGGL((se_kernel<MODE, 1024>), dim3(grid), dim3(1024), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -392,6 +470,8 @@ namespace
     return l;
   }
   bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+  std::atomic<int> g_ft_block{0}; // threads per integral of dfrg_ft_batch, 0 = automatic
 } // namespace
 
 // ----------------------------------------------------------------------------------------------
@@ -401,14 +481,14 @@ struct dfrg_se_batch {
   dfrg_qp *qp = nullptr;
   int n_int = 0;
   double x_extent = 20.0, k = 1.0;
-  std::vector<int> nq, na;
-  std::vector<double> p;
+  // per integral (host)
+  std::vector<int> nq, na, xq_off, xa_off, tab_off, tab_n;
+  std::vector<double> p, coord; // coord: c, 1/b, (n-1)/bias
   unsigned long long evals = 0;
   // device data
-  double *d_p = nullptr, *d_nodes = nullptr, *d_tab64 = nullptr, *d_coord = nullptr;
+  double *d_nodes = nullptr, *d_tab64 = nullptr;
   double2 *d_tabA = nullptr;
   SECellZ *d_tabB = nullptr;
-  int *d_ints = nullptr; // nq, na, log2na, xq_off, xa_off, tab_off, tab_n, first_slot, n_slots
   SETile *d_tiles = nullptr;
   double *d_partials = nullptr;
   unsigned *d_counters = nullptr, *d_queue = nullptr;
@@ -436,55 +516,58 @@ namespace
       return (long long)b->nq[a] * b->na[a] * b->na[a] > (long long)b->nq[c] * b->na[c] * b->na[c];
     });
     std::vector<SETile> tiles;
-    std::vector<int> first(n), nslots(n);
     int slot = 0;
     for (int i : order) {
       const int per_q = b->na[i] * b->na[i];
       int qpt = std::max(1, std::min(b->nq[i], b->tile_evals / per_q));
       qpt = std::min(qpt, kMaxQ);
-      first[i] = slot;
-      int cnt = 0;
+      const int first = slot;
+      const int cnt = (b->nq[i] + qpt - 1) / qpt;
       for (int q0 = 0; q0 < b->nq[i]; q0 += qpt) {
-        tiles.push_back({i, q0, std::min(qpt, b->nq[i] - q0), slot++});
-        ++cnt;
+        SETile t{};
+        t.p = b->p[i];
+        t.c = b->coord[3 * i];
+        t.inv_b = b->coord[3 * i + 1];
+        t.scale = b->coord[3 * i + 2];
+        t.integral = i;
+        t.q0 = q0;
+        t.nqt = std::min(qpt, b->nq[i] - q0);
+        t.slot = slot++;
+        t.na = b->na[i];
+        t.l2na = ilog2(b->na[i]);
+        t.xq_off = b->xq_off[i];
+        t.xa_off = b->xa_off[i];
+        t.nq = b->nq[i];
+        t.tab_off = b->tab_off[i];
+        t.ntab = b->tab_n[i];
+        t.first_slot = first;
+        t.n_slots = cnt;
+        tiles.push_back(t);
       }
-      nslots[i] = cnt;
     }
     b->n_tiles = (int)tiles.size();
     if ((size_t)b->n_tiles > b->tiles_cap) {
       hipFree(b->d_tiles);
       hipFree(b->d_partials);
+      b->d_tiles = nullptr;
+      b->d_partials = nullptr;
       b->tiles_cap = b->n_tiles;
       DFRG_HIP(hipMalloc((void **)&b->d_tiles, sizeof(SETile) * b->tiles_cap));
       DFRG_HIP(hipMalloc((void **)&b->d_partials, sizeof(double) * b->tiles_cap));
     }
     DFRG_HIP(hipMemcpy(b->d_tiles, tiles.data(), sizeof(SETile) * tiles.size(), hipMemcpyHostToDevice));
-    DFRG_HIP(hipMemcpy(b->d_ints + 7 * n, first.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    DFRG_HIP(hipMemcpy(b->d_ints + 8 * n, nslots.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     return DFRG_OK;
   }
 
   SEArgs se_args(dfrg_se_batch *b, double *d_out)
   {
-    const int n = b->n_int;
     SEArgs a;
     a.n_tiles = b->n_tiles;
     a.tiles = b->d_tiles;
-    a.p = b->d_p;
-    a.nq = b->d_ints;
-    a.na = b->d_ints + n;
-    a.log2na = b->d_ints + 2 * n;
-    a.xq_off = b->d_ints + 3 * n;
-    a.xa_off = b->d_ints + 4 * n;
-    a.tab_off = b->d_ints + 5 * n;
-    a.tab_n = b->d_ints + 6 * n;
-    a.first_slot = b->d_ints + 7 * n;
-    a.n_slots = b->d_ints + 8 * n;
     a.nodes = b->d_nodes;
     a.tab64 = b->d_tab64;
     a.tabA = b->d_tabA;
     a.tabB = b->d_tabB;
-    a.coord = b->d_coord;
     a.x_extent = b->x_extent;
     a.k = b->k;
     a.pref = 2.0 * kPi / ((2.0 * kPi) * ((2.0 * kPi) * ((2.0 * kPi) * (2.0 * kPi))));
@@ -500,8 +583,8 @@ namespace
   {
     const SEArgs a = se_args(b, d_out);
     const int grid = b->grid > 0 ? std::min(b->grid, b->n_tiles) : b->n_tiles;
-    hipError_t e = mode == DFRG_FP64 ? launch_se<DFRG_FP64>(b->block, grid, 0, s, a)
-                                     : launch_se<DFRG_MIXED>(b->block, grid, 0, s, a);
+    hipError_t e = mode == DFRG_FP64 ? launch_se<DFRG_FP64>(b->block, grid, s, a)
+                                     : launch_se<DFRG_MIXED>(b->block, grid, s, a);
     if (e != hipSuccess) {
       set_error("launch of se_kernel failed: %s", hipGetErrorString(e));
       return DFRG_ERR_HIP;
@@ -545,6 +628,14 @@ int dfrg_threshold_batch(int d, const double *d_m2, int n_int, const double *d_x
   return DFRG_OK;
 }
 
+int dfrg_ft_set_block(int block)
+{
+  DFRG_CHECK_ARG(block == 0 || block == 64 || block == 128 || block == 256 || block == 512,
+                 "block %d not in {0,64,128,256,512}", block);
+  g_ft_block = block;
+  return DFRG_OK;
+}
+
 int dfrg_ft_batch(int d, int radial_mode, int n_int, const double *d_T, const double *d_p0, const double *d_pabs,
                   const double *d_m1, const double *d_m2, const double *d_mx, const double *d_mw, const int *d_m_off,
                   const int *d_m_n, const double *d_x, const double *d_w, int nx, double extent, double k, int mode,
@@ -559,13 +650,32 @@ int dfrg_ft_batch(int d, int radial_mode, int n_int, const double *d_T, const do
   FTArgs a{d, radial_mode, n_int, nx, d_T, d_p0, d_pabs, d_m1, d_m2, d_mx, d_mw, d_m_off, d_m_n, d_x, d_w,
            extent, k, 0.0, d_out};
   a.pref = S_d_host(d - 1) / std::pow(2.0 * kPi, d - 1);
-  constexpr int B = 256;
-  const size_t shm = sizeof(double) * (4 * nx + B / 64);
+  // threads per integral: the caller's choice (dfrg_ft_set_block), else enough waves for about
+  // two per SIMD over the whole batch, 1 to 8 per integral and not more than radial nodes
+  int B = g_ft_block;
+  if (B == 0) {
+    const int simds = 4 * (context().cu_count > 0 ? context().cu_count : 256);
+    int nw = 1;
+    while (nw < 8 && nw * 2 <= nx && (long long)n_int * nw < 2LL * simds) nw *= 2;
+    B = 64 * nw;
+  }
   hipStream_t s = resolve_stream(stream);
-  if (mode == DFRG_FP64)
-    hipLaunchKernelGGL((ft_kernel<DFRG_FP64, B>), dim3(n_int), dim3(B), shm, s, a);
-  else
-    hipLaunchKernelGGL((ft_kernel<DFRG_MIXED, B>), dim3(n_int), dim3(B), shm, s, a);
+  const size_t shm = sizeof(double) * (4 * nx + B / 64);
+  if (mode == DFRG_FP64) {
+    switch (B) {
+    case 64: hipLaunchKernelGGL((ft_kernel<DFRG_FP64, 64>), dim3(n_int), dim3(64), shm, s, a); break;
+    case 128: hipLaunchKernelGGL((ft_kernel<DFRG_FP64, 128>), dim3(n_int), dim3(128), shm, s, a); break;
+    case 256: hipLaunchKernelGGL((ft_kernel<DFRG_FP64, 256>), dim3(n_int), dim3(256), shm, s, a); break;
+    default: hipLaunchKernelGGL((ft_kernel<DFRG_FP64, 512>), dim3(n_int), dim3(512), shm, s, a); break;
+    }
+  } else {
+    switch (B) {
+    case 64: hipLaunchKernelGGL((ft_kernel<DFRG_MIXED, 64>), dim3(n_int), dim3(64), shm, s, a); break;
+    case 128: hipLaunchKernelGGL((ft_kernel<DFRG_MIXED, 128>), dim3(n_int), dim3(128), shm, s, a); break;
+    case 256: hipLaunchKernelGGL((ft_kernel<DFRG_MIXED, 256>), dim3(n_int), dim3(256), shm, s, a); break;
+    default: hipLaunchKernelGGL((ft_kernel<DFRG_MIXED, 512>), dim3(n_int), dim3(512), shm, s, a); break;
+    }
+  }
   DFRG_LAUNCHED("ft_kernel");
   return DFRG_OK;
 }
@@ -617,21 +727,19 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
     nodes.insert(nodes.end(), (const double *)hx, (const double *)hx + o);
     nodes.insert(nodes.end(), (const double *)hw, (const double *)hw + o);
   }
-  std::vector<int> ints(9 * n_int, 0);
-  std::vector<double> coord(3 * n_int);
+  b->xq_off.resize(n_int);
+  b->xa_off.resize(n_int);
+  b->tab_off.assign(tab_off, tab_off + n_int);
+  b->tab_n.assign(tab_n, tab_n + n_int);
+  b->coord.resize(3 * n_int);
   for (int i = 0; i < n_int; ++i) {
-    ints[i] = nq[i];
-    ints[n_int + i] = na[i];
-    ints[2 * n_int + i] = ilog2(na[i]);
-    ints[3 * n_int + i] = off_of[nq[i]];
-    ints[4 * n_int + i] = off_of[na[i]];
-    ints[5 * n_int + i] = tab_off[i];
-    ints[6 * n_int + i] = tab_n[i];
+    b->xq_off[i] = off_of[nq[i]];
+    b->xa_off[i] = off_of[na[i]];
     // LogarithmicCoordinates1D: b = (stop - start)/expm1(bias); idx = log1p((y-c)/b) (n-1)/bias
     const double bb = (stop[i] - start[i]) / std::expm1(bias[i]);
-    coord[3 * i] = start[i];
-    coord[3 * i + 1] = 1.0 / bb;
-    coord[3 * i + 2] = (tab_n[i] - 1.0) / bias[i];
+    b->coord[3 * i] = start[i];
+    b->coord[3 * i + 1] = 1.0 / bb;
+    b->coord[3 * i + 2] = (tab_n[i] - 1.0) / bias[i];
     b->evals += (unsigned long long)nq[i] * na[i] * na[i];
   }
   // cell data of the mixed path (coordinates.hh:141-196 LogarithmicCoordinates1D): node
@@ -661,22 +769,16 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
       return fail(DFRG_ERR_HIP);                                                                               \
     }                                                                                                          \
   } while (0)
-  DFRG_HIP_B(hipMalloc((void **)&b->d_p, sizeof(double) * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_nodes, sizeof(double) * nodes.size()));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tab64, sizeof(double) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tabA, sizeof(double2) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tabB, sizeof(SECellZ) * tables_len));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_coord, sizeof(double) * 3 * n_int));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_ints, sizeof(int) * 9 * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_counters, sizeof(unsigned) * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_queue, sizeof(unsigned) * 2));
-  DFRG_HIP_B(hipMemcpy(b->d_p, p, sizeof(double) * n_int, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tab64, tables, sizeof(double) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tabA, tabA.data(), sizeof(double2) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tabB, tabB.data(), sizeof(SECellZ) * tables_len, hipMemcpyHostToDevice));
-  DFRG_HIP_B(hipMemcpy(b->d_coord, coord.data(), sizeof(double) * 3 * n_int, hipMemcpyHostToDevice));
-  DFRG_HIP_B(hipMemcpy(b->d_ints, ints.data(), sizeof(int) * 9 * n_int, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemset(b->d_counters, 0, sizeof(unsigned) * n_int));
   DFRG_HIP_B(hipMemset(b->d_queue, 0, sizeof(unsigned) * 2));
 #undef DFRG_HIP_B
@@ -703,13 +805,10 @@ int dfrg_se_batch_destroy(dfrg_se_batch *b)
   if (!b) return DFRG_OK;
   hipDeviceSynchronize();
   se_drop_graph(b);
-  hipFree(b->d_p);
   hipFree(b->d_nodes);
   hipFree(b->d_tab64);
   hipFree(b->d_tabA);
   hipFree(b->d_tabB);
-  hipFree(b->d_coord);
-  hipFree(b->d_ints);
   hipFree(b->d_tiles);
   hipFree(b->d_partials);
   hipFree(b->d_counters);
@@ -722,8 +821,8 @@ int dfrg_se_batch_set_shape(dfrg_se_batch *b, int block, int tile_evals, int gri
 {
   DFRG_TRY(require_init());
   DFRG_CHECK_ARG(b != nullptr, "NULL batch");
-  DFRG_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512, "block %d not in {64,128,256,512}",
-                 block);
+  DFRG_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512 || block == 1024,
+                 "block %d not in {64,128,256,512,1024}", block);
   DFRG_CHECK_ARG(tile_evals >= 1, "tile_evals must be positive");
   DFRG_CHECK_ARG(grid >= 0, "grid must be >= 0");
   DFRG_HIP(hipDeviceSynchronize());
@@ -831,14 +930,14 @@ int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cac
   DFRG_HIP(hipEventCreate(&e0));
   DFRG_HIP(hipEventCreate(&e1));
   const int cus = context().cu_count > 0 ? context().cu_count : 256;
-  const int blocks[] = {128, 256, 512};
+  const int blocks[] = {128, 256, 512, 1024};
   const int tiles[] = {1024, 2048, 4096, 8192, 16384, 32768};
   float best = 1e30f;
   int bb = b->block, bt = b->tile_evals, bg = b->grid;
   for (int bl : blocks) {
     for (int te : tiles) {
       for (int gmul : {0, 2, 4, 8}) {
-        const int gr = gmul == 0 ? 0 : gmul * cus * (512 / bl);
+        const int gr = gmul == 0 ? 0 : gmul * cus * 512 / bl;
         if (dfrg_se_batch_set_shape(b, bl, te, gr) != DFRG_OK) continue;
         se_launch(b, mode, d_out, s); // warm-up
         hipEventRecord(e0, s);

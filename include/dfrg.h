@@ -92,8 +92,9 @@ int dfrg_gl_generate(int order, double *d_x, double *d_w, void *stream);
 /* Matsubara rule of MatsubaraQuadrature<double>::reinit(T, E, step, min_size, max_size,
  * vacuum_quad_size, precision_factor) (matsubara.cu:33-85, 128-161), generated on the GPU:
  * Monien's Jacobi matrix, eigenvalues by parallel Sturm bisection, weights from the first
- * eigenvector component of a twisted LDL^T/UDU^T factorisation.  Writes at most cap nodes; *size receives the rule size and *T_rule the
- * rule temperature (0 for the vacuum rule). Synchronises `stream`. */
+ * eigenvector component of a twisted LDL^T/UDU^T factorisation.  Writes at most cap nodes;
+ * *size receives the rule size and *T_rule the rule temperature (0 for the vacuum rule).
+ * Synchronises `stream`. */
 int dfrg_matsubara_generate(double T, double E, int step, int min_size, int max_size, int vacuum_quad_size,
                             int precision_factor, double *d_x, double *d_w, int cap, int *size, double *T_rule,
                             void *stream);
@@ -162,6 +163,11 @@ int dfrg_ft_batch(int d, int radial_mode, int n_int, const double *d_T, const do
                   const double *d_m1, const double *d_m2, const double *d_mx, const double *d_mw, const int *d_m_off,
                   const int *d_m_n, const double *d_x, const double *d_w, int nx, double extent, double k, int mode,
                   double *d_out, void *stream);
+
+/* Threads per integral of dfrg_ft_batch (64, 128, 256 or 512; 0 = automatic: about two waves
+ * per SIMD over the batch).  Process-wide; replaces the max_block_size constructor argument of
+ * IntegratorFiniteTq0GPU (integrator_finiteTq0_gpu.hh:53-68) for the batched form. */
+int dfrg_ft_set_block(int block);
 
 /* ------------------------------------------------------------------------------------------ */
 /* LPA flow server: continuous batching of O(N) LPA flows with an on-device RK4 stepper.       */

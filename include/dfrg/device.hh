@@ -71,31 +71,66 @@ namespace dfrg
     };
 
     // ------------------------------------------------------------------------------------------
-    // Deterministic cross-block reduction.  Each block writes its partial, publishes it with a
-    // release fence and takes a ticket; the block that takes the last ticket of a group sums all
-    // partials of that group in index order (compensated) and resets the group's counter so the
-    // next launch (or graph replay) starts from zero.  Returns true in the finishing block, with
-    // the group total in *total (valid in thread 0).
+    // Cross-block publication without cache-wide fences.  Each XCD has its own L2, so a release
+    // fence at device scope writes back the whole L2 and an acquire fence invalidates it, which
+    // costs microseconds per block.  Partials are instead written and read with device-scope
+    // relaxed atomics, which are coherent across the XCDs by themselves; waiting for the store
+    // counter before the ticket increment makes the published value complete before any block
+    // can observe the ticket.  No ordinary (non-atomic) store needs to be published.
+    // ------------------------------------------------------------------------------------------
+    __device__ __forceinline__ void publish(double *p, double v)
+    {
+      __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void publish(float *p, float v)
+    {
+      __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ double fetch(const double *p)
+    {
+      return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ float fetch(const float *p)
+    {
+      return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // Wait until this wave's stores (and every other vector memory operation) have completed;
+    // the memory clobber also keeps the compiler from moving memory accesses across it.
+    __device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+    __device__ __forceinline__ unsigned take_ticket(unsigned *counter)
+    {
+      const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::: "memory"); // reads of partials stay after the ticket
+      return t;
+    }
+    __device__ __forceinline__ void reset_counter(unsigned *counter)
+    {
+      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Deterministic cross-block reduction.  Each block publishes its partial and takes a ticket;
+    // the block that takes the last ticket of a group sums all partials of that group in index
+    // order (compensated) and resets the group's counter so the next launch (or graph replay)
+    // starts from zero.  Returns true in the finishing block, with the group total in *total
+    // (valid in thread 0).
     // ------------------------------------------------------------------------------------------
     __device__ __forceinline__ bool finish_group(double partial, double *partials, int slot, int first_slot,
                                                  int n_slots, unsigned *counter, double *total)
     {
       __shared__ bool last;
       if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[slot], partial, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned ticket = atomicAdd(counter, 1u);
-        last = (ticket == (unsigned)(n_slots - 1));
+        publish(&partials[slot], partial);
+        wait_stores();
+        last = take_ticket(counter) == (unsigned)(n_slots - 1);
       }
       __syncthreads();
       if (!last) return false;
       if (threadIdx.x == 0) {
-        __threadfence();
         KahanSum s;
-        for (int i = 0; i < n_slots; ++i)
-          s.add(__hip_atomic_load(&partials[first_slot + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (int i = 0; i < n_slots; ++i) s.add(fetch(&partials[first_slot + i]));
         *total = s.value();
-        atomicExch(counter, 0u);
+        reset_counter(counter);
       }
       return true;
     }

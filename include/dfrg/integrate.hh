@@ -70,6 +70,24 @@ namespace dfrg
       return t;
     }
 
+    // Partials of type NT through device-scope atomics, component-wise (dfrg::dev::publish).
+    template <class NT> __device__ __forceinline__ void publish_nt(NT *p, const NT &v)
+    {
+      if constexpr (std::is_same_v<NT, double> || std::is_same_v<NT, float>)
+        dev::publish(p, v);
+      else {
+        dev::publish(&p->re, v.re);
+        dev::publish(&p->im, v.im);
+      }
+    }
+    template <class NT> __device__ __forceinline__ NT fetch_nt(const NT *p)
+    {
+      if constexpr (std::is_same_v<NT, double> || std::is_same_v<NT, float>)
+        return dev::fetch(p);
+      else
+        return NT(dev::fetch(&p->re), dev::fetch(&p->im));
+    }
+
     // Publish this block's partial; the last block adds all partials in block order and writes
     // *result (then resets the counter for the next use of the scratch block).
     template <class NT> __device__ __forceinline__ void finish_blocks(NT part, NT *partials, unsigned *counter,
@@ -77,18 +95,16 @@ namespace dfrg
     {
       __shared__ bool last;
       if (threadIdx.x == 0) {
-        partials[blockIdx.x] = part;
-        __threadfence();
-        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+        publish_nt(&partials[blockIdx.x], part);
+        dev::wait_stores();
+        last = dev::take_ticket(counter) == gridDim.x - 1;
       }
       __syncthreads();
       if (last && threadIdx.x == 0) {
-        __threadfence();
-        // the acquire fence above orders these loads after every block's published partial
-        NT s = partials[0];
-        for (unsigned i = 1; i < gridDim.x; ++i) s += partials[i];
+        NT s = fetch_nt(&partials[0]);
+        for (unsigned i = 1; i < gridDim.x; ++i) s += fetch_nt(&partials[i]);
         *result = s;
-        atomicExch(counter, 0u);
+        dev::reset_counter(counter);
       }
     }
 
