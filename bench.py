@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--flows", type=int, default=48, help="flows in the serving run")
+    ap.add_argument("--workloads", default="threshold,finite_T,uneven,flows",
+                    help="comma-separated workloads measured after the headline")
     ap.add_argument("--shape", action="append", default=[], metavar="MODE:BLOCK,TILE,GRID",
                     help="fixed launch shape instead of the autotuner for the headline (MODE fp64|mixed); "
                          "for counter runs, where serialised dispatches distort the tuner's timings")
@@ -373,7 +375,10 @@ def main():
                                "evals_per_s": evals / t_cpu, "cpu_model": cpu_model()}
     if not args.only_headline:
         work = {}
+        chosen = set(args.workloads.split(","))
         for name, fn in (("threshold", threshold), ("finite_T", finite_T), ("uneven", uneven), ("flows", flows)):
+            if name not in chosen:
+                continue
             t0 = time.perf_counter()
             work[name] = fn(D, qp, args)
             log(f"{name} done in {time.perf_counter() - t0:.1f} s")

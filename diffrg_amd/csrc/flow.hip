@@ -12,10 +12,20 @@
 // Device layout: every resident flow owns 4 n doubles in one HBM pool (u, RK accumulator, two
 // stage buffers) and a slot in SoA descriptor arrays.  A point table lists (slot, i) of every
 // resident grid point; one wave evaluates one point (one Gauss-Legendre node per lane, wave64 DPP
-// sums), so all resident flows advance together in each launch.  One RK step is five launches
-// (four stages and an advance kernel that counts steps and flags finished / non-finite flows);
-// steps_per_launch steps are captured into one hipGraph.  Between graph launches the host retires
-// finished flows and admits queued ones while their HBM footprint fits the budget.
+// sums), so all resident flows advance together in each launch.  One RK step is four launches,
+// one per stage; steps_per_launch steps are captured into one hipGraph.  A flow's step index is
+// the launch's base step (written by the host before each graph launch) plus the step within the
+// graph minus the base step at its admission, so flows admitted at different times share the
+// graph and a finished flow simply stops updating.  The graph's last node advances the device copy
+// of the base step, which the host mirrors.  Between graph launches the host retires
+// finished and non-finite flows and admits queued ones while their HBM footprint fits the budget.
+//
+// Node factors: with x_j the GL nodes, xr_j = x_j x_extent, the regulator terms are
+// q^2 = k^2 xr, R = k^2 e^-xr, RBdot = 2 k^2 e^-xr (1 + xr) and the d = 3 element times weight is
+// k^3 sqrt(xr) w_j x_extent / (4 pi^2), so per node only A_j = xr_j + e^-xr_j (E^2 = k^2 A_j + m^2)
+// and D_j = sqrt(xr_j) w_j x_extent e^-xr_j (1 + xr_j) / (2 pi^2) (element * weight * RBdot = k^5 D_j)
+// depend on j; both are fixed by the quadrature and tabulated once.  k(t) is tabulated per step and
+// stage exactly as the oracle's stepper forms it.
 #include "internal.hh"
 #include "dfrg/device.hh"
 
@@ -35,43 +45,47 @@ namespace
   constexpr double kPi = 3.14159265358979323846;
 
   struct FlowDev {
-    int *n, *off, *step, *status, *bad; // per slot; status 0 idle, 1 running, 2 done, 3 failed
-    double *h, *T, *N;                  // grid spacing, temperature, O(N)
-    int2 *points;                       // (slot, i) of every resident grid point
-    int *n_points;                      // device copy of the point count
-    double *pool;                       // state: u | acc | y0 | y1 per flow
-    const double *x, *w;                // GL nodes on [0,1]
+    int *n, *off, *admit, *status, *bad; // per slot; status 0 idle, 1 running (host-owned)
+    double *h, *T, *N;                   // grid spacing, temperature, O(N)
+    int2 *points;                        // (slot, i) of every resident grid point
+    int *n_points;                       // device copy of the point count
+    int *base;                           // global step index of the first step of this launch
+    double *pool;                        // state: u | acc | y0 | y1 per flow
+    const double *A, *D;                 // node factors (header comment)
+    const double *kt;                    // k at (step, stage point): 3 per step, t, t + dt/2, t + dt
     int nx, n_steps;
-    double x_extent, dt;
+    double dt;
   };
 
-  // S2(E) with e = exp(-E/T): coth = (1+e)/(1-e), csch^2 = 4e/(1-e)^2, 1-e = -expm1(-E/T)
-  __device__ __forceinline__ double bose_S2(double E, double T)
+  // S2(E) = coth(E/2T)/(4E^3) + csch^2(E/2T)/(8 T E^2) with em1 = expm1(-E/T), e = 1 + em1,
+  // coth = (1+e)/(1-e), csch^2 = 4e/(1-e)^2, put over one denominator:
+  //   S2 = (2 T (1+e)(1-e) + 4 e E) / (8 T E^3 (1-e)^2)
+  __device__ __forceinline__ double bose_S2(double E, double T, double inv_T)
   {
-    const double e = exp(-E / T);
-    const double ome = -expm1(-E / T);
-    const double coth = (1.0 + e) / ome;
-    const double csch2 = 4.0 * e / (ome * ome);
-    return coth / (4.0 * E * E * E) + csch2 / (8.0 * T * E * E);
+    const double em1 = expm1(-E * inv_T);
+    const double e = 1.0 + em1, ome = -em1;
+    const double num = fma(2.0 * T * (1.0 + e), ome, 4.0 * e * E);
+    return num / (8.0 * T * (E * E * E) * (ome * ome));
   }
 
-  __global__ __launch_bounds__(kFlowBlock) void flow_stage_kernel(FlowDev F, int stage)
+  __global__ __launch_bounds__(kFlowBlock) void flow_stage_kernel(FlowDev F, int stage, int local)
   {
     const int lane = threadIdx.x & 63;
     const int wave_id = (blockIdx.x * kFlowBlock + threadIdx.x) >> 6;
     const int n_waves = (gridDim.x * kFlowBlock) >> 6;
     const int P = *F.n_points;
+    const int base = *F.base + local;
     for (int w = wave_id; w < P; w += n_waves) {
       const int2 pt = F.points[w];
       const int s = pt.x, i = pt.y;
-      if (F.status[s] != 1) continue; // wave-uniform
+      const int step = base - F.admit[s];
+      if (F.status[s] != 1 || step < 0 || step >= F.n_steps) continue; // wave-uniform
       const int n = F.n[s];
       double *u = F.pool + F.off[s];
       double *acc = u + n, *y0 = u + 2 * n, *y1 = u + 3 * n;
       const double *in = stage == 1 ? u : (stage == 3 ? y1 : y0);
       const double h = F.h[s], T = F.T[s], Nn = F.N[s];
-      const double t = F.step[s] * F.dt;
-      const double k = exp(-(stage == 1 ? t : (stage == 4 ? t + F.dt : t + 0.5 * F.dt)));
+      const double k = F.kt[3 * step + (stage == 1 ? 0 : (stage == 4 ? 2 : 1))];
       double up, upp;
       if (i == 0) {
         up = (-3.0 * in[0] + 4.0 * in[1] - in[2]) / (2.0 * h);
@@ -86,23 +100,16 @@ namespace
       const double rho = i * h;
       const double m2pi = in[i], m2sig = in[i] + 2.0 * rho * up;
       // two threshold integrals, lane j evaluates node j (d = 3 radial element)
-      const double k2 = k * k;
+      const double k2 = k * k, inv_T = 1.0 / T;
       double ipi = 0.0, isg = 0.0;
       for (int j = lane; j < F.nx; j += 64) {
-        const double q = k * sqrt(F.x[j] * F.x_extent);
-        const double el = 4.0 * kPi * (q / 2.0 * k2) / (8.0 * kPi * kPi * kPi);
-        const double wgt = F.w[j] * F.x_extent;
-        const double q2 = q * q;
-        const double xr = q2 / k2;
-        const double ex = exp(-xr);
-        const double R = k2 * ex;
-        const double rdot = 2.0 * ex * k2 * (1.0 + xr);
-        const double Epi = sqrt(q2 + R + m2pi), Esg = sqrt(q2 + R + m2sig);
-        ipi += el * wgt * (rdot * bose_S2(Epi, T));
-        isg += el * wgt * (rdot * bose_S2(Esg, T));
+        const double a = F.A[j], dd = F.D[j];
+        ipi = fma(dd, bose_S2(sqrt(fma(k2, a, m2pi)), T, inv_T), ipi);
+        isg = fma(dd, bose_S2(sqrt(fma(k2, a, m2sig)), T, inv_T), isg);
       }
-      ipi = dev::wave_sum(ipi);
-      isg = dev::wave_sum(isg);
+      const double k5 = k2 * k2 * k;
+      ipi = k5 * dev::wave_sum(ipi);
+      isg = k5 * dev::wave_sum(isg);
       if (lane == 0) {
         const double f = 0.5 * ((Nn - 1.0) * up * ipi + (3.0 * up + 2.0 * rho * upp) * isg);
         const double dt = F.dt;
@@ -124,17 +131,10 @@ namespace
     }
   }
 
-  // one lane per slot: count the step, flag finished and non-finite flows
-  __global__ void flow_advance_kernel(FlowDev F, int n_slots)
+  // last node of each graph launch: the next launch starts spl steps later
+  __global__ void flow_base_kernel(int *base, int spl)
   {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_slots || F.status[s] != 1) return;
-    const int st = F.step[s] + 1;
-    F.step[s] = st;
-    if (F.bad[s])
-      F.status[s] = 3;
-    else if (st >= F.n_steps)
-      F.status[s] = 2;
+    if (threadIdx.x == 0) *base += spl;
   }
 
   double now_s()
@@ -147,7 +147,7 @@ struct FlowRecord {
   dfrg_flow_desc desc;
   int state = 0; // 0 queued, 1 resident, 2 finished, 3 failed
   double t_submit = 0, t_done = 0;
-  int slot = -1, off = -1;
+  int slot = -1, off = -1, admit = 0;
   std::vector<double> u;
 };
 
@@ -166,6 +166,8 @@ struct dfrg_fs {
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  int base = 0;           // global step index of the next launch's first step (mirrors dev.base)
+  double *d_nodes = nullptr; // A | D | kt
   std::vector<double> latencies;
   double busy_s = 0;
   int finished = 0;
@@ -208,12 +210,10 @@ namespace
     hipStream_t cs;
     DFRG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     DFRG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int st = 0; st < fs->steps_per_launch; ++st) {
+    for (int st = 0; st < fs->steps_per_launch; ++st)
       for (int stage = 1; stage <= 4; ++stage)
-        hipLaunchKernelGGL(flow_stage_kernel, dim3(kFlowGrid), dim3(kFlowBlock), 0, cs, fs->dev, stage);
-      hipLaunchKernelGGL(flow_advance_kernel, dim3((fs->max_resident + 63) / 64), dim3(64), 0, cs, fs->dev,
-                         fs->max_resident);
-    }
+        hipLaunchKernelGGL(flow_stage_kernel, dim3(kFlowGrid), dim3(kFlowBlock), 0, cs, fs->dev, stage, st);
+    hipLaunchKernelGGL(flow_base_kernel, dim3(1), dim3(64), 0, cs, fs->dev.base, fs->steps_per_launch);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cs, &g);
     hipStreamDestroy(cs);
@@ -229,16 +229,18 @@ namespace
   // retire finished / failed flows: copy u back, free their slot and HBM range
   int fs_retire(dfrg_fs *fs)
   {
-    std::vector<int> status(fs->max_resident);
-    DFRG_HIP(hipMemcpy(status.data(), fs->dev.status, sizeof(int) * fs->max_resident, hipMemcpyDeviceToHost));
+    std::vector<int> bad(fs->max_resident);
+    DFRG_HIP(hipMemcpy(bad.data(), fs->dev.bad, sizeof(int) * fs->max_resident, hipMemcpyDeviceToHost));
     const double t = now_s();
     for (int s = 0; s < fs->max_resident; ++s) {
-      if (fs->slot_flow[s] < 0 || (status[s] != 2 && status[s] != 3)) continue;
+      if (fs->slot_flow[s] < 0) continue;
       FlowRecord &r = fs->flows[fs->slot_flow[s]];
+      const bool done = fs->base - r.admit >= fs->n_steps;
+      if (!done && !bad[s]) continue;
       const int n = r.desc.n_grid;
       r.u.resize(n);
       DFRG_HIP(hipMemcpy(r.u.data(), fs->dev.pool + r.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      r.state = status[s];
+      r.state = bad[s] ? 3 : 2;
       r.t_done = t;
       if (r.state == 2) {
         fs->latencies.push_back(r.t_done - r.t_submit);
@@ -277,7 +279,7 @@ namespace
       const int one = 1, zero = 0;
       DFRG_HIP(hipMemcpy(fs->dev.n + slot, &n, sizeof(int), hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.off + slot, &off, sizeof(int), hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.step + slot, &zero, sizeof(int), hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.admit + slot, &fs->base, sizeof(int), hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.bad + slot, &zero, sizeof(int), hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.h + slot, &h, sizeof(double), hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.T + slot, &r.desc.T, sizeof(double), hipMemcpyHostToDevice));
@@ -286,6 +288,7 @@ namespace
       fs->slot_flow[slot] = id;
       r.slot = slot;
       r.off = off;
+      r.admit = fs->base;
       r.state = 1;
       fs->points_dirty = true;
     }
@@ -341,16 +344,28 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
   FlowDev &d = fs->dev;
   d.nx = nx;
   d.n_steps = n_steps;
-  d.x_extent = fs->x_extent;
   d.dt = std::log(1.0 / k_end) / n_steps;
-  const void *dx = nullptr, *dw = nullptr;
-  int rc = dfrg_qp_gl(qp, nx, 0, &dx, &dw, nullptr, nullptr);
+  const void *hx = nullptr, *hw = nullptr;
+  int rc = dfrg_qp_gl(qp, nx, 0, nullptr, nullptr, &hx, &hw);
   if (rc != DFRG_OK) {
     delete fs;
     return rc;
   }
-  d.x = (const double *)dx;
-  d.w = (const double *)dw;
+  // node factors and k(t) (header comment); k as the oracle's stepper forms it: t = s dt,
+  // exp(-t), exp(-(t + dt/2)), exp(-(t + dt))
+  std::vector<double> nodes(2 * nx + 3 * (size_t)n_steps);
+  for (int j = 0; j < nx; ++j) {
+    const double xr = ((const double *)hx)[j] * fs->x_extent;
+    const double e = std::exp(-xr);
+    nodes[j] = xr + e;
+    nodes[nx + j] = std::sqrt(xr) * (((const double *)hw)[j] * fs->x_extent) * e * (1.0 + xr) / (2.0 * kPi * kPi);
+  }
+  for (int st = 0; st < n_steps; ++st) {
+    const double t = st * d.dt;
+    nodes[2 * nx + 3 * st] = std::exp(-t);
+    nodes[2 * nx + 3 * st + 1] = std::exp(-(t + 0.5 * d.dt));
+    nodes[2 * nx + 3 * st + 2] = std::exp(-(t + d.dt));
+  }
   const size_t pts_bytes = sizeof(int2) * std::min(max_points, fs->pool_doubles / 4 + 1);
 #define FS_ALLOC(ptr, bytes)                                                                                   \
   do {                                                                                                         \
@@ -362,7 +377,7 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
   } while (0)
   FS_ALLOC(d.n, sizeof(int) * max_resident);
   FS_ALLOC(d.off, sizeof(int) * max_resident);
-  FS_ALLOC(d.step, sizeof(int) * max_resident);
+  FS_ALLOC(d.admit, sizeof(int) * max_resident);
   FS_ALLOC(d.status, sizeof(int) * max_resident);
   FS_ALLOC(d.bad, sizeof(int) * max_resident);
   FS_ALLOC(d.h, sizeof(double) * max_resident);
@@ -370,8 +385,18 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
   FS_ALLOC(d.N, sizeof(double) * max_resident);
   FS_ALLOC(d.points, pts_bytes);
   FS_ALLOC(d.n_points, sizeof(int));
+  FS_ALLOC(d.base, sizeof(int));
+  FS_ALLOC(fs->d_nodes, sizeof(double) * nodes.size());
   FS_ALLOC(d.pool, sizeof(double) * fs->pool_doubles);
 #undef FS_ALLOC
+  d.A = fs->d_nodes;
+  d.D = fs->d_nodes + nx;
+  d.kt = fs->d_nodes + 2 * nx;
+  if (hipMemcpy(fs->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("flow server: node table upload failed");
+    dfrg_fs_destroy(fs);
+    return DFRG_ERR_HIP;
+  }
   if (hipStreamCreateWithFlags(&fs->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("flow server: stream creation failed");
     dfrg_fs_destroy(fs);
@@ -390,8 +415,9 @@ int dfrg_fs_destroy(dfrg_fs *fs)
   if (fs->graph) hipGraphDestroy(fs->graph);
   if (fs->stream) hipStreamDestroy(fs->stream);
   FlowDev &d = fs->dev;
-  for (void *p : {(void *)d.n, (void *)d.off, (void *)d.step, (void *)d.status, (void *)d.bad, (void *)d.h,
-                  (void *)d.T, (void *)d.N, (void *)d.points, (void *)d.n_points, (void *)d.pool})
+  for (void *p : {(void *)d.n, (void *)d.off, (void *)d.admit, (void *)d.status, (void *)d.bad, (void *)d.h,
+                  (void *)d.T, (void *)d.N, (void *)d.points, (void *)d.n_points, (void *)d.base, (void *)d.pool,
+                  (void *)fs->d_nodes})
     if (p) hipFree(p);
   delete fs;
   return DFRG_OK;
@@ -426,6 +452,7 @@ int dfrg_fs_progress(dfrg_fs *fs, int *resident, int *queued, int *done)
     const double t0 = now_s();
     DFRG_HIP(hipGraphLaunch(fs->graph_exec, fs->stream));
     DFRG_HIP(hipStreamSynchronize(fs->stream));
+    fs->base += fs->steps_per_launch;
     fs->busy_s += now_s() - t0;
     DFRG_TRY(fs_retire(fs));
     DFRG_TRY(fs_admit(fs));
