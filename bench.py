@@ -34,12 +34,13 @@ PEAK_FP64_VALU = 78.6e12
 # one wave64 VALU instruction per clock per CU (256 CUs x 4 SIMD16 x 2.4 GHz / 4 clocks)
 PEAK_VALU_WAVE_INSTR = 256 * 2.4e9
 MIXED_TOL = 1e-8
+GRAPH_SETS = 10  # call sets per hipGraph launch
 
 # Per integrand evaluation of the innermost loop of se_kernel (FP64, MIXED): FP operations
 # (FMA = 2), VALU instructions and quarter-rate transcendentals, counted from the gfx950
 # assembly of csrc/integrals.hip by `python -m diffrg_amd.isa_count`.
-SE_FLOP = {0: 223.5, 1: 53.0}
-SE_VALU = {0: 203, 1: 56}
+SE_FLOP = {0: 223.5, 1: 51.0}
+SE_VALU = {0: 203, 1: 50.5}
 
 
 def valu_issue_frac(mode, evals, seconds):
@@ -111,6 +112,26 @@ def time_steps(D, fn, steps, warmup):
     return e0.elapsed_ms(e1) / steps, wall
 
 
+def time_streams(D, fns, steps, warmup):
+    """Throughput of independent call sets issued round-robin on len(fns) streams: fns[s]() enqueues
+    one call set on stream s (with its own output and scratch).  Host clock between two full
+    synchronisations; returns ms per call set."""
+    for i in range(warmup):
+        fns[i % len(fns)][0]()
+    for _, st in fns:
+        st.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        fns[i % len(fns)][0]()
+    for _, st in fns:
+        st.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+N_STREAMS = 4
+BATCH = 8  # call sets per launch in the batched measurements of the small workloads
+
+
 def rel_err(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.abs(np.asarray(b))))
 
@@ -154,6 +175,10 @@ def selfenergy(D, qp, args):
     ref_fn = lambda: O.c1_selfenergy(c["p"], [c["Z"]] * n, [coord] * n, [xq] * n, [xa] * n, c["x_extent"], c["k"])
     ref = ref_fn()
     res = {}
+    ms_objs = []
+    for _ in range(N_STREAMS):
+        ms_objs.append((D.SEBatch(qp, c["p"], c["nq"], c["na"], [c["Z"]] * n, *[[v] * n for v in c["coord"]],
+                                  x_extent=c["x_extent"], k=c["k"]), D.DeviceArray(n), D.Stream()))
     fixed = {m.split(":")[0]: [int(v) for v in m.split(":")[1].split(",")] for m in args.shape}
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
         if name in fixed:
@@ -161,9 +186,27 @@ def selfenergy(D, qp, args):
             shape = b.shape()
         else:
             shape = b.autotune(mode, reps=5)
-        ms, wall = time_steps(D, lambda: b.run(out, mode), args.steps, args.warmup)
-        res[name] = {"ms": ms, "shape": shape, "err": rel_err(out.to_host(), ref)}
-        log(f"selfenergy {name}: {ms * 1e3:.1f} us/step, err {res[name]['err']:.2e}, shape {shape}")
+        ms_eager, _ = time_steps(D, lambda: b.run(out, mode), args.steps, args.warmup)
+        err_eager = rel_err(out.to_host(), ref)
+        # the same call sets replayed from a hipGraph of GRAPH_SETS launches
+        per = GRAPH_SETS
+        ms_graph, _ = time_steps(D, lambda: b.graph_launch(out, mode, launches=per), max(1, args.steps // per),
+                                 max(1, args.warmup // per))
+        ms_graph /= per
+        err = max(err_eager, rel_err(out.to_host(), ref))
+        # independent call sets on N_STREAMS streams, one batch object (own scratch) per stream
+        fns = [(lambda bb=bb, o=o, st=st: bb.run(o, mode, stream=st.ptr), st) for bb, o, st in ms_objs]
+        for bb, _, _ in ms_objs:
+            bb.set_shape(*[shape[k] for k in ("block", "tile_evals", "grid")])
+        ms_streams = time_streams(D, fns, args.steps, args.warmup)
+        for _, o, _ in ms_objs:
+            err = max(err, rel_err(o.to_host(), ref))
+        ms = min(ms_eager, ms_graph)
+        res[name] = {"ms": ms, "ms_eager": ms_eager, "ms_graph": ms_graph, "ms_streams": ms_streams, "shape": shape,
+                     "err": err, "launch": "hipGraph" if ms_graph < ms_eager else "eager"}
+        log(f"selfenergy {name}: eager {ms_eager * 1e3:.1f} us, graph {ms_graph * 1e3:.1f} us, "
+            f"{N_STREAMS} streams {ms_streams * 1e3:.1f} us per call set, "
+            f"err {err:.2e}, shape {shape}")
     return {"n": n, "evals": b.evals, "res": res, "ref_fn": ref_fn}
 
 
@@ -182,6 +225,14 @@ def threshold(D, qp, args):
                            args.steps, args.warmup)
         r[name] = {"integrals_per_s": len(c["m2"]) / (ms * 1e-3), "us_per_step": ms * 1e3,
                    "max_rel_err": rel_err(out.to_host(), ref)}
+    # BATCH call sets in one launch (the same 256 integrals repeated): throughput without the fixed
+    # per-launch latency that dominates a 256-integral launch
+    d_m2b = D.DeviceArray.from_host(np.tile(c["m2"], BATCH))
+    outb = D.DeviceArray(BATCH * len(c["m2"]))
+    for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
+        ms, _ = time_steps(D, lambda: D.threshold_batch(d_m2b, d_x, d_w, outb, c["x_extent"], c["k"], mode, c["d"]),
+                           args.steps, args.warmup)
+        r[name][f"integrals_per_s_batched_x{BATCH}"] = BATCH * len(c["m2"]) / (ms * 1e-3)
     t_cpu, runs = cpu_timed(ref_fn, 2.0)
     r["cpu_integrals_per_s"] = len(c["m2"]) / t_cpu
     r["config"] = "256 m2 ~ U[-0.8,4], 64 GL nodes in q^2 over [0,20], d=4"
@@ -224,8 +275,33 @@ def finite_T(D, qp, args):
         r[name] = {"integrals_per_s": len(c["T"]) / (ms * 1e-3), "evals_per_s": evals / (ms * 1e-3),
                    "us_per_step": ms * 1e3, "max_rel_err": float(err.max()), "block": best[1],
                    "max_rel_err_per_T": {str(t): float(err[c["T"] == t].max()) for t in W.TEMPS}}
+    # BATCH call sets in one launch (the same 512 externals repeated)
+    devb = dict(dev)
+    for key in ("T", "p0", "pabs", "m1", "m2"):
+        devb[key] = Dh(np.tile(c[key], BATCH))
+    devb["m_off"] = Dh(np.tile(np.array([off[t] for t in c["T"]], np.int32), BATCH))
+    devb["m_n"] = Dh(np.tile(np.array([len(c["rules"][t][0]) for t in c["T"]], np.int32), BATCH))
+    outb = D.DeviceArray(BATCH * len(c["T"]))
+    for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
+        run = lambda: D.ft_batch(devb, outb, c["q_max"], c["k"], mode, c["d"], c["radial_mode"])
+        best = None
+        for block in (64, 128, 256, 512):
+            D.ft_set_block(block)
+            t, _ = time_steps(D, run, 20, 2)
+            if best is None or t < best[0]:
+                best = (t, block)
+        D.ft_set_block(best[1])
+        ms, _ = time_steps(D, run, args.steps, args.warmup)
+        r[name][f"integrals_per_s_batched_x{BATCH}"] = BATCH * len(c["T"]) / (ms * 1e-3)
+        r[name][f"block_batched_x{BATCH}"] = best[1]
+        if name == "mixed":
+            got = outb.to_host()
+            r[name][f"max_rel_err_batched_x{BATCH}"] = float(np.max(np.abs(got - np.tile(ref, BATCH)) /
+                                                                     np.abs(np.tile(ref, BATCH))))
     D.ft_set_block(0)
     r["mixed_over_plain"] = r["mixed"]["integrals_per_s"] / r["fp64"]["integrals_per_s"]
+    r[f"mixed_over_plain_batched_x{BATCH}"] = (r["mixed"][f"integrals_per_s_batched_x{BATCH}"] /
+                                              r["fp64"][f"integrals_per_s_batched_x{BATCH}"])
     t_cpu, _ = cpu_timed(ref_fn, 2.0)
     r["cpu_integrals_per_s"] = len(c["T"]) / t_cpu
     r["config"] = "512 externals, T in {0.01..0.3}, explicit Matsubara n in [-64,64], 48 GL in |q| over [0,6]"
@@ -349,11 +425,16 @@ def main():
         "dtype": "fp32-seed/fp64-acc", "data": "synthetic (SplitMix64 seed 0xD1FF0000+c)",
         "config": {"workload": "two-angle self-energy: 64 p on log grid [1e-3,40], nodes {64,32,32}, "
                                "128-point Z table (4,194,304 evals per step)",
-                   "integrals_per_step": n, "evals_per_step": evals},
+                   "integrals_per_step": n, "evals_per_step": evals,
+                   "launch": f"{mx['launch']} (eager launches or hipGraph of {GRAPH_SETS} call sets, the faster)"},
         "mixed": {"integrals_per_s": n / (mx["ms"] * 1e-3), "evals_per_s": evals / (mx["ms"] * 1e-3),
-                  "max_rel_err_vs_fp64_oracle": mx["err"], "shape": mx["shape"]},
+                  "max_rel_err_vs_fp64_oracle": mx["err"], "shape": mx["shape"], "launch": mx["launch"],
+                  "us_eager": mx["ms_eager"] * 1e3, "us_graph": mx["ms_graph"] * 1e3,
+                  f"integrals_per_s_{N_STREAMS}_streams": n / (mx["ms_streams"] * 1e-3)},
         "plain_fp64": {"integrals_per_s": n / (fp["ms"] * 1e-3), "evals_per_s": evals / (fp["ms"] * 1e-3),
-                       "ms_per_step": fp["ms"], "max_rel_err_vs_oracle": fp["err"], "shape": fp["shape"]},
+                       "ms_per_step": fp["ms"], "max_rel_err_vs_oracle": fp["err"], "shape": fp["shape"],
+                       "launch": fp["launch"], "us_eager": fp["ms_eager"] * 1e3, "us_graph": fp["ms_graph"] * 1e3,
+                       f"integrals_per_s_{N_STREAMS}_streams": n / (fp["ms_streams"] * 1e-3)},
         "mixed_over_plain": fp["ms"] / mx["ms"],
     }
     achieved = SE_FLOP[1] * evals / (mx["ms"] * 1e-3) / 1e12

@@ -186,6 +186,29 @@ def stream_sync(stream=None) -> None:
     check(lib().dfrg_stream_sync(stream))
 
 
+class Stream:
+    """A non-blocking HIP stream owned by the library (dfrg_stream_create)."""
+
+    def __init__(self):
+        h = _vp()
+        check(lib().dfrg_stream_create(C.byref(h)))
+        self.ptr = h.value
+
+    def synchronize(self):
+        check(lib().dfrg_stream_sync(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            lib().dfrg_stream_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Event:
     def __init__(self):
         p = _vp()

@@ -117,6 +117,13 @@ namespace
     const double k = A.k, k2 = k * k;
     const double T = A.T[i], p0 = A.p0[i], pa = A.pabs[i], m1 = A.m1[i], m2 = A.m2[i];
     const int off = A.m_off[i], nm = A.m_n[i];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // this lane's first Matsubara node, loaded before the q-only terms so the two loads overlap
+    double wn0 = 0.0, wgt0 = 0.0;
+    if (lane < nm) {
+      wn0 = A.mx[off + lane];
+      wgt0 = A.mw[off + lane];
+    }
     double acc = 0.0;
     // q-only terms, and the zero mode T f(q, 0) of sum = T f(0) + sum_n w_n (f(x_n) + f(-x_n))
     for (int j = threadIdx.x; j < nx; j += BLOCK) {
@@ -155,11 +162,10 @@ namespace
     // Matsubara nodes across the lanes (fixed per lane), radial nodes across the waves; the
     // q-only terms are wave-uniform LDS reads
     constexpr int NW = BLOCK / 64;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int n0 = 0; n0 < nm; n0 += 64) {
       const int n = n0 + lane;
       if (n >= nm) break;
-      const double wn = A.mx[off + n], wgt = A.mw[off + n];
+      const double wn = n0 == 0 ? wn0 : A.mx[off + n], wgt = n0 == 0 ? wgt0 : A.mw[off + n];
       const double dp = wn + p0, dm = -wn + p0;
       if (MODE == DFRG_FP64) {
         // the reference's expression tree: three FP64 divisions per pair
@@ -211,12 +217,18 @@ namespace
     float dz, pad;
   };
 
+  // mixed-path cell data of node i: y_i^2 (FP64), 1/(b + y_i - c) and y_i (FP32)
+  struct SECellY {
+    double y2;
+    float inv_span, y;
+  };
+
   struct SEArgs {
     int n_tiles;
     const SETile *tiles;
     const double *nodes;
     const double *tab64;  // FP64 tables
-    const double2 *tabA;  // mixed-path cell data: (y_i, 1/(b + y_i - c))
+    const SECellY *tabA;  // mixed-path cell data: (y_i^2, 1/(b + y_i - c), y_i)
     const SECellZ *tabB;  // mixed-path cell data: (Z_i, Z_{i+1} - Z_i)
     double x_extent, k, pref; // pref = 2pi/(2pi)^4
     double *partials;
@@ -230,19 +242,6 @@ namespace
   constexpr int kMaxAng = 64;
   constexpr int kMaxQ = 128;
 
-  // Log-coordinate lerp of the FP64 values of a table (coordinates.hh:182-186 backward,
-  // linear_interpolation_1D.hh:75-92): idx = log1p((y - c)/b) (n-1)/bias clamped to [0, n-1].
-  template <typename V> __device__ __forceinline__ double lerp_log_cells(V val, int n, double c, double inv_b,
-                                                                         double scale, double y)
-  {
-    double idx = log1p((y - c) * inv_b) * scale;
-    idx = fmax(0.0, fmin(idx, (double)(n - 1)));
-    const double fl = floor(idx);
-    const int i0 = (int)fl;
-    const int i1 = (int)ceil(idx);
-    return val(i0) * (1.0 - idx + fl) + val(i1) * (idx - fl);
-  }
-
   // G(|p-q|) = 1/(Z(|p-q|) |p-q|^2 + R(|p-q|^2)) in FP64, the reference's expression tree.
   __device__ __forceinline__ double se_G_fp64(const double *tab, int ntab, double c, double inv_b, double scale,
                                               double k2, double pq2)
@@ -253,43 +252,74 @@ namespace
   }
 
   struct SEMixed {
-    const double2 *tabA;
+    const SECellY *tabA;
     const SECellZ *tabB;
     const double *exp2t; // 2^(j/64), j = 0..63
     float cf, ibf, scl, sf;
-    double ek64, k2;
+    double ek64;
     int imax;
   };
 
   // G(|p-q|) in mixed precision: FP32 hardware seeds replace the FP64 library sequences, FP64 is
-  // kept where the error budget needs it.  sqrt from an FP32 seed and one FP64 Newton step; Z cell
-  // found with the FP32 log2, the in-cell position s*log1p(u) (|u| <~ 0.05) in FP32 with the
-  // rounding of 1+u restored, Z assembled in FP64 from the FP64 node value; R = k^2 2^t with t
-  // split on a 1/64 grid (table) and a degree-4 polynomial remainder in FP64 (t is clamped at
-  // -1020, where R is below 1e-300 of the denominator); the denominator in FP64; the reciprocal
-  // from an FP32 seed refined in FP64.  Per-evaluation error ~1e-9 relative.
+  // kept where the error budget needs it.  y = |p-q| as an FP32 seed finds the Z cell through the
+  // FP32 log2; the in-cell offset y - y_i = (|p-q|^2 - y_i^2)/(y + y_i) takes the difference in
+  // FP64 (no cancellation error) and the rest in FP32; the in-cell position s*log1p(u)
+  // (|u| <~ 0.05) in FP32 with the rounding of 1+u restored, Z assembled in FP64 from the FP64 node
+  // value; R = k^2 2^t with t split on a 1/64 grid (table of k^2 2^(j/64)) and a degree-4
+  // polynomial remainder in FP64 (2^t for t below -2^25/64 underflows to 0 through the exponent);
+  // the denominator in FP64; the reciprocal from an FP32 seed refined in FP64.  Per-evaluation
+  // error ~1e-9 relative.
   __device__ __forceinline__ double se_G_mixed(const SEMixed &m, double pq2d)
   {
     const float ys = __builtin_amdgcn_sqrtf((float)pq2d);
-    const double ysd = (double)ys;
-    const double yd = fma(fma(-ysd, ysd, pq2d), (double)(0.5f * __builtin_amdgcn_rcpf(ys)), ysd);
     const float coarse = __builtin_amdgcn_logf(fmaf(ys - m.cf, m.ibf, 1.0f)) * m.scl;
     const int i0 = min(max((int)__builtin_floorf(coarse), 0), m.imax);
-    const double2 ca = m.tabA[i0];
+    const SECellY ca = m.tabA[i0];
     const SECellZ cz = m.tabB[i0];
-    const float u = (float)((yd - ca.x) * ca.y);
+    const float num = (float)(pq2d - ca.y2);
+    const float u = num * (ca.inv_span * __builtin_amdgcn_rcpf(fmaxf(ys + ca.y, 1e-30f)));
     const float w1 = 1.0f + u;
     const float ru = u - (w1 - 1.0f); // rounding error of 1 + u
     float fr = fmaf(__builtin_amdgcn_logf(w1), m.scl, ru * m.sf);
     fr = fmaxf(0.0f, fminf(fr, 1.0f)); // NaN-safe; realises the [0, n-1] clamp
     const double Z = cz.z + (double)(fr * cz.dz);
-    const double t64 = fmax(pq2d * m.ek64, -64.0 * 1020.0); // 64 t, t = -|p-q|^2/k^2 log2(e)
+    const double t64 = pq2d * m.ek64; // 64 t, t = -|p-q|^2/k^2 log2(e)
     const double tn = rint(t64);
     const int jn = (int)tn;
     const double r = (t64 - tn) * (kLn2 / 64.0);
     const double pe = fma(fma(fma(fma(r, 1.0 / 24.0, 1.0 / 6.0), r, 0.5), r, 1.0), r, 1.0);
-    const double R = m.k2 * ldexp(pe * m.exp2t[jn & 63], jn >> 6);
+    const double R = ldexp(pe * m.exp2t[jn & 63], jn >> 6);
     return dev::rcp_seeded(fma(Z, pq2d, R));
+  }
+
+  // exp(-t), t >= 0, to FP64 accuracy: t on a 1/64 grid (2^(j/64) from the LDS table) and a degree-6
+  // polynomial of the remainder |r| <= ln2/128 (truncation below 1e-16 relative).
+  __device__ __forceinline__ double exp_neg_tab(double t, const double *exp2t)
+  {
+    const double t64 = fmax(-t * (64.0 * kLog2e), -64.0 * 1020.0);
+    const double tn = rint(t64);
+    const int jn = (int)tn;
+    const double r = (t64 - tn) * (kLn2 / 64.0);
+    double pe = fma(r, 1.0 / 720.0, 1.0 / 120.0);
+    pe = fma(pe, r, 1.0 / 24.0);
+    pe = fma(pe, r, 1.0 / 6.0);
+    pe = fma(pe, r, 0.5);
+    pe = fma(pe, r, 1.0);
+    pe = fma(pe, r, 1.0);
+    return ldexp(pe * exp2t[jn & 63], jn >> 6);
+  }
+
+  // log1p(t) to FP64 accuracy from the FP32 hardware log2 and one Newton step on e^l = 1 + t,
+  // with 1 + t carried exactly as w + err (TwoSum).  Quadratic convergence from the ~1e-7 seed
+  // leaves ~1e-14 absolute.  t <= -1 gives NaN, which the callers' clamps map to index 0.
+  __device__ __forceinline__ double log1p_seeded(double t, const double *exp2t)
+  {
+    const double w = 1.0 + t;
+    const double bv = w - 1.0;
+    const double err = (1.0 - (w - bv)) + (t - bv);
+    const double l0 = (double)(__builtin_amdgcn_logf((float)w) * (float)kLn2);
+    const double e = exp_neg_tab(l0, exp2t);
+    return l0 + fma(w, e, -1.0) + err * e;
   }
 
   // Sum over the block's share of (cos1, cos2) pairs and radial nodes of one tile.  Lanes own
@@ -343,17 +373,22 @@ namespace
   {
     constexpr bool F64 = MODE == DFRG_FP64;
     __shared__ double s_tab64[F64 ? kMaxTab : 1];
-    __shared__ double2 s_tabA[F64 ? 1 : kMaxTab];
+    __shared__ SECellY s_tabA[F64 ? 1 : kMaxTab];
     __shared__ SECellZ s_tabB[F64 ? 1 : kMaxTab];
-    __shared__ double s_exp2[F64 ? 1 : 64];
+    __shared__ double s_exp2[F64 ? 1 : 64];  // 2^(j/64)
+    __shared__ double s_exp2k[F64 ? 1 : 64]; // k^2 2^(j/64)
     __shared__ double2 s_ang[kMaxAng]; // (cos1, 2 w1 sqrt(1 - cos1^2))
     __shared__ double s_w2[kMaxAng];   // 2 w2
     __shared__ double s_qs[kMaxQ], s_qc[kMaxQ], s_qt[kMaxQ];
+    __shared__ double2 s_qw[kMaxQ]; // (q, w_q x_extent)
     __shared__ double s_red[BLOCK / 64];
     __shared__ int s_tile;
 
     if (!F64)
-      for (int j = threadIdx.x; j < 64; j += BLOCK) s_exp2[j] = exp2(j * (1.0 / 64.0));
+      for (int j = threadIdx.x; j < 64; j += BLOCK) {
+        s_exp2[j] = exp2(j * (1.0 / 64.0));
+        s_exp2k[j] = A.k * A.k * s_exp2[j];
+      }
     const double k = A.k, k2 = k * k;
     int cur = -1; // integral whose table and angular terms are in LDS (block-uniform)
     for (int iter = 0;; ++iter) {
@@ -393,23 +428,37 @@ namespace
         const double q = k * sqrt(xq[T.q0 + j] * A.x_extent);
         s_qs[j] = T.p * T.p + q * q;
         s_qc[j] = -2.0 * T.p * q;
+        s_qw[j] = make_double2(q, xq[T.nq + T.q0 + j] * A.x_extent);
       }
       __syncthreads();
-      // q-only factor in FP64 from the staged table: el * w_q x_ext * RBdot(q) G(q)^2
+      // q-only factor from the staged table: el * w_q x_ext * RBdot(q) G(q)^2.  Plain mode: the
+      // reference's FP64 expression tree.  Mixed mode: FP64-accurate seeded forms (FP32 hardware
+      // log2 / reciprocal seeds with FP64 Newton steps, table exp), since this factor multiplies
+      // every evaluation of its radial node and its error would not average out.
       for (int j = threadIdx.x; j < T.nqt; j += BLOCK) {
-        const int iq = T.q0 + j;
-        const double q = k * sqrt(xq[iq] * A.x_extent);
-        const double q2 = q * q;
-        const double xr = q2 / k2;
-        const double ex = exp(-xr);
-        double Zq;
-        if (F64)
-          Zq = dev::lerp_log_f64(s_tab64, T.ntab, T.c, T.inv_b, T.scale, q);
-        else
-          Zq = lerp_log_cells([&](int m) { return s_tabB[m].z; }, T.ntab, T.c, T.inv_b, T.scale, q);
-        const double Gq = 1.0 / (Zq * q2 + k2 * ex);
-        const double el = A.pref * (q2 * 0.5 * k2);
-        s_qt[j] = el * (xq[T.nq + iq] * A.x_extent) * (2.0 * ex * k2 * (1.0 + xr)) * Gq * Gq;
+        const double2 qw = s_qw[j];
+        const double q = qw.x, q2 = q * q;
+        double qt;
+        if (F64) {
+          const double xr = q2 / k2;
+          const double ex = exp(-xr);
+          const double Zq = dev::lerp_log_f64(s_tab64, T.ntab, T.c, T.inv_b, T.scale, q);
+          const double Gq = 1.0 / (Zq * q2 + k2 * ex);
+          const double el = A.pref * (q2 * 0.5 * k2);
+          qt = el * qw.y * (2.0 * ex * k2 * (1.0 + xr)) * Gq * Gq;
+        } else {
+          const double xr = q2 * dev::rcp_seeded(k2);
+          const double ex = exp_neg_tab(xr, s_exp2);
+          double idx = log1p_seeded((q - T.c) * T.inv_b, s_exp2) * T.scale;
+          idx = fmax(0.0, fmin(idx, (double)(T.ntab - 1)));
+          const double fl = floor(idx);
+          const int i0 = (int)fl, i1 = (int)ceil(idx);
+          const double Zq = s_tabB[i0].z * (1.0 - idx + fl) + s_tabB[i1].z * (idx - fl);
+          const double Gq = dev::rcp_seeded(Zq * q2 + k2 * ex);
+          const double el = A.pref * (q2 * 0.5 * k2);
+          qt = el * qw.y * (2.0 * ex * k2 * (1.0 + xr)) * Gq * Gq;
+        }
+        s_qt[j] = qt;
       }
       cur = i;
       __syncthreads();
@@ -422,13 +471,12 @@ namespace
         SEMixed m;
         m.tabA = s_tabA;
         m.tabB = s_tabB;
-        m.exp2t = s_exp2;
+        m.exp2t = s_exp2k;
         m.cf = (float)T.c;
         m.ibf = (float)T.inv_b;
         m.scl = (float)(T.scale * kLn2);
         m.sf = (float)T.scale;
         m.ek64 = -64.0 * kLog2e / k2;
-        m.k2 = k2;
         m.imax = T.ntab - 2;
         const auto g = [&](double pq2) { return se_G_mixed(m, pq2); };
         acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
@@ -487,7 +535,7 @@ struct dfrg_se_batch {
   unsigned long long evals = 0;
   // device data
   double *d_nodes = nullptr, *d_tab64 = nullptr;
-  double2 *d_tabA = nullptr;
+  SECellY *d_tabA = nullptr;
   SECellZ *d_tabB = nullptr;
   SETile *d_tiles = nullptr;
   double *d_partials = nullptr;
@@ -744,7 +792,7 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
   }
   // cell data of the mixed path (coordinates.hh:141-196 LogarithmicCoordinates1D): node
   // y_i = b expm1(a i/(n-1)) + c, 1/(b + y_i - c), Z_i and Z_{i+1} - Z_i
-  std::vector<double2> tabA(tables_len);
+  std::vector<SECellY> tabA(tables_len);
   std::vector<SECellZ> tabB(tables_len);
   for (int i = 0; i < n_int; ++i) {
     const double a = bias[i], bb = (stop[i] - start[i]) / std::expm1(a), cc = start[i];
@@ -753,7 +801,7 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
       const double yj = bb * std::expm1(a * j * (1.0 / (n - 1.0))) + cc;
       const double v = tables[tab_off[i] + j];
       const double nv = (j + 1 < n) ? tables[tab_off[i] + j + 1] : v;
-      tabA[tab_off[i] + j] = make_double2(yj, 1.0 / (bb + yj - cc));
+      tabA[tab_off[i] + j] = SECellY{yj * yj, (float)(1.0 / (bb + yj - cc)), (float)yj};
       tabB[tab_off[i] + j] = SECellZ{v, (float)(nv - v), 0.0f};
     }
   }
@@ -771,13 +819,13 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
   } while (0)
   DFRG_HIP_B(hipMalloc((void **)&b->d_nodes, sizeof(double) * nodes.size()));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tab64, sizeof(double) * tables_len));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_tabA, sizeof(double2) * tables_len));
+  DFRG_HIP_B(hipMalloc((void **)&b->d_tabA, sizeof(SECellY) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tabB, sizeof(SECellZ) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_counters, sizeof(unsigned) * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_queue, sizeof(unsigned) * 2));
   DFRG_HIP_B(hipMemcpy(b->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tab64, tables, sizeof(double) * tables_len, hipMemcpyHostToDevice));
-  DFRG_HIP_B(hipMemcpy(b->d_tabA, tabA.data(), sizeof(double2) * tables_len, hipMemcpyHostToDevice));
+  DFRG_HIP_B(hipMemcpy(b->d_tabA, tabA.data(), sizeof(SECellY) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tabB, tabB.data(), sizeof(SECellZ) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemset(b->d_counters, 0, sizeof(unsigned) * n_int));
   DFRG_HIP_B(hipMemset(b->d_queue, 0, sizeof(unsigned) * 2));
