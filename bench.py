@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--shape", action="append", default=[], metavar="MODE:BLOCK,TILE,GRID",
                     help="fixed launch shape instead of the autotuner for the headline (MODE fp64|mixed); "
                          "for counter runs, where serialised dispatches distort the tuner's timings")
+    ap.add_argument("--eager-only", action="store_true",
+                    help="headline: time eager launches only (no hipGraph or multi-stream runs), so a kernel trace "
+                         "ends with exactly the timed launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -132,6 +135,10 @@ N_STREAMS = 4
 BATCH = 8  # call sets per launch in the batched measurements of the small workloads
 
 
+def finite(v):
+    return v if math.isfinite(v) and v != 0.0 else None
+
+
 def rel_err(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.abs(np.asarray(b))))
 
@@ -176,7 +183,7 @@ def selfenergy(D, qp, args):
     ref = ref_fn()
     res = {}
     ms_objs = []
-    for _ in range(N_STREAMS):
+    for _ in range(0 if args.eager_only else N_STREAMS):
         ms_objs.append((D.SEBatch(qp, c["p"], c["nq"], c["na"], [c["Z"]] * n, *[[v] * n for v in c["coord"]],
                                   x_extent=c["x_extent"], k=c["k"]), D.DeviceArray(n), D.Stream()))
     fixed = {m.split(":")[0]: [int(v) for v in m.split(":")[1].split(",")] for m in args.shape}
@@ -188,19 +195,22 @@ def selfenergy(D, qp, args):
             shape = b.autotune(mode, reps=5)
         ms_eager, _ = time_steps(D, lambda: b.run(out, mode), args.steps, args.warmup)
         err_eager = rel_err(out.to_host(), ref)
-        # the same call sets replayed from a hipGraph of GRAPH_SETS launches
-        per = GRAPH_SETS
-        ms_graph, _ = time_steps(D, lambda: b.graph_launch(out, mode, launches=per), max(1, args.steps // per),
-                                 max(1, args.warmup // per))
-        ms_graph /= per
-        err = max(err_eager, rel_err(out.to_host(), ref))
-        # independent call sets on N_STREAMS streams, one batch object (own scratch) per stream
-        fns = [(lambda bb=bb, o=o, st=st: bb.run(o, mode, stream=st.ptr), st) for bb, o, st in ms_objs]
-        for bb, _, _ in ms_objs:
-            bb.set_shape(*[shape[k] for k in ("block", "tile_evals", "grid")])
-        ms_streams = time_streams(D, fns, args.steps, args.warmup)
-        for _, o, _ in ms_objs:
-            err = max(err, rel_err(o.to_host(), ref))
+        err = err_eager
+        ms_graph = ms_streams = float("inf")
+        if not args.eager_only:
+            # the same call sets replayed from a hipGraph of GRAPH_SETS launches
+            per = GRAPH_SETS
+            ms_graph, _ = time_steps(D, lambda: b.graph_launch(out, mode, launches=per), max(1, args.steps // per),
+                                     max(1, args.warmup // per))
+            ms_graph /= per
+            err = max(err, rel_err(out.to_host(), ref))
+            # independent call sets on N_STREAMS streams, one batch object (own scratch) per stream
+            fns = [(lambda bb=bb, o=o, st=st: bb.run(o, mode, stream=st.ptr), st) for bb, o, st in ms_objs]
+            for bb, _, _ in ms_objs:
+                bb.set_shape(*[shape[k] for k in ("block", "tile_evals", "grid")])
+            ms_streams = time_streams(D, fns, args.steps, args.warmup)
+            for _, o, _ in ms_objs:
+                err = max(err, rel_err(o.to_host(), ref))
         ms = min(ms_eager, ms_graph)
         res[name] = {"ms": ms, "ms_eager": ms_eager, "ms_graph": ms_graph, "ms_streams": ms_streams, "shape": shape,
                      "err": err, "launch": "hipGraph" if ms_graph < ms_eager else "eager"}
@@ -429,12 +439,12 @@ def main():
                    "launch": f"{mx['launch']} (eager launches or hipGraph of {GRAPH_SETS} call sets, the faster)"},
         "mixed": {"integrals_per_s": n / (mx["ms"] * 1e-3), "evals_per_s": evals / (mx["ms"] * 1e-3),
                   "max_rel_err_vs_fp64_oracle": mx["err"], "shape": mx["shape"], "launch": mx["launch"],
-                  "us_eager": mx["ms_eager"] * 1e3, "us_graph": mx["ms_graph"] * 1e3,
-                  f"integrals_per_s_{N_STREAMS}_streams": n / (mx["ms_streams"] * 1e-3)},
+                  "us_eager": mx["ms_eager"] * 1e3, "us_graph": finite(mx["ms_graph"] * 1e3),
+                  f"integrals_per_s_{N_STREAMS}_streams": finite(n / (mx["ms_streams"] * 1e-3))},
         "plain_fp64": {"integrals_per_s": n / (fp["ms"] * 1e-3), "evals_per_s": evals / (fp["ms"] * 1e-3),
                        "ms_per_step": fp["ms"], "max_rel_err_vs_oracle": fp["err"], "shape": fp["shape"],
-                       "launch": fp["launch"], "us_eager": fp["ms_eager"] * 1e3, "us_graph": fp["ms_graph"] * 1e3,
-                       f"integrals_per_s_{N_STREAMS}_streams": n / (fp["ms_streams"] * 1e-3)},
+                       "launch": fp["launch"], "us_eager": fp["ms_eager"] * 1e3, "us_graph": finite(fp["ms_graph"] * 1e3),
+                       f"integrals_per_s_{N_STREAMS}_streams": finite(n / (fp["ms_streams"] * 1e-3))},
         "mixed_over_plain": fp["ms"] / mx["ms"],
     }
     achieved = SE_FLOP[1] * evals / (mx["ms"] * 1e-3) / 1e12
