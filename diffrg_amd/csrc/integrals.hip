@@ -257,7 +257,7 @@ namespace
     const double *exp2t; // 2^(j/64), j = 0..63
     float cf, ibf, scl, sf;
     double ek64;
-    int imax;
+    float imaxf; // n - 2
   };
 
   // G(|p-q|) in mixed precision: FP32 hardware seeds replace the FP64 library sequences, FP64 is
@@ -272,8 +272,9 @@ namespace
   __device__ __forceinline__ double se_G_mixed(const SEMixed &m, double pq2d)
   {
     const float ys = __builtin_amdgcn_sqrtf((float)pq2d);
+    // cell index: clamp to [0, n-2] first (NaN-safe), then truncation is floor
     const float coarse = __builtin_amdgcn_logf(fmaf(ys - m.cf, m.ibf, 1.0f)) * m.scl;
-    const int i0 = min(max((int)__builtin_floorf(coarse), 0), m.imax);
+    const int i0 = (int)fminf(fmaxf(coarse, 0.0f), m.imaxf);
     const SECellY ca = m.tabA[i0];
     const SECellZ cz = m.tabB[i0];
     const float num = (float)(pq2d - ca.y2);
@@ -477,7 +478,7 @@ namespace
         m.scl = (float)(T.scale * kLn2);
         m.sf = (float)T.scale;
         m.ek64 = -64.0 * kLog2e / k2;
-        m.imax = T.ntab - 2;
+        m.imaxf = (float)(T.ntab - 2);
         const auto g = [&](double pq2) { return se_G_mixed(m, pq2); };
         acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
       }

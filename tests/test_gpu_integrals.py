@@ -158,6 +158,28 @@ def test_finite_T_fp64_and_mixed(gpu, qp):
         assert err.max() <= tol
 
 
+@pytest.mark.parametrize("block", [64, 128, 256, 512])
+def test_finite_T_block_sizes(gpu, qp, block):
+    # 100 Matsubara nodes (a full and a partial 64-lane chunk) and 37 radial nodes (not a multiple
+    # of the waves per integral)
+    c = W.finite_T(n=64, nx=37, n_max=100)
+    dev, x, w = finite_T_device(gpu, c, qp)
+    mats = [c["rules"][t] for t in c["T"]]
+    ref = O.c2_finiteT(c["T"], c["p0"], c["pabs"], c["m1"], c["m2"], mats, x, w, c["q_max"], c["k"], 1.0, c["d"],
+                       c["radial_mode"])
+    out = gpu.DeviceArray(len(c["T"]))
+    gpu.ft_set_block(block)
+    try:
+        for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+            print(f"[case] finite-T 64x37x201 block={block} mode={mode}", flush=True)
+            gpu.ft_batch(dev, out, c["q_max"], c["k"], mode, c["d"], c["radial_mode"])
+            assert rel_err(out.to_host(), ref).max() <= tol
+    finally:
+        gpu.ft_set_block(0)
+    with pytest.raises(gpu.DfrgInvalidArgument):
+        gpu.ft_set_block(96)
+
+
 def test_finite_T_reference_mapping(gpu, qp):
     # radial_mode 0: the reference's x = q^2/k^2 mapping with a Monien rule from the provider
     c = W.finite_T(n=32)
