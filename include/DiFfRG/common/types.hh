@@ -6,6 +6,10 @@
 
 namespace DiFfRG
 {
+  // anything indexable, e.g. a grid of external momenta (types.hh:16 in the reference)
+  template <typename T>
+  concept IsContainer = requires(T x) { x[0]; };
+
   namespace get_type
   {
     namespace internal

@@ -8,6 +8,14 @@
 #include <stdexcept>
 #include <vector>
 
+// coordinate systems: a ctype, a dimension and a size (reference coordinates.hh:15-19)
+template <typename T>
+concept IsCoordinate = requires(T t) {
+  typename T::ctype;
+  T::dim;
+  t.size();
+};
+
 namespace DiFfRG
 {
   template <typename NT> class LinearCoordinates1D

@@ -1,11 +1,14 @@
 // DiFfRG/physics/integration/integrator_4D_gpu.hh -- Integrator4DGPU<NT, KERNEL>
 // (reference integrator_4D_gpu.hh:62-352):
 //   int dq dcos1 dcos2 dphi (2pi)^-4 q^2/2 k^2 sqrt(1 - cos1^2) KERNEL::kernel(q, cos1, cos2, phi, k, t...) + constant
-// with q^2 in [0, x_extent k^2].  One launch per call; the three angular orders must be equal.
+// with q^2 in [0, x_extent k^2].  One launch per get() and per batch of request()s; the three
+// angular orders must be equal.
 #pragma once
 
 #include <array>
 #include <future>
+#include <memory>
+#include <tuple>
 #include <stdexcept>
 
 #include <DiFfRG/common/json.hh>
@@ -43,10 +46,21 @@ namespace DiFfRG
 
     template <typename... T> NT get(const ctype k, const T &...t) const { return launch(k, t...).wait(); }
 
+    // Deferred: requests are batched, and reading one future launches the whole batch
+    // (dfrg::tmpl::RequestBatcher); results are bitwise those of get().
     template <typename... T> std::future<NT> request(const ctype k, const T &...t) const
     {
-      auto p = launch(k, t...);
-      return std::async(std::launch::deferred, [p]() mutable { return p.wait(); });
+      using Args = std::tuple<ctype, T...>;
+      const long long na = grid_sizes[1];
+      const long long nodes = (long long)grid_sizes[0] * na * na * na;
+      return batcher->enqueue(0ULL, dfrg::tmpl::grid_for(nodes), KERNEL::constant(k, t...), Args(k, t...), [this]() {
+        return [xq = ptr_x_quadrature_p, wq = ptr_x_quadrature_w, xa = ptr_ang_quadrature_p,
+                wa = ptr_ang_quadrature_w, nx = (int)grid_sizes[0], na = (int)grid_sizes[1],
+                xe = x_extent](const Args *a, int r, int nb, NT *part, unsigned *cnt, NT *res, hipStream_t s) {
+          hipLaunchKernelGGL((dfrg::tmpl::angle3_batch_kernel<ctype, NT, KERNEL, Args>), dim3(r * nb),
+                             dim3(dfrg::tmpl::kBlock), 0, s, xq, wq, xa, wa, nx, na, xe, a, nb, part, cnt, res);
+        };
+      });
     }
 
   private:
@@ -67,5 +81,6 @@ namespace DiFfRG
     std::array<uint, 4> grid_sizes;
     ctype x_extent;
     const ctype *ptr_x_quadrature_p, *ptr_x_quadrature_w, *ptr_ang_quadrature_p, *ptr_ang_quadrature_w;
+    std::shared_ptr<dfrg::tmpl::RequestBatcher<NT>> batcher = std::make_shared<dfrg::tmpl::RequestBatcher<NT>>();
   };
 } // namespace DiFfRG

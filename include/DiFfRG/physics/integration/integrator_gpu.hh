@@ -1,11 +1,14 @@
 // DiFfRG/physics/integration/integrator_gpu.hh -- IntegratorGPU<d, NT, KERNEL>: radial d-dim
-// momentum integral (reference integrator_gpu.hh:21-176), one HIP launch per call.
+// momentum integral (reference integrator_gpu.hh:21-176): one HIP launch per get(), one per batch
+// of request()s.
 //   get(k, t...) = KERNEL::constant(k, t...) + sum_j S_d q_j^{d-2}/2 k^2/(2pi)^d w_j x_ext KERNEL::kernel(q_j, k, t...)
 // with q_j = k sqrt(x_j x_ext) on the provider's Gauss-Legendre nodes x_j in [0,1].
 #pragma once
 
 #include <array>
 #include <future>
+#include <memory>
+#include <tuple>
 
 #include <DiFfRG/common/json.hh>
 #include <DiFfRG/common/math.hh>
@@ -43,10 +46,19 @@ namespace DiFfRG
 
     template <typename... T> NT get(const ctype k, const T &...t) const { return launch(k, t...).wait(); }
 
+    // Deferred: requests are batched, and reading one future launches the whole batch
+    // (dfrg::tmpl::RequestBatcher); results are bitwise those of get().
     template <typename... T> std::future<NT> request(const ctype k, const T &...t) const
     {
-      auto p = launch(k, t...);
-      return std::async(std::launch::deferred, [p]() mutable { return p.wait(); });
+      using Args = std::tuple<ctype, T...>;
+      const int nb = dfrg::tmpl::grid_for(grid_size);
+      return batcher->enqueue(0ULL, nb, KERNEL::constant(k, t...), Args(k, t...), [this]() {
+        return [xq = ptr_x_quadrature_p, wq = ptr_x_quadrature_w, n = (int)grid_size, xe = x_extent](
+                   const Args *a, int r, int nb, NT *part, unsigned *cnt, NT *res, hipStream_t s) {
+          hipLaunchKernelGGL((dfrg::tmpl::radial_batch_kernel<d, ctype, NT, KERNEL, Args>), dim3(r * nb),
+                             dim3(dfrg::tmpl::kBlock), 0, s, xq, wq, n, xe, a, nb, part, cnt, res);
+        };
+      });
     }
 
   private:
@@ -66,5 +78,6 @@ namespace DiFfRG
     ctype x_extent;
     const ctype *ptr_x_quadrature_p;
     const ctype *ptr_x_quadrature_w;
+    std::shared_ptr<dfrg::tmpl::RequestBatcher<NT>> batcher = std::make_shared<dfrg::tmpl::RequestBatcher<NT>>();
   };
 } // namespace DiFfRG

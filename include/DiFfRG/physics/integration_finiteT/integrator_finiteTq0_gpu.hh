@@ -10,6 +10,8 @@
 #include <cmath>
 #include <future>
 #include <limits>
+#include <memory>
+#include <tuple>
 
 #include <DiFfRG/common/json.hh>
 #include <DiFfRG/common/math.hh>
@@ -59,22 +61,41 @@ namespace DiFfRG
 
     template <typename... T> NT get(const ctype k, const T &...t) { return launch(k, t...).wait(); }
 
+    // Deferred: requests are batched, and reading one future launches the whole batch
+    // (dfrg::tmpl::RequestBatcher); a change of the Matsubara rule (T, or E following k) starts a
+    // new batch.  Results are bitwise those of get().
     template <typename... T> std::future<NT> request(const ctype k, const T &...t)
     {
-      auto p = launch(k, t...);
-      return std::async(std::launch::deferred, [p]() mutable { return p.wait(); });
+      track_E(k);
+      using Args = std::tuple<ctype, T...>;
+      const long long nodes = (long long)grid_sizes[0] * grid_sizes[1];
+      const unsigned long long state = (unsigned long long)(uintptr_t)ptr_matsubara_quadrature_p;
+      return batcher->enqueue(state, dfrg::tmpl::grid_for(nodes), KERNEL::constant(k, t...), Args(k, t...), [this]() {
+        return [xq = ptr_x_quadrature_p, wq = ptr_x_quadrature_w, mx = ptr_matsubara_quadrature_p,
+                mw = ptr_matsubara_quadrature_w, nx = (int)grid_sizes[0], nm = (int)grid_sizes[1], xe = x_extent,
+                temp = m_T](const Args *a, int r, int nb, NT *part, unsigned *cnt, NT *res, hipStream_t s) {
+          hipLaunchKernelGGL((dfrg::tmpl::finiteTq0_batch_kernel<d, ctype, NT, KERNEL, Args>), dim3(r * nb),
+                             dim3(dfrg::tmpl::kBlock), 0, s, xq, wq, mx, mw, nx, nm, xe, temp, a, nb, part, cnt, res);
+        };
+      });
     }
 
     uint matsubara_size() const { return grid_sizes[1]; }
     ctype typical_E() const { return m_E; }
 
   private:
-    template <typename... T> dfrg::tmpl::Pending<NT> launch(const ctype k, const T &...t)
+    // E follows k with a 2.5% hysteresis unless set explicitly (integrator_finiteTq0_gpu.hh:276-281)
+    void track_E(const ctype k)
     {
       if (!manual_E && (std::abs(k - m_E) / std::max(k, m_E) > 2.5e-2)) {
         set_T(m_T, k);
         manual_E = false;
       }
+    }
+
+    template <typename... T> dfrg::tmpl::Pending<NT> launch(const ctype k, const T &...t)
+    {
+      track_E(k);
       auto p = dfrg::tmpl::Pending<NT>::make();
       p.constant = KERNEL::constant(k, t...);
       const long long nodes = (long long)grid_sizes[0] * grid_sizes[1];
@@ -94,5 +115,6 @@ namespace DiFfRG
     bool manual_E;
     const ctype *ptr_x_quadrature_p, *ptr_x_quadrature_w;
     const ctype *ptr_matsubara_quadrature_p = nullptr, *ptr_matsubara_quadrature_w = nullptr;
+    std::shared_ptr<dfrg::tmpl::RequestBatcher<NT>> batcher = std::make_shared<dfrg::tmpl::RequestBatcher<NT>>();
   };
 } // namespace DiFfRG

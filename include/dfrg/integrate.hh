@@ -17,6 +17,12 @@
 #include <cstdint>
 #include <cstring>
 #include <future>
+#include <stdexcept>
+#include <vector>
+#include <mutex>
+#include <memory>
+#include <functional>
+#include <tuple>
 #include <type_traits>
 
 #include <DiFfRG/common/complex_math.hh>
@@ -88,38 +94,41 @@ namespace dfrg
         return NT(dev::fetch(&p->re), dev::fetch(&p->im));
     }
 
-    // Publish this block's partial; the last block adds all partials in block order and writes
-    // *result (then resets the counter for the next use of the scratch block).
-    template <class NT> __device__ __forceinline__ void finish_blocks(NT part, NT *partials, unsigned *counter,
-                                                                      NT *result)
+    // Publish this block's partial in slot `slot` of a group of n_slots; the last block of the group
+    // adds the partials in slot order and writes *result (then resets the counter for the next use
+    // of the scratch block).
+    template <class NT>
+    __device__ __forceinline__ void finish_slots(NT part, NT *partials, int slot, int n_slots, unsigned *counter,
+                                                 NT *result)
     {
       __shared__ bool last;
       if (threadIdx.x == 0) {
-        publish_nt(&partials[blockIdx.x], part);
+        publish_nt(&partials[slot], part);
         dev::wait_stores();
-        last = dev::take_ticket(counter) == gridDim.x - 1;
+        last = dev::take_ticket(counter) == (unsigned)(n_slots - 1);
       }
       __syncthreads();
       if (last && threadIdx.x == 0) {
         NT s = fetch_nt(&partials[0]);
-        for (unsigned i = 1; i < gridDim.x; ++i) s += fetch_nt(&partials[i]);
+        for (int i = 1; i < n_slots; ++i) s += fetch_nt(&partials[i]);
         *result = s;
         dev::reset_counter(counter);
       }
     }
 
     // ------------------------------------------------------------------------------------------
-    // node kernels
+    // per-thread node sums.  Block b of nb takes nodes b*kBlock + tid, stepping nb*kBlock; a
+    // single call launches nb blocks, a batch launches nb blocks per request with the same
+    // partition, so a batched result is bitwise the single call's.
     // ------------------------------------------------------------------------------------------
     // IntegratorGPU<d> (integrator_gpu.hh:34-60)
     template <int d, class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void radial_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                            int nx, ctype x_extent, ctype k, NT *partials,
-                                                            unsigned *counter, NT *result, T... t)
+    __device__ __forceinline__ NT radial_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq, int nx,
+                                             ctype x_extent, int b, int nb, ctype k, const T &...t)
     {
       NT acc = NT(0);
       constexpr ctype S_d = DiFfRG::S_d_prec<ctype>(d);
-      for (int i = blockIdx.x * kBlock + threadIdx.x; i < nx; i += gridDim.x * kBlock) {
+      for (int i = b * kBlock + threadIdx.x; i < nx; i += nb * kBlock) {
         const ctype weight = wq[i] * x_extent;
         const ctype q = k * sqrt(xq[i] * x_extent);
         const ctype int_element = S_d * (DiFfRG::powr<d - 2>(q) / 2 * DiFfRG::powr<2>(k)) /
@@ -127,19 +136,18 @@ namespace dfrg
         const NT res = KERNEL::kernel(q, k, t...);
         acc += int_element * res * weight;
       }
-      finish_blocks(block_reduce(acc), partials, counter, result);
+      return acc;
     }
 
     // Integrator4D2AngGPU (integrator_4D_2ang_gpu.hh:20-45): node (q, cos1, cos2), cos2 fastest
     template <class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void angle2_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
-                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
-                                                            unsigned *counter, NT *result, T... t)
+    __device__ __forceinline__ NT angle2_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                             const ctype *__restrict__ xa, const ctype *__restrict__ wa, int nx,
+                                             int na, ctype x_extent, int b, int nb, ctype k, const T &...t)
     {
       NT acc = NT(0);
       const long long W = (long long)nx * na * na;
-      for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < W; i += (long long)gridDim.x * kBlock) {
+      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
         const int i2 = (int)(i % na);
         const long long r = i / na;
         const int i1 = (int)(r % na);
@@ -154,19 +162,18 @@ namespace dfrg
         const ctype cos2 = 2 * (xa[i2] - (ctype)0.5);
         acc += KERNEL::kernel(q, cos1, cos2, k, t...) * int_element * weight;
       }
-      finish_blocks(block_reduce(acc), partials, counter, result);
+      return acc;
     }
 
     // Integrator4DGPU (integrator_4D_gpu.hh:20-49): node (q, cos1, phi, cos2), cos2 fastest
     template <class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void angle3_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
-                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
-                                                            unsigned *counter, NT *result, T... t)
+    __device__ __forceinline__ NT angle3_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                             const ctype *__restrict__ xa, const ctype *__restrict__ wa, int nx,
+                                             int na, ctype x_extent, int b, int nb, ctype k, const T &...t)
     {
       NT acc = NT(0);
       const long long W = (long long)nx * na * na * na;
-      for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < W; i += (long long)gridDim.x * kBlock) {
+      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
         const int i2 = (int)(i % na); // cos2
         long long r = i / na;
         const int i3 = (int)(r % na); // phi
@@ -182,22 +189,20 @@ namespace dfrg
         const ctype cos2 = 2 * (xa[i2] - (ctype)0.5);
         acc += KERNEL::kernel(q, cos1, cos2, phi, k, t...) * int_element * weight * 2 * wa[i2];
       }
-      finish_blocks(block_reduce(acc), partials, counter, result);
+      return acc;
     }
 
     // IntegratorFiniteTq0GPU<d> (integrator_finiteTq0_gpu.hh:20-44): node (q, Matsubara n)
     template <int d, class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void finiteTq0_kernel(const ctype *__restrict__ xq,
-                                                               const ctype *__restrict__ wq,
-                                                               const ctype *__restrict__ mx,
-                                                               const ctype *__restrict__ mw, int nx, int nm,
-                                                               ctype x_extent, ctype m_T, ctype k, NT *partials,
-                                                               unsigned *counter, NT *result, T... t)
+    __device__ __forceinline__ NT finiteTq0_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx,
+                                                int nm, ctype x_extent, ctype m_T, int b, int nb, ctype k,
+                                                const T &...t)
     {
       NT acc = NT(0);
       constexpr ctype S_dm1 = DiFfRG::S_d_prec<ctype>(d - 1);
       const long long W = (long long)nx * nm;
-      for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < W; i += (long long)gridDim.x * kBlock) {
+      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
         const int iy = (int)(i % nm);
         const int ix = (int)(i / nm);
         const ctype q = k * sqrt(xq[ix] * x_extent);
@@ -210,7 +215,116 @@ namespace dfrg
         if (m_T > 0 && iy == 0) res += int_element * x_weight * m_T * KERNEL::kernel(q, (ctype)0, k, t...);
         acc += res;
       }
-      finish_blocks(block_reduce(acc), partials, counter, result);
+      return acc;
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // kernels: one call (arguments by value), or a batch of requests whose (k, t...) tuples sit in
+    // device memory, nb blocks per request
+    // ------------------------------------------------------------------------------------------
+    template <int d, class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void radial_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                            int nx, ctype x_extent, ctype k, NT *partials,
+                                                            unsigned *counter, NT *result, T... t)
+    {
+      const NT acc = radial_sum<d, ctype, NT, KERNEL>(xq, wq, nx, x_extent, blockIdx.x, gridDim.x, k, t...);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+    }
+    template <int d, class ctype, class NT, class KERNEL, class Args>
+    __global__ __launch_bounds__(kBlock) void radial_batch_kernel(const ctype *__restrict__ xq,
+                                                                  const ctype *__restrict__ wq, int nx, ctype x_extent,
+                                                                  const Args *args, int nb, NT *partials,
+                                                                  unsigned *counters, NT *results)
+    {
+      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
+      const NT acc = std::apply(
+          [&](const ctype &k, const auto &...t) {
+            return radial_sum<d, ctype, NT, KERNEL>(xq, wq, nx, x_extent, b, nb, k, t...);
+          },
+          args[r]);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+    }
+
+    template <class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void angle2_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
+                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
+                                                            unsigned *counter, NT *result, T... t)
+    {
+      const NT acc = angle2_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+    }
+    template <class ctype, class NT, class KERNEL, class Args>
+    __global__ __launch_bounds__(kBlock) void angle2_batch_kernel(const ctype *__restrict__ xq,
+                                                                  const ctype *__restrict__ wq,
+                                                                  const ctype *__restrict__ xa,
+                                                                  const ctype *__restrict__ wa, int nx, int na,
+                                                                  ctype x_extent, const Args *args, int nb,
+                                                                  NT *partials, unsigned *counters, NT *results)
+    {
+      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
+      const NT acc = std::apply(
+          [&](const ctype &k, const auto &...t) {
+            return angle2_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
+          },
+          args[r]);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+    }
+
+    template <class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void angle3_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
+                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
+                                                            unsigned *counter, NT *result, T... t)
+    {
+      const NT acc = angle3_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+    }
+    template <class ctype, class NT, class KERNEL, class Args>
+    __global__ __launch_bounds__(kBlock) void angle3_batch_kernel(const ctype *__restrict__ xq,
+                                                                  const ctype *__restrict__ wq,
+                                                                  const ctype *__restrict__ xa,
+                                                                  const ctype *__restrict__ wa, int nx, int na,
+                                                                  ctype x_extent, const Args *args, int nb,
+                                                                  NT *partials, unsigned *counters, NT *results)
+    {
+      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
+      const NT acc = std::apply(
+          [&](const ctype &k, const auto &...t) {
+            return angle3_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
+          },
+          args[r]);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+    }
+
+    template <int d, class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void finiteTq0_kernel(const ctype *__restrict__ xq,
+                                                               const ctype *__restrict__ wq,
+                                                               const ctype *__restrict__ mx,
+                                                               const ctype *__restrict__ mw, int nx, int nm,
+                                                               ctype x_extent, ctype m_T, ctype k, NT *partials,
+                                                               unsigned *counter, NT *result, T... t)
+    {
+      const NT acc =
+          finiteTq0_sum<d, ctype, NT, KERNEL>(xq, wq, mx, mw, nx, nm, x_extent, m_T, blockIdx.x, gridDim.x, k, t...);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+    }
+    template <int d, class ctype, class NT, class KERNEL, class Args>
+    __global__ __launch_bounds__(kBlock) void finiteTq0_batch_kernel(const ctype *__restrict__ xq,
+                                                                     const ctype *__restrict__ wq,
+                                                                     const ctype *__restrict__ mx,
+                                                                     const ctype *__restrict__ mw, int nx, int nm,
+                                                                     ctype x_extent, ctype m_T, const Args *args,
+                                                                     int nb, NT *partials, unsigned *counters,
+                                                                     NT *results)
+    {
+      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
+      const NT acc = std::apply(
+          [&](const ctype &k, const auto &...t) {
+            return finiteTq0_sum<d, ctype, NT, KERNEL>(xq, wq, mx, mw, nx, nm, x_extent, m_T, b, nb, k, t...);
+          },
+          args[r]);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
     }
 
     // ------------------------------------------------------------------------------------------
@@ -267,6 +381,116 @@ namespace dfrg
         dfrg_pool_host_free(h_result);
         return constant + v;
       }
+    };
+
+    // ------------------------------------------------------------------------------------------
+    // batched requests.  request() calls of one integrator are collected: their (k, t...) tuples
+    // wait on the host, and reading any of their futures launches one kernel for all of them (nb
+    // blocks per request, the single-call partition) and copies all results back at once.  This
+    // turns a request_data loop over a grid of external points into one launch and one sync.
+    // ------------------------------------------------------------------------------------------
+    constexpr size_t kMaxBatch = size_t(1) << 16;
+
+    struct BatchBase {
+      virtual ~BatchBase() = default;
+    };
+
+    template <class NT, class Args> struct RequestBatch : BatchBase {
+      using Launch = std::function<void(const Args *d_args, int n, int nb, NT *partials, unsigned *counters,
+                                        NT *results, hipStream_t stream)>;
+      std::mutex m;
+      std::vector<Args> args;
+      std::vector<NT> constants, results;
+      int nb = 1;
+      bool done = false;
+      Launch launch;
+
+      NT get(size_t i)
+      {
+        std::lock_guard<std::mutex> lock(m);
+        if (!done) {
+          run();
+          done = true;
+        }
+        return constants[i] + results[i];
+      }
+
+    private:
+      static size_t up(size_t b) { return (b + 255) / 256 * 256; }
+      void run()
+      {
+        const int n = (int)args.size();
+        void *sv = nullptr;
+        DiFfRG::dfrg_check(dfrg_thread_stream(&sv), "thread stream");
+        const hipStream_t s = (hipStream_t)sv;
+        const size_t o_part = up(sizeof(Args) * n);
+        const size_t o_cnt = o_part + up(sizeof(NT) * (size_t)n * nb);
+        const size_t o_res = o_cnt + up(sizeof(unsigned) * n);
+        const size_t bytes = o_res + up(sizeof(NT) * n);
+        const size_t h_res_off = up(sizeof(Args) * n);
+        void *d = nullptr, *h = nullptr;
+        DiFfRG::dfrg_check(dfrg_pool_alloc(&d, bytes), "batch scratch");
+        DiFfRG::dfrg_check(dfrg_pool_host_alloc(&h, h_res_off + sizeof(NT) * n), "batch staging");
+        char *dc = (char *)d, *hc = (char *)h;
+        std::memcpy(hc, (const void *)args.data(), sizeof(Args) * n);
+        bool ok = hipMemcpyAsync(dc, hc, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+                  hipMemsetAsync(dc + o_cnt, 0, sizeof(unsigned) * n, s) == hipSuccess;
+        if (ok) {
+          launch((const Args *)dc, n, nb, (NT *)(dc + o_part), (unsigned *)(dc + o_cnt), (NT *)(dc + o_res), s);
+          ok = hipGetLastError() == hipSuccess &&
+               hipMemcpyAsync(hc + h_res_off, dc + o_res, sizeof(NT) * n, hipMemcpyDeviceToHost, s) == hipSuccess &&
+               hipStreamSynchronize(s) == hipSuccess;
+        }
+        if (ok) {
+          results.resize(n);
+          std::memcpy((void *)results.data(), hc + h_res_off, sizeof(NT) * n);
+        }
+        dfrg_pool_free(d);
+        dfrg_pool_host_free(h);
+        args.clear();
+        args.shrink_to_fit();
+        if (!ok) throw std::runtime_error("integrator: batched request launch failed");
+      }
+    };
+
+    // Collects the requests of one integrator into batches.  A new batch opens when the argument
+    // types or the integrator state that the launch depends on (`state`, e.g. the Matsubara rule)
+    // change, when the open batch has been read, or at kMaxBatch requests.
+    template <class NT> class RequestBatcher
+    {
+    public:
+      template <class Args, class MakeLaunch>
+      std::future<NT> enqueue(unsigned long long state, int nb, NT constant, Args &&args, MakeLaunch &&make_launch)
+      {
+        using B = RequestBatch<NT, std::decay_t<Args>>;
+        std::shared_ptr<B> b;
+        size_t idx = 0;
+        {
+          std::lock_guard<std::mutex> lock(m);
+          b = std::dynamic_pointer_cast<B>(open);
+          for (int attempt = 0;; ++attempt) {
+            if (!b || state != open_state || attempt > 0) {
+              b = std::make_shared<B>();
+              b->nb = nb;
+              b->launch = make_launch();
+              open = b;
+              open_state = state;
+            }
+            std::lock_guard<std::mutex> lb(b->m);
+            if (b->done || b->args.size() >= kMaxBatch) continue; // read already, or full: open another
+            idx = b->args.size();
+            b->args.push_back(std::forward<Args>(args));
+            b->constants.push_back(constant);
+            break;
+          }
+        }
+        return std::async(std::launch::deferred, [b, idx]() { return b->get(idx); });
+      }
+
+    private:
+      std::mutex m;
+      std::shared_ptr<BatchBase> open;
+      unsigned long long open_state = 0;
     };
   } // namespace tmpl
 } // namespace dfrg

@@ -17,7 +17,9 @@
 #include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_gpu.hh>
 #include <DiFfRG/physics/interpolation/linear_interpolation_1D.hh>
 #include <DiFfRG/physics/regulators.hh>
+#include <DiFfRG/physics/utils.hh>
 
+#include <chrono>
 #include <memory>
 #include <string>
 #include <vector>
@@ -348,6 +350,105 @@ int kat_finiteT_twoprop(int nx, double x_extent, double k, const double *T, cons
 int kat_unequal_angles()
 {
   return guard([&] { Integrator4D2AngGPU<double, Poly2Ang> integ(provider(), {{16, 8, 16}}, 1.0); });
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// batched requests
+// ---------------------------------------------------------------------------------------------
+namespace
+{
+  // the shape of a generated X_integrator wrapper (Examples/*/flows/*/X.hh): templated on the
+  // result type, forwarding to the integrator
+  struct SelfEnergyWrapper {
+    Integrator4D2AngGPU<double, SelfEnergyKernel> integ;
+    template <typename NT, typename... T> std::future<NT> request(double k, const T &...t)
+    {
+      return integ.request(k, t...);
+    }
+    template <typename NT, typename... T> NT get(double k, const T &...t) { return integ.get(k, t...); }
+  };
+} // namespace
+
+extern "C" {
+
+// the self-energy call set with one get() per external momentum (one launch each)
+int kat_selfenergy_get(int nq, int na, double x_extent, double k, const double *p, int n_p, const double *Z,
+                       int n_tab, double start, double stop, double bias, double *out)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<double> coords((uint)n_tab, start, stop, bias);
+    ZInterp interp(Z, coords);
+    Integrator4D2AngGPU<double, SelfEnergyKernel> integ(provider(), {{(uint)nq, (uint)na, (uint)na}}, x_extent);
+    for (int i = 0; i < n_p; ++i) out[i] = integ.get(k, p[i], interp);
+  });
+}
+
+// request_data / update_data and get_data over the external momenta of a log grid
+int kat_request_data(int nq, int na, double x_extent, double k, int n_p, double p_start, double p_stop,
+                     double p_bias, const double *Z, int n_tab, double start, double stop, double bias,
+                     double *out_req, double *out_get, double *p_out)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<double> coords((uint)n_tab, start, stop, bias);
+    LogarithmicCoordinates1D<double> pgrid((uint)n_p, p_start, p_stop, p_bias);
+    ZInterp interp(Z, coords);
+    SelfEnergyWrapper w{Integrator4D2AngGPU<double, SelfEnergyKernel>(provider(), {{(uint)nq, (uint)na, (uint)na}},
+                                                                       x_extent)};
+    auto futures = request_data<double>(w, pgrid, k, interp);
+    update_data(futures, out_req);
+    get_data<double>(out_get, w, make_grid(pgrid), k, std::tuple<ZInterp>(interp));
+    const auto g = make_grid(pgrid);
+    for (int i = 0; i < n_p; ++i) p_out[i] = g[i];
+  });
+}
+
+// finite-T requests over several k (E follows k, so the rule and the batch change on the way)
+// against get() at the same k
+int kat_finiteT_requests(int nx, double x_extent, double T, const double *ks, int n, const double *c,
+                         const double *den, double *out_req, double *out_get)
+{
+  return guard([&] {
+    IntegratorFiniteTq0GPU<4, double, PolyFiniteT> a(provider(), {{(uint)nx}}, x_extent, T);
+    IntegratorFiniteTq0GPU<4, double, PolyFiniteT> b(provider(), {{(uint)nx}}, x_extent, T);
+    std::vector<std::future<double>> f;
+    for (int i = 0; i < n; ++i)
+      f.push_back(a.request(ks[i], 0.0, c[0], c[1], c[2], c[3], c[4], c[5], den[0], den[1], den[2], den[3]));
+    for (int i = 0; i < n; ++i)
+      out_get[i] = b.get(ks[i], 0.0, c[0], c[1], c[2], c[3], c[4], c[5], den[0], den[1], den[2], den[3]);
+    for (int i = 0; i < n; ++i) out_req[i] = f[i].get();
+  });
+}
+
+// wall-clock seconds of n_p get() calls and of n_p requests read back (one batch), repeated reps
+// times each after one warm-up round
+int kat_request_timing(int nq, int na, double x_extent, double k, const double *p, int n_p, const double *Z,
+                       int n_tab, double start, double stop, double bias, int reps, double *t_get, double *t_req)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<double> coords((uint)n_tab, start, stop, bias);
+    ZInterp interp(Z, coords);
+    Integrator4D2AngGPU<double, SelfEnergyKernel> integ(provider(), {{(uint)nq, (uint)na, (uint)na}}, x_extent);
+    std::vector<double> out(n_p);
+    auto gets = [&] {
+      for (int i = 0; i < n_p; ++i) out[i] = integ.get(k, p[i], interp);
+    };
+    auto reqs = [&] {
+      std::vector<std::future<double>> f;
+      for (int i = 0; i < n_p; ++i) f.push_back(integ.request(k, p[i], interp));
+      for (int i = 0; i < n_p; ++i) out[i] = f[i].get();
+    };
+    gets();
+    reqs();
+    auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; ++r) gets();
+    auto t1 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; ++r) reqs();
+    auto t2 = std::chrono::steady_clock::now();
+    *t_get = std::chrono::duration<double>(t1 - t0).count() / reps;
+    *t_req = std::chrono::duration<double>(t2 - t1).count() / reps;
+  });
 }
 
 } // extern "C"

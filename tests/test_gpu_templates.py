@@ -209,3 +209,63 @@ def test_errors(kat):
     print("[case] unequal angular orders", flush=True)
     assert kat.kat_unequal_angles() == -2
     assert b"Grid sizes must be currently equal" in kat.kat_last_error()
+
+
+# ------------------------------------------------------------------------------ batched requests
+
+def _se_case(n_p=16, nq=32, na=16):
+    c = W.selfenergy(n_p=n_p, nq=nq, na=na)
+    return c, arr(c["p"]), arr(c["Z"])
+
+
+def test_batched_requests_match_get_bitwise(kat, qp):
+    c, p, Z = _se_case()
+    req, get = np.empty(len(p)), np.empty(len(p))
+    args = (32, 16, D(20.0), D(1.0), P(p), len(p), P(Z), len(Z), D(1e-3), D(40.0), D(6.0))
+    print("[case] self-energy: 16 batched requests vs 16 get()", flush=True)
+    call(kat, "kat_selfenergy", *args, P(req))
+    call(kat, "kat_selfenergy_get", *args, P(get))
+    assert np.array_equal(req, get)  # same block partition and reduction order
+    coord = O.logcoord(len(Z), 1e-3, 40.0, 6.0)
+    ref = O.c1_selfenergy(p, [Z] * len(p), [coord] * len(p), [(qp.get_points(32), qp.get_weights(32))] * len(p),
+                          [(qp.get_points(16), qp.get_weights(16))] * len(p))
+    assert np.max(np.abs(req - ref) / ref) <= TOL
+
+
+def test_request_data_update_data_get_data(kat, qp):
+    c, _, Z = _se_case()
+    n_p = 24
+    req, get, pg = np.empty(n_p), np.empty(n_p), np.empty(n_p)
+    print("[case] request_data / update_data / get_data over a 24-point log grid", flush=True)
+    call(kat, "kat_request_data", 32, 16, D(20.0), D(1.0), n_p, D(1e-2), D(10.0), D(3.0), P(Z), len(Z), D(1e-3),
+         D(40.0), D(6.0), P(req), P(get), P(pg))
+    assert np.allclose(pg, W.log_coord_forward(n_p, 1e-2, 10.0, 3.0), rtol=1e-15, atol=0)
+    assert np.array_equal(req, get)
+    coord = O.logcoord(len(Z), 1e-3, 40.0, 6.0)
+    ref = O.c1_selfenergy(pg, [Z] * n_p, [coord] * n_p, [(qp.get_points(32), qp.get_weights(32))] * n_p,
+                          [(qp.get_points(16), qp.get_weights(16))] * n_p)
+    assert np.max(np.abs(req - ref) / ref) <= TOL
+
+
+def test_finiteT_requests_follow_E(kat):
+    # k moves by more than the 2.5% hysteresis several times: the rule (and the batch) changes
+    ks = arr([1.0, 0.99, 0.9, 0.89, 0.7, 0.5, 0.49, 0.3, 0.1, 0.05])
+    c = arr([1.0, 0.3, -0.2, 0.1, 0.0, 0.05])
+    den = arr([0.04, 0.0, 1.1, 0.0])
+    req, get = np.empty(len(ks)), np.empty(len(ks))
+    print("[case] finite-T requests with E tracking", flush=True)
+    call(kat, "kat_finiteT_requests", 32, D(2.0), D(0.2), P(ks), len(ks), P(c), P(den), P(req), P(get))
+    print(f"  max rel diff {np.max(np.abs(req - get) / np.abs(get)):.3e}", flush=True)
+    assert np.all(np.isfinite(req))
+    assert np.array_equal(req, get)
+
+
+def test_request_batching_is_one_launch_per_batch(kat):
+    c, p, Z = _se_case(n_p=64, nq=64, na=32)
+    tg, tr = D(), D()
+    print("[case] 64 self-energy requests: get() loop vs one batch", flush=True)
+    call(kat, "kat_request_timing", 64, 32, D(20.0), D(1.0), P(p), len(p), P(Z), len(Z), D(1e-3), D(40.0), D(6.0),
+         5, C.byref(tg), C.byref(tr))
+    print(f"  64 x get(): {tg.value * 1e3:.3f} ms, 64 requests in one batch: {tr.value * 1e3:.3f} ms "
+          f"({tg.value / tr.value:.1f}x)", flush=True)
+    assert tr.value < tg.value
