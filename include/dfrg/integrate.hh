@@ -218,6 +218,95 @@ namespace dfrg
       return acc;
     }
 
+    // IntegratorAngleGPU<d> (integrator_angle_gpu.hh:21-40): node (q, cos), cos fastest
+    template <int d, class ctype, class NT, class KERNEL, class... T>
+    __device__ __forceinline__ NT angle1_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                             const ctype *__restrict__ xa, const ctype *__restrict__ wa, int nx,
+                                             int na, ctype x_extent, int b, int nb, ctype k, const T &...t)
+    {
+      NT acc = NT(0);
+      constexpr ctype S_d = DiFfRG::S_d_prec<ctype>(d);
+      const long long W = (long long)nx * na;
+      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
+        const int iy = (int)(i % na);
+        const int ix = (int)(i / na);
+        const ctype q = k * sqrt(xq[ix] * x_extent);
+        const ctype cos = 2 * (xa[iy] - (ctype)0.5);
+        const ctype int_element = S_d                                                // solid angle
+                                  * (DiFfRG::powr<d - 2>(q) / (ctype)2 * DiFfRG::powr<2>(k)) // x = q^2/k^2
+                                  * (1 / (ctype)2)                                   // cos integral divided out
+                                  / DiFfRG::powr<d>(2 * (ctype)M_PI);                // Fourier factor
+        const ctype weight = 2 * wa[iy] * wq[ix] * x_extent;
+        acc += int_element * weight * KERNEL::kernel(q, cos, k, t...);
+      }
+      return acc;
+    }
+
+    // Integrator4DFiniteTq0GPU (integrator_4D_finiteTq0_gpu.hh:20-55): node (q, cos, phi, Matsubara n)
+    template <class ctype, class NT, class KERNEL, class... T>
+    __device__ __forceinline__ NT angle2_finiteT_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                     const ctype *__restrict__ xa, const ctype *__restrict__ wa,
+                                                     const ctype *__restrict__ mx, const ctype *__restrict__ mw,
+                                                     int nx, int na, int nm, ctype x_extent, ctype m_T, int b, int nb,
+                                                     ctype k, const T &...t)
+    {
+      NT acc = NT(0);
+      const long long W = (long long)nx * na * na * nm;
+      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
+        const int in = (int)(i % nm);
+        long long r = i / nm;
+        const int iz = (int)(r % na); // phi
+        r /= na;
+        const int iy = (int)(r % na); // cos
+        const int ix = (int)(r / na);
+        const ctype q = k * sqrt(xq[ix] * x_extent);
+        const ctype int_element = (DiFfRG::powr<1>(q) / (ctype)2 * DiFfRG::powr<2>(k)) // x = q^2/k^2, 3 dims
+                                  / DiFfRG::powr<3>(2 * (ctype)M_PI);                   // Fourier factor
+        const ctype cos = 2 * (xa[iy] - (ctype)0.5);
+        const ctype phi = 2 * (ctype)M_PI * xa[iz];
+        const ctype x_weight = 2 * (ctype)M_PI * wa[iz] * 2 * wa[iy] * wq[ix] * x_extent;
+        const ctype q0 = mx[in];
+        const ctype weight = x_weight * mw[in];
+        NT res = int_element * weight *
+                 (KERNEL::kernel(q, cos, phi, q0, k, t...) + KERNEL::kernel(q, cos, phi, -q0, k, t...));
+        if (m_T > 0 && in == 0) res += int_element * x_weight * m_T * KERNEL::kernel(q, cos, phi, (ctype)0, k, t...);
+        acc += res;
+      }
+      return acc;
+    }
+
+    // IntegratorAngleFiniteTq0GPU<d> (integrator_angle_finiteTq0_gpu.hh:20-50): node (q, cos, Matsubara n)
+    template <int d, class ctype, class NT, class KERNEL, class... T>
+    __device__ __forceinline__ NT angle1_finiteT_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                     const ctype *__restrict__ xa, const ctype *__restrict__ wa,
+                                                     const ctype *__restrict__ mx, const ctype *__restrict__ mw,
+                                                     int nx, int na, int nm, ctype x_extent, ctype m_T, int b, int nb,
+                                                     ctype k, const T &...t)
+    {
+      NT acc = NT(0);
+      constexpr ctype S_dm1 = DiFfRG::S_d_prec<ctype>(d - 1);
+      const long long W = (long long)nx * na * nm;
+      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
+        const int in = (int)(i % nm);
+        const long long r = i / nm;
+        const int iy = (int)(r % na);
+        const int ix = (int)(r / na);
+        const ctype q = k * sqrt(xq[ix] * x_extent);
+        const ctype int_element = S_dm1                                                // solid angle
+                                  * (DiFfRG::powr<d - 3>(q) / (ctype)2 * DiFfRG::powr<2>(k)) // x = q^2/k^2
+                                  * (1 / (ctype)2)                                     // cos integral divided out
+                                  / DiFfRG::powr<d - 1>(2 * (ctype)M_PI);              // Fourier factor
+        const ctype cos = 2 * (xa[iy] - (ctype)0.5);
+        const ctype x_weight = 2 * wa[iy] * wq[ix] * x_extent;
+        const ctype q0 = mx[in];
+        const ctype weight = x_weight * mw[in];
+        NT res = int_element * weight * (KERNEL::kernel(q, cos, q0, k, t...) + KERNEL::kernel(q, cos, -q0, k, t...));
+        if (m_T > 0 && in == 0) res += int_element * x_weight * m_T * KERNEL::kernel(q, cos, (ctype)0, k, t...);
+        acc += res;
+      }
+      return acc;
+    }
+
     // ------------------------------------------------------------------------------------------
     // kernels: one call (arguments by value), or a batch of requests whose (k, t...) tuples sit in
     // device memory, nb blocks per request
@@ -322,6 +411,85 @@ namespace dfrg
       const NT acc = std::apply(
           [&](const ctype &k, const auto &...t) {
             return finiteTq0_sum<d, ctype, NT, KERNEL>(xq, wq, mx, mw, nx, nm, x_extent, m_T, b, nb, k, t...);
+          },
+          args[r]);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+    }
+
+    template <int d, class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void angle1_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
+                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
+                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
+                                                            unsigned *counter, NT *result, T... t)
+    {
+      const NT acc =
+          angle1_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+    }
+    template <int d, class ctype, class NT, class KERNEL, class Args>
+    __global__ __launch_bounds__(kBlock) void angle1_batch_kernel(const ctype *__restrict__ xq,
+                                                                  const ctype *__restrict__ wq,
+                                                                  const ctype *__restrict__ xa,
+                                                                  const ctype *__restrict__ wa, int nx, int na,
+                                                                  ctype x_extent, const Args *args, int nb,
+                                                                  NT *partials, unsigned *counters, NT *results)
+    {
+      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
+      const NT acc = std::apply(
+          [&](const ctype &k, const auto &...t) {
+            return angle1_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
+          },
+          args[r]);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+    }
+
+    template <class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void angle2_finiteT_kernel(
+        const ctype *__restrict__ xq, const ctype *__restrict__ wq, const ctype *__restrict__ xa,
+        const ctype *__restrict__ wa, const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx, int na,
+        int nm, ctype x_extent, ctype m_T, ctype k, NT *partials, unsigned *counter, NT *result, T... t)
+    {
+      const NT acc = angle2_finiteT_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T,
+                                                           blockIdx.x, gridDim.x, k, t...);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+    }
+    template <class ctype, class NT, class KERNEL, class Args>
+    __global__ __launch_bounds__(kBlock) void angle2_finiteT_batch_kernel(
+        const ctype *__restrict__ xq, const ctype *__restrict__ wq, const ctype *__restrict__ xa,
+        const ctype *__restrict__ wa, const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx, int na,
+        int nm, ctype x_extent, ctype m_T, const Args *args, int nb, NT *partials, unsigned *counters, NT *results)
+    {
+      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
+      const NT acc = std::apply(
+          [&](const ctype &k, const auto &...t) {
+            return angle2_finiteT_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T, b, nb, k,
+                                                         t...);
+          },
+          args[r]);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+    }
+
+    template <int d, class ctype, class NT, class KERNEL, class... T>
+    __global__ __launch_bounds__(kBlock) void angle1_finiteT_kernel(
+        const ctype *__restrict__ xq, const ctype *__restrict__ wq, const ctype *__restrict__ xa,
+        const ctype *__restrict__ wa, const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx, int na,
+        int nm, ctype x_extent, ctype m_T, ctype k, NT *partials, unsigned *counter, NT *result, T... t)
+    {
+      const NT acc = angle1_finiteT_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T,
+                                                              blockIdx.x, gridDim.x, k, t...);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+    }
+    template <int d, class ctype, class NT, class KERNEL, class Args>
+    __global__ __launch_bounds__(kBlock) void angle1_finiteT_batch_kernel(
+        const ctype *__restrict__ xq, const ctype *__restrict__ wq, const ctype *__restrict__ xa,
+        const ctype *__restrict__ wa, const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx, int na,
+        int nm, ctype x_extent, ctype m_T, const Args *args, int nb, NT *partials, unsigned *counters, NT *results)
+    {
+      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
+      const NT acc = std::apply(
+          [&](const ctype &k, const auto &...t) {
+            return angle1_finiteT_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T, b, nb,
+                                                            k, t...);
           },
           args[r]);
       finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);

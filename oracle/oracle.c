@@ -563,6 +563,87 @@ double orc_kat_finiteT(int d, const double *x, const double *w, int nx, double x
   return constant + value;
 }
 
+double orc_kat_angle(int d, const double *xq, const double *wq, int nq, const double *xa, const double *wa, int na,
+                     double x_extent, double k, const double *c6, const double *a4, double constant)
+{
+  /* integrator_angle_cpu.hh:66-79: S_d q^{d-2}/2 k^2 (1/2) / (2pi)^d, weight 2 w_cos w_x x_extent */
+  const double Sd = S_d(d);
+  double value = 0.0;
+  for (int ix = 0; ix < nq; ++ix) {
+    const double q = k * sqrt(xq[ix] * x_extent);
+    double part = 0.0;
+    for (int iy = 0; iy < na; ++iy) {
+      const double cos1 = 2.0 * (xa[iy] - 0.5);
+      const double int_element = Sd * (ipow(q, d - 2) / 2.0 * (k * k)) * 0.5 / ipow(2.0 * M_PI, d);
+      const double weight = 2.0 * wa[iy] * wq[ix] * x_extent;
+      part += int_element * weight * poly6(c6, q) * poly4(a4, cos1);
+    }
+    value += part;
+  }
+  return constant + value;
+}
+
+static inline double poly5(const double *c, double v)
+{
+  return c[0] + c[1] * v + c[2] * v * v + c[3] * v * v * v + c[4] * v * v * v * v;
+}
+
+double orc_kat_4d_finiteT(const double *xq, const double *wq, int nq, const double *xa, const double *wa, int na,
+                          double x_extent, double k, const double *mx, const double *mw, int nm, double T,
+                          const double *c6, const double *a4, const double *p4, const double *d5, double constant)
+{
+  /* integrator_4D_finiteTq0_cpu.hh:95-117: q/2 k^2 / (2pi)^3, weight 2pi w_phi 2 w_cos w_x x_extent,
+   * sum_n w_n (f(q0_n) + f(-q0_n)) + T f(0) */
+  double value = 0.0;
+  for (int ix = 0; ix < nq; ++ix) {
+    const double q = k * sqrt(xq[ix] * x_extent);
+    const double int_element = (q / 2.0 * (k * k)) / ipow(2.0 * M_PI, 3);
+    double part = 0.0;
+    for (int iy = 0; iy < na; ++iy) {
+      const double cos1 = 2.0 * (xa[iy] - 0.5);
+      for (int iz = 0; iz < na; ++iz) {
+        const double phi = 2.0 * M_PI * xa[iz];
+        const double x_weight = 2.0 * M_PI * wa[iz] * 2.0 * wa[iy] * wq[ix] * x_extent;
+        const double sp = poly6(c6, q) * poly4(a4, cos1) * poly4(p4, phi);
+        for (int in = 0; in < nm; ++in) {
+          const double q0 = mx[in];
+          part += int_element * x_weight * mw[in] * (sp / poly5(d5, q0) + sp / poly5(d5, -q0));
+          if (T > 0 && in == 0) part += int_element * x_weight * T * sp / poly5(d5, 0.0);
+        }
+      }
+    }
+    value += part;
+  }
+  return constant + value;
+}
+
+double orc_kat_angle_finiteT(int d, const double *xq, const double *wq, int nq, const double *xa, const double *wa,
+                             int na, double x_extent, double k, const double *mx, const double *mw, int nm, double T,
+                             const double *c6, const double *a4, const double *d4, double constant)
+{
+  /* integrator_angle_finiteTq0_cpu.hh:95-118: S_{d-1} q^{d-3}/2 k^2 (1/2) / (2pi)^{d-1},
+   * weight 2 w_cos w_x x_extent, Matsubara sum as above */
+  const double Sdm1 = S_d(d - 1);
+  double value = 0.0;
+  for (int ix = 0; ix < nq; ++ix) {
+    const double q = k * sqrt(xq[ix] * x_extent);
+    const double int_element = Sdm1 * (ipow(q, d - 3) / 2.0 * (k * k)) * 0.5 / ipow(2.0 * M_PI, d - 1);
+    double part = 0.0;
+    for (int iy = 0; iy < na; ++iy) {
+      const double cos1 = 2.0 * (xa[iy] - 0.5);
+      const double x_weight = 2.0 * wa[iy] * wq[ix] * x_extent;
+      const double sp = poly6(c6, q) * poly4(a4, cos1);
+      for (int in = 0; in < nm; ++in) {
+        const double q0 = mx[in];
+        part += int_element * x_weight * mw[in] * (sp / poly4(d4, q0) + sp / poly4(d4, -q0));
+        if (T > 0 && in == 0) part += int_element * x_weight * T * sp / poly4(d4, 0.0);
+      }
+    }
+    value += part;
+  }
+  return constant + value;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* LPA flow                                                                                     */
 /* ------------------------------------------------------------------------------------------ */

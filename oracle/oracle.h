@@ -110,6 +110,19 @@ double orc_kat_4d(const double *xq, const double *wq, int nq, const double *xa, 
 double orc_kat_finiteT(int d, const double *x, const double *w, int nx, double x_extent, double k,
                        const double *mx, const double *mw, int nm, double T, const double *c6,
                        const double *d4, double constant);
+/* angle: poly(q) poly1(cos) in d dims (integrator_angle_cpu.cc, integrator_angle_cpu.hh:60-80) */
+double orc_kat_angle(int d, const double *xq, const double *wq, int nq, const double *xa, const double *wa, int na,
+                     double x_extent, double k, const double *c6, const double *a4, double constant);
+/* 4D finite-T q0: poly(q) poly1(cos) poly2(phi) / poly5(q0) (integrator_4D_finiteTq0_cpu.cc:14-40,
+ * integrator_4D_finiteTq0_cpu.hh:90-120); equal cos and phi orders */
+double orc_kat_4d_finiteT(const double *xq, const double *wq, int nq, const double *xa, const double *wa, int na,
+                          double x_extent, double k, const double *mx, const double *mw, int nm, double T,
+                          const double *c6, const double *a4, const double *p4, const double *d5, double constant);
+/* angle finite-T q0 in d-1 spatial dims: poly(q) poly1(cos) / poly4(q0)
+ * (integrator_angle_finiteTq0_gpu.cu:14-34, integrator_angle_finiteTq0_cpu.hh:95-120) */
+double orc_kat_angle_finiteT(int d, const double *xq, const double *wq, int nq, const double *xa, const double *wa,
+                             int na, double x_extent, double k, const double *mx, const double *mw, int nm, double T,
+                             const double *c6, const double *a4, const double *d4, double constant);
 
 /* ---------------- LPA flow (uniform rho grid, explicit RK4) ---------------- */
 /* Finite-T d=3 threshold integral with the closed-form bosonic Matsubara sum,

@@ -89,6 +89,15 @@ def lib():
         L.orc_kat_finiteT.restype = C.c_double
         L.orc_kat_finiteT.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_int,
                                       C.c_double, _dp, _dp, C.c_double]
+        L.orc_kat_angle.restype = C.c_double
+        L.orc_kat_angle.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp,
+                                    C.c_double]
+        L.orc_kat_4d_finiteT.restype = C.c_double
+        L.orc_kat_4d_finiteT.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp,
+                                         C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_double]
+        L.orc_kat_angle_finiteT.restype = C.c_double
+        L.orc_kat_angle_finiteT.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                            _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double]
         L.orc_lpa_threshold.restype = C.c_double
         L.orc_lpa_threshold.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int, C.c_double,
                                         C.c_double]
@@ -270,6 +279,24 @@ def kat_finiteT(d, x, w, x_extent, k, mx, mw, T, c6, d4, constant):
     x, w, mx, mw, c6, d4 = (_f64(v) for v in (x, w, mx, mw, c6, d4))
     return lib().orc_kat_finiteT(d, _p(x), _p(w), len(x), x_extent, k, _p(mx), _p(mw), len(mx), T, _p(c6), _p(d4),
                                  constant)
+
+
+def kat_angle(d, xq, wq, xa, wa, x_extent, k, c6, a4, constant):
+    xq, wq, xa, wa, c6, a4 = (_f64(v) for v in (xq, wq, xa, wa, c6, a4))
+    return lib().orc_kat_angle(d, _p(xq), _p(wq), len(xq), _p(xa), _p(wa), len(xa), x_extent, k, _p(c6), _p(a4),
+                               constant)
+
+
+def kat_4d_finiteT(xq, wq, xa, wa, x_extent, k, mx, mw, T, c6, a4, p4, d5, constant):
+    xq, wq, xa, wa, mx, mw, c6, a4, p4, d5 = (_f64(v) for v in (xq, wq, xa, wa, mx, mw, c6, a4, p4, d5))
+    return lib().orc_kat_4d_finiteT(_p(xq), _p(wq), len(xq), _p(xa), _p(wa), len(xa), x_extent, k, _p(mx), _p(mw),
+                                    len(mx), T, _p(c6), _p(a4), _p(p4), _p(d5), constant)
+
+
+def kat_angle_finiteT(d, xq, wq, xa, wa, x_extent, k, mx, mw, T, c6, a4, d4, constant):
+    xq, wq, xa, wa, mx, mw, c6, a4, d4 = (_f64(v) for v in (xq, wq, xa, wa, mx, mw, c6, a4, d4))
+    return lib().orc_kat_angle_finiteT(d, _p(xq), _p(wq), len(xq), _p(xa), _p(wa), len(xa), x_extent, k, _p(mx),
+                                       _p(mw), len(mx), T, _p(c6), _p(a4), _p(d4), constant)
 
 
 def lpa_threshold(m2, T, k, x, w, x_extent=20.0, b=1.0):

@@ -11,8 +11,14 @@
 #include <DiFfRG/physics/integration/integrator_4D_2ang_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_angle_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_angle_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_cpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_gpu.hh>
 #include <DiFfRG/physics/interpolation/linear_interpolation_1D.hh>
@@ -135,6 +141,67 @@ namespace
              (d0 + d1 * powr<1>(q0) + d2 * powr<2>(q0) + d3 * powr<3>(q0));
     }
     static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+
+  // integrator_angle_gpu.cu:14-32
+  struct PolyAngle {
+    static __forceinline__ __host__ __device__ auto kernel(const double q, const double cos, const double,
+                                                           const double, const double x0, const double x1,
+                                                           const double x2, const double x3, const double x4,
+                                                           const double x5, const double a0, const double a1,
+                                                           const double a2, const double a3)
+    {
+      return (x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q)) *
+             (a0 + a1 * powr<1>(cos) + a2 * powr<2>(cos) + a3 * powr<3>(cos));
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+  // integrator_4D_finiteTq0_gpu.cu:16-39
+  struct Poly4DFiniteT {
+    static __forceinline__ __host__ __device__ auto
+    kernel(const double q, const double cos, const double phi, const double q0, const double, const double,
+           const double x0, const double x1, const double x2, const double x3, const double x4, const double x5,
+           const double a0, const double a1, const double a2, const double a3, const double p0, const double p1,
+           const double p2, const double p3, const double d0, const double d1, const double d2, const double d3,
+           const double d4)
+    {
+      return (x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q)) *
+             (a0 + a1 * powr<1>(cos) + a2 * powr<2>(cos) + a3 * powr<3>(cos)) *
+             (p0 + p1 * powr<1>(phi) + p2 * powr<2>(phi) + p3 * powr<3>(phi)) /
+             (d0 + d1 * powr<1>(q0) + d2 * powr<2>(q0) + d3 * powr<3>(q0) + d4 * powr<4>(q0));
+    }
+    static __forceinline__ __host__ __device__ auto
+    constant(const double, const double c, const double, const double, const double, const double, const double,
+             const double, const double, const double, const double, const double, const double, const double,
+             const double, const double, const double, const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+  // integrator_angle_finiteTq0_gpu.cu:14-34
+  struct PolyAngleFiniteT {
+    static __forceinline__ __host__ __device__ auto
+    kernel(const double q, const double cos, const double q0, const double, const double, const double x0,
+           const double x1, const double x2, const double x3, const double x4, const double x5, const double a0,
+           const double a1, const double a2, const double a3, const double d0, const double d1, const double d2,
+           const double d3)
+    {
+      return (x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q)) *
+             (a0 + a1 * powr<1>(cos) + a2 * powr<2>(cos) + a3 * powr<3>(cos)) /
+             (d0 + d1 * powr<1>(q0) + d2 * powr<2>(q0) + d3 * powr<3>(q0));
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
                                                              const double, const double, const double, const double,
                                                              const double, const double, const double, const double)
     {
@@ -448,6 +515,110 @@ int kat_request_timing(int nq, int na, double x_extent, double k, const double *
     auto t2 = std::chrono::steady_clock::now();
     *t_get = std::chrono::duration<double>(t1 - t0).count() / reps;
     *t_req = std::chrono::duration<double>(t2 - t1).count() / reps;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+// angle and finite-T variants: get() on the GPU and host templates, and requests against get()
+// ---------------------------------------------------------------------------------------------
+
+int kat_angle(int device, int d, int nx, int na, double x_extent, double k, const double *c, const double *a,
+              double constant, double *out)
+{
+  return guard([&] {
+    auto run = [&](auto integ) {
+      *out = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3]);
+    };
+    const std::array<uint, 2> g{{(uint)nx, (uint)na}};
+    switch (d * 2 + device) {
+    case 4: run(IntegratorAngleTBB<2, double, PolyAngle>(provider(), g, x_extent)); break;
+    case 5: run(IntegratorAngleGPU<2, double, PolyAngle>(provider(), g, x_extent)); break;
+    case 6: run(IntegratorAngleTBB<3, double, PolyAngle>(provider(), g, x_extent)); break;
+    case 7: run(IntegratorAngleGPU<3, double, PolyAngle>(provider(), g, x_extent)); break;
+    case 8: run(IntegratorAngleTBB<4, double, PolyAngle>(provider(), g, x_extent)); break;
+    case 9: run(IntegratorAngleGPU<4, double, PolyAngle>(provider(), g, x_extent)); break;
+    default: throw std::invalid_argument("unsupported dimension");
+    }
+  });
+}
+
+// *nm receives the Matsubara rule size (GPU template only)
+int kat_4d_finiteT(int device, int nx, int na, double x_extent, double T, double k, const double *c, const double *a,
+                   const double *p, const double *den, double constant, double *out, int *nm)
+{
+  return guard([&] {
+    auto run = [&](auto &integ) {
+      *out = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], p[0], p[1], p[2],
+                       p[3], den[0], den[1], den[2], den[3], den[4]);
+    };
+    if (device) {
+      Integrator4DFiniteTq0GPU<double, Poly4DFiniteT> i(provider(), {{(uint)nx, (uint)na, (uint)na}}, x_extent, T);
+      run(i);
+      *nm = (int)i.matsubara_size();
+    } else {
+      Integrator4DFiniteTq0TBB<double, Poly4DFiniteT> i(provider(), {{(uint)nx, (uint)na, (uint)na, 0u}}, x_extent, T);
+      run(i);
+    }
+  });
+}
+
+int kat_angle_finiteT(int device, int d, int nx, int na, double x_extent, double T, double k, const double *c,
+                      const double *a, const double *den, double constant, double *out, int *nm)
+{
+  return guard([&] {
+    auto run = [&](auto &integ) {
+      *out = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], den[0], den[1], den[2],
+                       den[3]);
+      if constexpr (requires { integ.matsubara_size(); }) *nm = (int)integ.matsubara_size();
+    };
+    const std::array<uint, 2> g{{(uint)nx, (uint)na}};
+    switch (d * 2 + device) {
+    case 6: { IntegratorAngleFiniteTq0TBB<3, double, PolyAngleFiniteT> i(provider(), g, x_extent, T); run(i); break; }
+    case 7: { IntegratorAngleFiniteTq0GPU<3, double, PolyAngleFiniteT> i(provider(), g, x_extent, T); run(i); break; }
+    case 8: { IntegratorAngleFiniteTq0TBB<4, double, PolyAngleFiniteT> i(provider(), g, x_extent, T); run(i); break; }
+    case 9: { IntegratorAngleFiniteTq0GPU<4, double, PolyAngleFiniteT> i(provider(), g, x_extent, T); run(i); break; }
+    default: throw std::invalid_argument("unsupported dimension");
+    }
+  });
+}
+
+// n requests over ks (one batch per Matsubara rule for the finite-T forms) against get() on a
+// second integrator at the same k; which = 0: angle d=4, 1: 4D finite-T, 2: angle finite-T d=4
+int kat_variant_requests(int which, int nx, int na, double x_extent, double T, const double *ks, int n,
+                         double *out_req, double *out_get)
+{
+  return guard([&] {
+    auto both = [&](auto &ra, auto &gb, auto &&call_req, auto &&call_get) {
+      std::vector<std::future<double>> f;
+      for (int i = 0; i < n; ++i) f.push_back(call_req(ra, ks[i]));
+      for (int i = 0; i < n; ++i) out_get[i] = call_get(gb, ks[i]);
+      for (int i = 0; i < n; ++i) out_req[i] = f[i].get();
+    };
+    const double c0 = 1.0, c1 = 0.3, c2 = -0.2, a0 = 1.0, a1 = 0.25, a2 = 0.1;
+    if (which == 0) {
+      IntegratorAngleGPU<4, double, PolyAngle> A(provider(), {{(uint)nx, (uint)na}}, x_extent);
+      IntegratorAngleGPU<4, double, PolyAngle> B(provider(), {{(uint)nx, (uint)na}}, x_extent);
+      both(A, B, [&](auto &I, double k) { return I.request(k, 0.0, c0, c1, c2, 0., 0., 0., a0, a1, a2, 0.); },
+           [&](auto &I, double k) { return I.get(k, 0.0, c0, c1, c2, 0., 0., 0., a0, a1, a2, 0.); });
+    } else if (which == 1) {
+      Integrator4DFiniteTq0GPU<double, Poly4DFiniteT> A(provider(), {{(uint)nx, (uint)na, (uint)na}}, x_extent, T);
+      Integrator4DFiniteTq0GPU<double, Poly4DFiniteT> B(provider(), {{(uint)nx, (uint)na, (uint)na}}, x_extent, T);
+      both(A, B,
+           [&](auto &I, double k) {
+             return I.request(k, 0.0, c0, c1, c2, 0., 0., 0., a0, a1, a2, 0., 1., 0.05, 0., 0., T * T, 0., 1.1, 0.,
+                              0.);
+           },
+           [&](auto &I, double k) {
+             return I.get(k, 0.0, c0, c1, c2, 0., 0., 0., a0, a1, a2, 0., 1., 0.05, 0., 0., T * T, 0., 1.1, 0., 0.);
+           });
+    } else if (which == 2) {
+      IntegratorAngleFiniteTq0GPU<4, double, PolyAngleFiniteT> A(provider(), {{(uint)nx, (uint)na}}, x_extent, T);
+      IntegratorAngleFiniteTq0GPU<4, double, PolyAngleFiniteT> B(provider(), {{(uint)nx, (uint)na}}, x_extent, T);
+      both(A, B,
+           [&](auto &I, double k) { return I.request(k, 0.0, c0, c1, c2, 0., 0., 0., a0, a1, a2, 0., T * T, 0., 1.1, 0.); },
+           [&](auto &I, double k) { return I.get(k, 0.0, c0, c1, c2, 0., 0., 0., a0, a1, a2, 0., T * T, 0., 1.1, 0.); });
+    } else
+      throw std::invalid_argument("unknown variant");
   });
 }
 

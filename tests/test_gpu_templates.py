@@ -205,6 +205,86 @@ def test_builtin_integrands_as_user_kernels(kat, qp):
     assert np.max(np.abs(out - ref) / ref) <= 1e-11
 
 
+@pytest.mark.parametrize("d", [2, 3, 4])
+def test_angle_template(kat, qp, d):
+    # IntegratorAngleGPU/TBB<d> (integrator_angle_gpu.cu): oracle parity and the polynomial closed form
+    rng = np.random.default_rng(70 + d)
+    xq, wq = qp.get_points(64), qp.get_weights(64)
+    xa, wa = qp.get_points(12), qp.get_weights(12)
+    for device in (1, 0):
+        x_ext, k = rng.uniform(1, 2), rng.uniform(0.1, 1)
+        c, a = arr(rng.uniform(-1, 1, 6)), arr(rng.uniform(-1, 1, 4))
+        const = rng.uniform(-1, 1)
+        print(f"[case] angle d={d} device={device}", flush=True)
+        out = D()
+        call(kat, "kat_angle", device, d, 64, 12, D(x_ext), D(k), P(c), P(a), D(const), C.byref(out))
+        ref = O.kat_angle(d, xq, wq, xa, wa, x_ext, k, c, a, const)
+        assert out.value == pytest.approx(ref, rel=TOL, abs=TOL * abs(const))
+        q_ext = math.sqrt(x_ext) * k
+        closed = const + S_d(d) / 2 * poly_int(c, 0, q_ext, d - 1) / (2 * math.pi) ** d * poly_int(a, -1, 1)
+        assert is_close(out.value, closed, 1e-6)
+
+
+def _sum_closed(T, val):
+    return 1 / T / math.tanh(0.5 / val) / 2 / val
+
+
+def _tracked_E(T, k):
+    return k if abs(k - 10 * T) / max(k, 10 * T) > 2.5e-2 else 10 * T
+
+
+def test_4d_finiteT_template(kat, qp):
+    # Integrator4DFiniteTq0GPU/TBB (integrator_4D_finiteTq0_gpu.cu:41-72): oracle parity, and the
+    # volume x Matsubara-sum closed form extended to polynomials in q, cos, phi (tolerance 5e-5)
+    rng = np.random.default_rng(81)
+    xq, wq = qp.get_points(64), qp.get_weights(64)
+    xa, wa = qp.get_points(16), qp.get_weights(16)
+    for device in (1, 0):
+        x_ext, T, k, val = rng.uniform(1, 2), rng.uniform(0.01, 1), rng.uniform(0.05, 1), rng.uniform(0.8, 1.2)
+        c, a, p = arr(rng.uniform(-1, 1, 6)), arr(rng.uniform(-1, 1, 4)), arr(rng.uniform(-1, 1, 4))
+        den = arr([T * T, 0, val * val, 0, 0])
+        const = rng.uniform(-1, 1)
+        print(f"[case] 4D finite-T device={device} T={T:.3f} k={k:.3f}", flush=True)
+        out, nm = D(), C.c_int()
+        call(kat, "kat_4d_finiteT", device, 64, 16, D(x_ext), D(T), D(k), P(c), P(a), P(p), P(den), D(const),
+             C.byref(out), C.byref(nm))
+        mx, mw, _ = O.matsubara_storage_rule(T, _tracked_E(T, k))
+        if device:
+            assert nm.value == len(mx)
+        ref = O.kat_4d_finiteT(xq, wq, xa, wa, x_ext, k, mx, mw, T, c, a, p, den, const)
+        assert out.value == pytest.approx(ref, rel=1e-11, abs=1e-11 * abs(const))
+        q_ext = math.sqrt(x_ext) * k
+        closed = const + poly_int(c, 0, q_ext, 2) * poly_int(a, -1, 1) * poly_int(p, 0, 2 * math.pi) \
+            / (2 * math.pi) ** 3 * _sum_closed(T, val)
+        assert is_close(out.value, closed, 5e-5)
+
+
+@pytest.mark.parametrize("d", [3, 4])
+def test_angle_finiteT_template(kat, qp, d):
+    # IntegratorAngleFiniteTq0GPU/TBB<d> (integrator_angle_finiteTq0_gpu.cu:36-56)
+    rng = np.random.default_rng(90 + d)
+    xq, wq = qp.get_points(64), qp.get_weights(64)
+    xa, wa = qp.get_points(16), qp.get_weights(16)
+    for device in (1, 0):
+        x_ext, T, k, val = rng.uniform(1, 2), rng.uniform(0.01, 1), rng.uniform(0.05, 1), rng.uniform(0.8, 1.2)
+        c, a = arr(rng.uniform(-1, 1, 6)), arr(rng.uniform(-1, 1, 4))
+        den = arr([T * T, 0, val * val, 0])
+        const = rng.uniform(-1, 1)
+        print(f"[case] angle finite-T d={d} device={device} T={T:.3f} k={k:.3f}", flush=True)
+        out, nm = D(), C.c_int()
+        call(kat, "kat_angle_finiteT", device, d, 64, 16, D(x_ext), D(T), D(k), P(c), P(a), P(den), D(const),
+             C.byref(out), C.byref(nm))
+        mx, mw, _ = O.matsubara_storage_rule(T, _tracked_E(T, k))
+        if device:
+            assert nm.value == len(mx)
+        ref = O.kat_angle_finiteT(d, xq, wq, xa, wa, x_ext, k, mx, mw, T, c, a, den, const)
+        assert out.value == pytest.approx(ref, rel=1e-11, abs=1e-11 * abs(const))
+        q_ext = math.sqrt(x_ext) * k
+        closed = const + S_d(d - 1) / 2 * poly_int(c, 0, q_ext, d - 2) * poly_int(a, -1, 1) \
+            / (2 * math.pi) ** (d - 1) * _sum_closed(T, val)
+        assert is_close(out.value, closed, 5e-5)
+
+
 def test_errors(kat):
     print("[case] unequal angular orders", flush=True)
     assert kat.kat_unequal_angles() == -2
@@ -257,6 +337,17 @@ def test_finiteT_requests_follow_E(kat):
     call(kat, "kat_finiteT_requests", 32, D(2.0), D(0.2), P(ks), len(ks), P(c), P(den), P(req), P(get))
     print(f"  max rel diff {np.max(np.abs(req - get) / np.abs(get)):.3e}", flush=True)
     assert np.all(np.isfinite(req))
+    assert np.array_equal(req, get)
+
+
+@pytest.mark.parametrize("which", [0, 1, 2], ids=["angle", "4D_finiteT", "angle_finiteT"])
+def test_variant_requests_match_get_bitwise(kat, which):
+    # k crosses the 2.5% E hysteresis several times: the finite-T forms change rule and batch
+    ks = arr([1.0, 0.99, 0.9, 0.89, 0.7, 0.5, 0.49, 0.3, 0.1, 0.05])
+    req, get = np.empty(len(ks)), np.empty(len(ks))
+    print(f"[case] requests vs get, variant {which}", flush=True)
+    call(kat, "kat_variant_requests", which, 32, 8, D(1.5), D(0.2), P(ks), len(ks), P(req), P(get))
+    assert np.all(np.isfinite(req)) and np.all(req != 0)
     assert np.array_equal(req, get)
 
 

@@ -111,6 +111,75 @@ def test_finiteT_known_answers(d):
         assert is_close(got, ref, 1e-2 if d == 2 else 1e-6)
 
 
+@pytest.mark.parametrize("d", [2, 3, 4])
+def test_angle_known_answers(d):
+    # integrator_angle_cpu.cc: volume V_d/(2pi)^d and S_d/2 int poly q^{d-1} dq int cos_poly / (2pi)^d
+    rng = np.random.default_rng(30 + d)
+    xq, wq = O.gl_nodes(64)
+    xa, wa = O.gl_nodes(12)
+    for _ in range(4):
+        x_ext, k = rng.uniform(1, 2), rng.uniform(0.1, 1)
+        q_ext = math.sqrt(x_ext * k * k)
+        vol = O.kat_angle(d, xq, wq, xa, wa, x_ext, k, [1, 0, 0, 0, 0, 0], [1, 0, 0, 0], 0.0)
+        assert is_close(vol, V_d(d, q_ext) / (2 * math.pi) ** d, 1e-6)
+        c, a = rng.uniform(-1, 1, 6), rng.uniform(-1, 1, 4)
+        const = rng.uniform(-1, 1)
+        ref = const + S_d(d) / 2 * poly_int(c, 0, q_ext, d - 1) / (2 * math.pi) ** d * poly_int(a, -1, 1)
+        assert is_close(O.kat_angle(d, xq, wq, xa, wa, x_ext, k, c, a, const), ref, 1e-6)
+        if d in (2, 4):  # polynomial in x and cos: exact to rounding
+            ce = np.array([c[0], 0, c[2], 0, c[4], 0])
+            ref = S_d(d) / 2 * poly_int(ce, 0, q_ext, d - 1) / (2 * math.pi) ** d * poly_int(a, -1, 1)
+            assert O.kat_angle(d, xq, wq, xa, wa, x_ext, k, ce, a, 0.0) == pytest.approx(ref, rel=1e-12)
+
+
+def _sum_closed(T, val):
+    # T sum_n 1/(T^2 + val^2 (2 pi n T)^2) = coth(1/(2 val)) / (2 val T)
+    return 1 / T / math.tanh(0.5 / val) / 2 / val
+
+
+def test_4d_finiteT_known_answers():
+    # integrator_4D_finiteTq0_cpu.cc:42-71 (volume x Matsubara sum, tolerance 5e-5), extended to
+    # polynomials in q, cos and phi, whose spatial integral factorises from the sum
+    rng = np.random.default_rng(44)
+    xq, wq = O.gl_nodes(64)
+    xa, wa = O.gl_nodes(16)
+    for _ in range(4):
+        x_ext, T, k, val = rng.uniform(1, 2), rng.uniform(0.01, 1), rng.uniform(0.05, 1), rng.uniform(0.8, 1.2)
+        mx, mw, _ = O.matsubara_storage_rule(T, k)
+        q_ext = math.sqrt(x_ext * k * k)
+        den = [T * T, 0, val * val, 0, 0]
+        one = [1, 0, 0, 0]
+        vol = O.kat_4d_finiteT(xq, wq, xa, wa, x_ext, k, mx, mw, T, [1, 0, 0, 0, 0, 0], one, one, den, 0.0)
+        assert is_close(vol, V_d(3, q_ext) / (2 * math.pi) ** 3 * _sum_closed(T, val), 5e-5)
+        c, a, p = rng.uniform(-1, 1, 6), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4)
+        const = rng.uniform(-1, 1)
+        ref = const + poly_int(c, 0, q_ext, 2) * poly_int(a, -1, 1) * poly_int(p, 0, 2 * math.pi) \
+            / (2 * math.pi) ** 3 * _sum_closed(T, val)
+        got = O.kat_4d_finiteT(xq, wq, xa, wa, x_ext, k, mx, mw, T, c, a, p, den, const)
+        assert is_close(got, ref, 5e-5)
+
+
+@pytest.mark.parametrize("d", [3, 4])
+def test_angle_finiteT_known_answers(d):
+    # integrator_angle_finiteTq0_gpu.cu:36-56 (volume x sum), extended to poly(q) poly(cos)
+    rng = np.random.default_rng(50 + d)
+    xq, wq = O.gl_nodes(64)
+    xa, wa = O.gl_nodes(16)
+    for _ in range(4):
+        x_ext, T, k, val = rng.uniform(1, 2), rng.uniform(0.01, 1), rng.uniform(0.05, 1), rng.uniform(0.8, 1.2)
+        mx, mw, _ = O.matsubara_storage_rule(T, k)
+        q_ext = math.sqrt(x_ext * k * k)
+        den = [T * T, 0, val * val, 0]
+        vol = O.kat_angle_finiteT(d, xq, wq, xa, wa, x_ext, k, mx, mw, T, [1, 0, 0, 0, 0, 0], [1, 0, 0, 0], den, 0.0)
+        assert is_close(vol, V_d(d - 1, q_ext) / (2 * math.pi) ** (d - 1) * _sum_closed(T, val), 5e-5)
+        c, a = rng.uniform(-1, 1, 6), rng.uniform(-1, 1, 4)
+        const = rng.uniform(-1, 1)
+        ref = const + S_d(d - 1) / 2 * poly_int(c, 0, q_ext, d - 2) * poly_int(a, -1, 1) \
+            / (2 * math.pi) ** (d - 1) * _sum_closed(T, val)
+        got = O.kat_angle_finiteT(d, xq, wq, xa, wa, x_ext, k, mx, mw, T, c, a, den, const)
+        assert is_close(got, ref, 5e-5)
+
+
 # ----------------------------------------------------------------------------- coordinates, lerp
 
 def test_log_coordinates_and_lerp():
