@@ -36,16 +36,29 @@ PEAK_VALU_WAVE_INSTR = 256 * 2.4e9
 MIXED_TOL = 1e-8
 GRAPH_SETS = 10  # call sets per hipGraph launch
 
-# Per integrand evaluation of the innermost loop of se_kernel (FP64, MIXED): FP operations
-# (FMA = 2), VALU instructions and quarter-rate transcendentals, counted from the gfx950
-# assembly of csrc/integrals.hip by `python -m diffrg_amd.isa_count`.
-SE_FLOP = {0: 223.5, 1: 51.0}
-SE_VALU = {0: 203, 1: 50.5}
+# (FP operations with FMA = 2, VALU instructions) per integrand evaluation in the two innermost
+# loops of se_kernel<MODE, BLOCK>, counted from the gfx950 assembly of csrc/integrals.hip by
+# `python -m diffrg_amd.isa_count`: "pairs" when a lane owns two angular pairs
+# (na^2 >= 2 BLOCK), "single" when it owns one pair and two radial nodes per iteration.
+# tests/test_isa_counts.py checks these figures against the current source.
+SE_OPS = {0: {"pairs": (223.5, 203.0), "single": (223.5, 205.0)},
+          1: {"pairs": (52.0, 50.0), "single": (52.5, 52.5)}}
+SE_OPS_BLOCK = {(1, 1024, "pairs"): (52.5, 50.5)}  # block sizes whose schedule differs
 
 
-def valu_issue_frac(mode, evals, seconds):
+def se_loop(na, block):
+    return "pairs" if 2 * min(na * na, block) <= na * na else "single"
+
+
+def se_ops(mode, na, block):
+    """(FLOP, VALU) per evaluation of the loop se_kernel<mode, block> runs for na angular nodes."""
+    loop = se_loop(na, block)
+    return SE_OPS_BLOCK.get((mode, block, loop), SE_OPS[mode][loop])
+
+
+def valu_issue_frac(valu_per_eval, evals, seconds):
     """Fraction of the chip's VALU issue slots the innermost loop alone fills at this rate."""
-    return SE_VALU[mode] * evals / 64 / seconds / PEAK_VALU_WAVE_INSTR
+    return valu_per_eval * evals / 64 / seconds / PEAK_VALU_WAVE_INSTR
 
 
 def parse():
@@ -217,7 +230,7 @@ def selfenergy(D, qp, args):
         log(f"selfenergy {name}: eager {ms_eager * 1e3:.1f} us, graph {ms_graph * 1e3:.1f} us, "
             f"{N_STREAMS} streams {ms_streams * 1e3:.1f} us per call set, "
             f"err {err:.2e}, shape {shape}")
-    return {"n": n, "evals": b.evals, "res": res, "ref_fn": ref_fn}
+    return {"n": n, "evals": b.evals, "na": c["na"], "res": res, "ref_fn": ref_fn}
 
 
 def threshold(D, qp, args):
@@ -342,7 +355,9 @@ def uneven(D, qp, args):
         ms /= per_graph
         r[name] = {"integrals_per_s": n / (ms * 1e-3), "evals_per_s": b.evals / (ms * 1e-3), "ms_per_batch": ms,
                    "max_rel_err": rel_err(out.to_host(), ref), "shape": shape,
-                   "valu_issue_frac": valu_issue_frac(mode, b.evals, ms * 1e-3)}
+                   "valu_issue_frac": sum(int(q) * int(a) ** 2 * se_ops(mode, int(a), shape["block"])[1]
+                                          for q, a in zip(c["nq"], c["na"])) / 64 / (ms * 1e-3)
+                                      / PEAK_VALU_WAVE_INSTR}
         log(f"uneven {name}: {ms:.3f} ms/batch, err {r[name]['max_rel_err']:.2e}, shape {shape}")
     r["mixed_over_plain"] = r["mixed"]["integrals_per_s"] / r["fp64"]["integrals_per_s"]
     r["cpu_integrals_per_s"] = n / t_cpu
@@ -447,16 +462,21 @@ def main():
                        f"integrals_per_s_{N_STREAMS}_streams": finite(n / (fp["ms_streams"] * 1e-3))},
         "mixed_over_plain": fp["ms"] / mx["ms"],
     }
-    achieved = SE_FLOP[1] * evals / (mx["ms"] * 1e-3) / 1e12
+    na = se["na"]
+    fl1, va1 = se_ops(1, na, mx["shape"]["block"])
+    fl0, va0 = se_ops(0, na, fp["shape"]["block"])
+    achieved = fl1 * evals / (mx["ms"] * 1e-3) / 1e12
     peak = PEAK_FP64_VALU / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                        "frac": achieved / peak, "traffic": measured_traffic(), "kernel": "se_kernel<MIXED>",
-                       "flop_per_eval": SE_FLOP[1], "evals_per_launch": evals,
-                       "valu_issue_frac": valu_issue_frac(1, evals, mx["ms"] * 1e-3),
-                       "plain_fp64": {"flop_per_eval": SE_FLOP[0],
-                                      "achieved": SE_FLOP[0] * evals / (fp["ms"] * 1e-3) / 1e12,
-                                      "frac": SE_FLOP[0] * evals / (fp["ms"] * 1e-3) / PEAK_FP64_VALU,
-                                      "valu_issue_frac": valu_issue_frac(0, evals, fp["ms"] * 1e-3)}}
+                       "flop_per_eval": fl1, "valu_per_eval": va1, "loop": se_loop(na, mx["shape"]["block"]),
+                       "evals_per_launch": evals,
+                       "valu_issue_frac": valu_issue_frac(va1, evals, mx["ms"] * 1e-3),
+                       "plain_fp64": {"flop_per_eval": fl0, "valu_per_eval": va0,
+                                      "loop": se_loop(na, fp["shape"]["block"]),
+                                      "achieved": fl0 * evals / (fp["ms"] * 1e-3) / 1e12,
+                                      "frac": fl0 * evals / (fp["ms"] * 1e-3) / PEAK_FP64_VALU,
+                                      "valu_issue_frac": valu_issue_frac(va0, evals, fp["ms"] * 1e-3)}}
     if not args.no_cpu_baseline:
         t_cpu, runs = cpu_timed(se["ref_fn"], args.cpu_seconds)
         from oracle import oracle as O
