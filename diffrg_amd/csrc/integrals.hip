@@ -231,9 +231,9 @@ namespace
     const SECellY *tabA;  // mixed-path cell data: (y_i^2, 1/(b + y_i - c), y_i)
     const SECellZ *tabB;  // mixed-path cell data: (Z_i, Z_{i+1} - Z_i)
     double x_extent, k, pref; // pref = 2pi/(2pi)^4
-    double *partials;
-    unsigned *counters; // per integral
-    unsigned *queue;    // [0] next tile, [1] exited blocks (persistent mode)
+    double *partials;   // one line per tile (dev::kSlotStride)
+    unsigned *counters; // one line per integral (dev::kCounterStride)
+    unsigned *queue;    // [0] next tile, [kCounterStride] exited blocks (persistent mode)
     int persistent;
     double *out;
   };
@@ -484,17 +484,18 @@ namespace
       }
       const double part = dev::block_sum(acc, s_red);
       double total = 0.0;
-      if (dev::finish_group(part, A.partials, T.slot, T.first_slot, T.n_slots, &A.counters[i], &total)) {
+      if (dev::finish_group(part, A.partials, T.slot, T.first_slot, T.n_slots,
+                            &A.counters[(long long)i * dev::kCounterStride], &total)) {
         if (threadIdx.x == 0) A.out[i] = total;
       }
     }
     // persistent mode: the last block out resets the queue for the next launch / replay
     // (every earlier queue access of this block is an atomic whose result has been used)
     if (A.persistent && threadIdx.x == 0) {
-      const unsigned done = dev::take_ticket(&A.queue[1]);
+      const unsigned done = dev::take_ticket(&A.queue[dev::kCounterStride]);
       if (done == gridDim.x - 1) {
         dev::reset_counter(&A.queue[0]);
-        dev::reset_counter(&A.queue[1]);
+        dev::reset_counter(&A.queue[dev::kCounterStride]);
       }
     }
   }
@@ -602,7 +603,7 @@ namespace
       b->d_partials = nullptr;
       b->tiles_cap = b->n_tiles;
       DFRG_HIP(hipMalloc((void **)&b->d_tiles, sizeof(SETile) * b->tiles_cap));
-      DFRG_HIP(hipMalloc((void **)&b->d_partials, sizeof(double) * b->tiles_cap));
+      DFRG_HIP(hipMalloc((void **)&b->d_partials, sizeof(double) * dev::kSlotStride * b->tiles_cap));
     }
     DFRG_HIP(hipMemcpy(b->d_tiles, tiles.data(), sizeof(SETile) * tiles.size(), hipMemcpyHostToDevice));
     return DFRG_OK;
@@ -822,14 +823,14 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
   DFRG_HIP_B(hipMalloc((void **)&b->d_tab64, sizeof(double) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tabA, sizeof(SECellY) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tabB, sizeof(SECellZ) * tables_len));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_counters, sizeof(unsigned) * n_int));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_queue, sizeof(unsigned) * 2));
+  DFRG_HIP_B(hipMalloc((void **)&b->d_counters, sizeof(unsigned) * dev::kCounterStride * n_int));
+  DFRG_HIP_B(hipMalloc((void **)&b->d_queue, sizeof(unsigned) * 2 * dev::kCounterStride));
   DFRG_HIP_B(hipMemcpy(b->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tab64, tables, sizeof(double) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tabA, tabA.data(), sizeof(SECellY) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tabB, tabB.data(), sizeof(SECellZ) * tables_len, hipMemcpyHostToDevice));
-  DFRG_HIP_B(hipMemset(b->d_counters, 0, sizeof(unsigned) * n_int));
-  DFRG_HIP_B(hipMemset(b->d_queue, 0, sizeof(unsigned) * 2));
+  DFRG_HIP_B(hipMemset(b->d_counters, 0, sizeof(unsigned) * dev::kCounterStride * n_int));
+  DFRG_HIP_B(hipMemset(b->d_queue, 0, sizeof(unsigned) * 2 * dev::kCounterStride));
 #undef DFRG_HIP_B
   // signature for the autotune cache
   {

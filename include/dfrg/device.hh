@@ -108,29 +108,59 @@ namespace dfrg
       __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
+    // Cross-block scratch takes one 128-byte line per partial slot and per counter.  Device-scope
+    // atomics and write-through stores that hit one line are serialised where the line is
+    // coherent, so slots and counters that many blocks touch at the same time get lines of their
+    // own.
+    constexpr int kSlotStride = 16;    // doubles per partial slot
+    constexpr int kCounterStride = 32; // unsigned per counter
+
+    // Lane i's value of v, for a wave-uniform i (two 32-bit lane reads into scalar registers).
+    __device__ __forceinline__ double lane_value(double v, int i)
+    {
+      const long long b = __double_as_longlong(v);
+      const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, i);
+      const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), i);
+      return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    }
+    __device__ __forceinline__ float lane_value(float v, int i)
+    {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+    }
+
     // ------------------------------------------------------------------------------------------
     // Deterministic cross-block reduction.  Each block publishes its partial and takes a ticket;
     // the block that takes the last ticket of a group sums all partials of that group in index
     // order (compensated) and resets the group's counter so the next launch (or graph replay)
-    // starts from zero.  Returns true in the finishing block, with the group total in *total
-    // (valid in thread 0).
+    // starts from zero.  Slot s lives at partials[s * kSlotStride].  The finishing block's first
+    // wave fetches the partials 64 at a time, one
+    // per lane, so the fetches of a group cost one memory round trip per 64 slots instead of one
+    // per slot; the sum then runs over the lanes in slot order.  Returns true in the finishing
+    // block, with the group total in *total (valid in thread 0).
     // ------------------------------------------------------------------------------------------
     __device__ __forceinline__ bool finish_group(double partial, double *partials, int slot, int first_slot,
                                                  int n_slots, unsigned *counter, double *total)
     {
       __shared__ bool last;
       if (threadIdx.x == 0) {
-        publish(&partials[slot], partial);
+        publish(&partials[(long long)slot * kSlotStride], partial);
         wait_stores();
         last = take_ticket(counter) == (unsigned)(n_slots - 1);
       }
       __syncthreads();
       if (!last) return false;
-      if (threadIdx.x == 0) {
+      if (threadIdx.x < 64) {
         KahanSum s;
-        for (int i = 0; i < n_slots; ++i) s.add(fetch(&partials[first_slot + i]));
-        *total = s.value();
-        reset_counter(counter);
+        for (int c = 0; c < n_slots; c += 64) {
+          const int j = c + (int)threadIdx.x;
+          const double v = j < n_slots ? fetch(&partials[(long long)(first_slot + j) * kSlotStride]) : 0.0;
+          const int m = n_slots - c < 64 ? n_slots - c : 64;
+          for (int l = 0; l < m; ++l) s.add(lane_value(v, l));
+        }
+        if (threadIdx.x == 0) {
+          *total = s.value();
+          reset_counter(counter);
+        }
       }
       return true;
     }

@@ -94,25 +94,48 @@ namespace dfrg
         return NT(dev::fetch(&p->re), dev::fetch(&p->im));
     }
 
-    // Publish this block's partial in slot `slot` of a group of n_slots; the last block of the group
-    // adds the partials in slot order and writes *result (then resets the counter for the next use
-    // of the scratch block).
+    // partial slots of a single call: one 128-byte line each (dfrg::dev::kSlotStride)
+    template <class NT> constexpr int slot_stride = sizeof(NT) >= 128 ? 1 : int(128 / sizeof(NT));
+
+    template <class NT> __device__ __forceinline__ NT lane_nt(const NT &v, int i)
+    {
+      if constexpr (std::is_same_v<NT, double> || std::is_same_v<NT, float>)
+        return dev::lane_value(v, i);
+      else
+        return NT(dev::lane_value(v.re, i), dev::lane_value(v.im, i));
+    }
+
+    // Publish this block's partial in slot `slot` (at partials[slot * stride]) of a group of n_slots;
+    // the last block of the group adds the partials in slot order and writes *result (then resets
+    // the counter for the next use of the scratch block).  The first wave of that block fetches the partials one per lane, 64 at
+    // a time, and adds them lane by lane in slot order.
     template <class NT>
     __device__ __forceinline__ void finish_slots(NT part, NT *partials, int slot, int n_slots, unsigned *counter,
-                                                 NT *result)
+                                                 NT *result, int stride = 1)
     {
       __shared__ bool last;
       if (threadIdx.x == 0) {
-        publish_nt(&partials[slot], part);
+        publish_nt(&partials[(long long)slot * stride], part);
         dev::wait_stores();
         last = dev::take_ticket(counter) == (unsigned)(n_slots - 1);
       }
       __syncthreads();
-      if (last && threadIdx.x == 0) {
-        NT s = fetch_nt(&partials[0]);
-        for (int i = 1; i < n_slots; ++i) s += fetch_nt(&partials[i]);
-        *result = s;
-        dev::reset_counter(counter);
+      if (last && threadIdx.x < 64) {
+        NT s = NT(0);
+        for (int c = 0; c < n_slots; c += 64) {
+          const int j = c + (int)threadIdx.x;
+          const NT v = j < n_slots ? fetch_nt(&partials[(long long)j * stride]) : NT(0);
+          const int m = n_slots - c < 64 ? n_slots - c : 64;
+          if (c == 0) {
+            s = lane_nt(v, 0);
+            for (int l = 1; l < m; ++l) s += lane_nt(v, l);
+          } else
+            for (int l = 0; l < m; ++l) s += lane_nt(v, l);
+        }
+        if (threadIdx.x == 0) {
+          *result = s;
+          dev::reset_counter(counter);
+        }
       }
     }
 
@@ -317,7 +340,7 @@ namespace dfrg
                                                             unsigned *counter, NT *result, T... t)
     {
       const NT acc = radial_sum<d, ctype, NT, KERNEL>(xq, wq, nx, x_extent, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
     }
     template <int d, class ctype, class NT, class KERNEL, class Args>
     __global__ __launch_bounds__(kBlock) void radial_batch_kernel(const ctype *__restrict__ xq,
@@ -331,7 +354,8 @@ namespace dfrg
             return radial_sum<d, ctype, NT, KERNEL>(xq, wq, nx, x_extent, b, nb, k, t...);
           },
           args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
+                   counters + (long long)r * dev::kCounterStride, results + r);
     }
 
     template <class ctype, class NT, class KERNEL, class... T>
@@ -341,7 +365,7 @@ namespace dfrg
                                                             unsigned *counter, NT *result, T... t)
     {
       const NT acc = angle2_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
     }
     template <class ctype, class NT, class KERNEL, class Args>
     __global__ __launch_bounds__(kBlock) void angle2_batch_kernel(const ctype *__restrict__ xq,
@@ -357,7 +381,8 @@ namespace dfrg
             return angle2_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
           },
           args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
+                   counters + (long long)r * dev::kCounterStride, results + r);
     }
 
     template <class ctype, class NT, class KERNEL, class... T>
@@ -367,7 +392,7 @@ namespace dfrg
                                                             unsigned *counter, NT *result, T... t)
     {
       const NT acc = angle3_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
     }
     template <class ctype, class NT, class KERNEL, class Args>
     __global__ __launch_bounds__(kBlock) void angle3_batch_kernel(const ctype *__restrict__ xq,
@@ -383,7 +408,8 @@ namespace dfrg
             return angle3_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
           },
           args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
+                   counters + (long long)r * dev::kCounterStride, results + r);
     }
 
     template <int d, class ctype, class NT, class KERNEL, class... T>
@@ -396,7 +422,7 @@ namespace dfrg
     {
       const NT acc =
           finiteTq0_sum<d, ctype, NT, KERNEL>(xq, wq, mx, mw, nx, nm, x_extent, m_T, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
     }
     template <int d, class ctype, class NT, class KERNEL, class Args>
     __global__ __launch_bounds__(kBlock) void finiteTq0_batch_kernel(const ctype *__restrict__ xq,
@@ -413,7 +439,8 @@ namespace dfrg
             return finiteTq0_sum<d, ctype, NT, KERNEL>(xq, wq, mx, mw, nx, nm, x_extent, m_T, b, nb, k, t...);
           },
           args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
+                   counters + (long long)r * dev::kCounterStride, results + r);
     }
 
     template <int d, class ctype, class NT, class KERNEL, class... T>
@@ -424,7 +451,7 @@ namespace dfrg
     {
       const NT acc =
           angle1_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
     }
     template <int d, class ctype, class NT, class KERNEL, class Args>
     __global__ __launch_bounds__(kBlock) void angle1_batch_kernel(const ctype *__restrict__ xq,
@@ -440,7 +467,8 @@ namespace dfrg
             return angle1_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
           },
           args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
+                   counters + (long long)r * dev::kCounterStride, results + r);
     }
 
     template <class ctype, class NT, class KERNEL, class... T>
@@ -451,7 +479,7 @@ namespace dfrg
     {
       const NT acc = angle2_finiteT_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T,
                                                            blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
     }
     template <class ctype, class NT, class KERNEL, class Args>
     __global__ __launch_bounds__(kBlock) void angle2_finiteT_batch_kernel(
@@ -466,7 +494,8 @@ namespace dfrg
                                                          t...);
           },
           args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
+                   counters + (long long)r * dev::kCounterStride, results + r);
     }
 
     template <int d, class ctype, class NT, class KERNEL, class... T>
@@ -477,7 +506,7 @@ namespace dfrg
     {
       const NT acc = angle1_finiteT_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T,
                                                               blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result);
+      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
     }
     template <int d, class ctype, class NT, class KERNEL, class Args>
     __global__ __launch_bounds__(kBlock) void angle1_finiteT_batch_kernel(
@@ -492,7 +521,8 @@ namespace dfrg
                                                             k, t...);
           },
           args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb, counters + r, results + r);
+      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
+                   counters + (long long)r * dev::kCounterStride, results + r);
     }
 
     // ------------------------------------------------------------------------------------------
@@ -515,7 +545,7 @@ namespace dfrg
       void *h_result = nullptr;
       NT constant = NT(0);
 
-      static constexpr size_t counter_offset = ((kMaxBlocks * sizeof(NT) + 255) / 256) * 256;
+      static constexpr size_t counter_offset = ((kMaxBlocks * slot_stride<NT> * sizeof(NT) + 255) / 256) * 256;
       static constexpr size_t result_offset = counter_offset + 256;
       static constexpr size_t bytes = result_offset + 256;
 
@@ -593,7 +623,7 @@ namespace dfrg
         const hipStream_t s = (hipStream_t)sv;
         const size_t o_part = up(sizeof(Args) * n);
         const size_t o_cnt = o_part + up(sizeof(NT) * (size_t)n * nb);
-        const size_t o_res = o_cnt + up(sizeof(unsigned) * n);
+        const size_t o_res = o_cnt + up(sizeof(unsigned) * dev::kCounterStride * n);
         const size_t bytes = o_res + up(sizeof(NT) * n);
         const size_t h_res_off = up(sizeof(Args) * n);
         void *d = nullptr, *h = nullptr;
@@ -602,7 +632,7 @@ namespace dfrg
         char *dc = (char *)d, *hc = (char *)h;
         std::memcpy(hc, (const void *)args.data(), sizeof(Args) * n);
         bool ok = hipMemcpyAsync(dc, hc, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
-                  hipMemsetAsync(dc + o_cnt, 0, sizeof(unsigned) * n, s) == hipSuccess;
+                  hipMemsetAsync(dc + o_cnt, 0, sizeof(unsigned) * dev::kCounterStride * n, s) == hipSuccess;
         if (ok) {
           launch((const Args *)dc, n, nb, (NT *)(dc + o_part), (unsigned *)(dc + o_cnt), (NT *)(dc + o_res), s);
           ok = hipGetLastError() == hipSuccess &&
