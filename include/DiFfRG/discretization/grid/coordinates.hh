@@ -1,11 +1,15 @@
-// DiFfRG/discretization/grid/coordinates.hh -- 1D index <-> momentum maps used by the
-// interpolators and for external-momentum grids (reference coordinates.hh:97-252).
+// DiFfRG/discretization/grid/coordinates.hh -- index <-> momentum maps used by the interpolators
+// and for external-momentum grids (reference coordinates.hh:15-252): linear and logarithmic 1D
+// coordinates and their N-dimensional products.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <stdexcept>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 // coordinate systems: a ctype, a dimension and a size (reference coordinates.hh:15-19)
@@ -79,11 +83,93 @@ namespace DiFfRG
     NT a, b, c;
   };
 
-  template <typename Coordinates> std::vector<typename Coordinates::ctype> make_grid(const Coordinates &coordinates)
+  // Product of 1D coordinate systems (reference coordinates.hh:24-95): forward/backward act axis by
+  // axis, sizes() lists the axis lengths, size() is their product.  Row-major order (last axis
+  // fastest) everywhere a pack indexes a table.
+  template <typename... Coordinates> class CoordinatePackND
   {
-    static_assert(Coordinates::dim == 1, "make_grid: 1D coordinates");
-    std::vector<typename Coordinates::ctype> grid(coordinates.size());
-    for (unsigned i = 0; i < coordinates.size(); ++i) grid[i] = coordinates.forward(i);
+    static_assert(sizeof...(Coordinates) > 0, "CoordinatePackND requires at least one coordinate system");
+    static_assert(((Coordinates::dim == 1) && ...), "CoordinatePackND requires all coordinates to have dim 1");
+    using First = std::tuple_element_t<0, std::tuple<Coordinates...>>;
+    static_assert((std::is_same_v<typename Coordinates::ctype, typename First::ctype> && ...),
+                  "CoordinatePackND requires one ctype");
+
+  public:
+    using ctype = typename First::ctype;
+    static constexpr unsigned dim = sizeof...(Coordinates);
+
+    CoordinatePackND(Coordinates... c) : coordinates(c...) {}
+
+    template <typename... I> __forceinline__ __host__ __device__ std::array<ctype, dim> forward(I &&...i) const
+    {
+      static_assert(sizeof...(I) == dim);
+      return map(std::make_index_sequence<dim>(), [](const auto &c, const auto &v) { return c.forward(v); }, i...);
+    }
+    template <typename... I> __forceinline__ __host__ __device__ std::array<ctype, dim> backward(I &&...i) const
+    {
+      static_assert(sizeof...(I) == dim);
+      return map(std::make_index_sequence<dim>(), [](const auto &c, const auto &v) { return c.backward(v); }, i...);
+    }
+    template <unsigned i> const auto &get_coordinates() const { return std::get<i>(coordinates); }
+    unsigned size() const
+    {
+      unsigned n = 1;
+      for (unsigned s : sizes()) n *= s;
+      return n;
+    }
+    std::array<unsigned, dim> sizes() const
+    {
+      return std::apply([](const auto &...c) { return std::array<unsigned, dim>{{c.size()...}}; }, coordinates);
+    }
+
+  private:
+    template <size_t... Is, typename F, typename... I>
+    __forceinline__ __host__ __device__ std::array<ctype, dim> map(std::index_sequence<Is...>, F f, const I &...i) const
+    {
+      const auto args = std::forward_as_tuple(i...);
+      return {{ctype(f(std::get<Is>(coordinates), std::get<Is>(args)))...}};
+    }
+    std::tuple<Coordinates...> coordinates;
+  };
+
+  // physical points of a coordinate system in table order (reference coordinates.hh:199-230):
+  // values for 1D, std::array points for 2D and 3D
+  template <typename Coordinates> auto make_grid(const Coordinates &coordinates)
+  {
+    using ctype = typename Coordinates::ctype;
+    if constexpr (Coordinates::dim == 1) {
+      std::vector<ctype> grid(coordinates.size());
+      for (unsigned i = 0; i < coordinates.size(); ++i) grid[i] = coordinates.forward(i);
+      return grid;
+    } else if constexpr (Coordinates::dim == 2) {
+      const auto n = coordinates.sizes();
+      std::vector<std::array<ctype, 2>> grid(coordinates.size());
+      for (unsigned i = 0; i < n[0]; ++i)
+        for (unsigned j = 0; j < n[1]; ++j) {
+          const auto v = coordinates.forward(i, j);
+          grid[i * n[1] + j] = {{ctype(v[0]), ctype(v[1])}};
+        }
+      return grid;
+    } else {
+      static_assert(Coordinates::dim == 3, "make_grid: 1D, 2D and 3D coordinates");
+      const auto n = coordinates.sizes();
+      std::vector<std::array<ctype, 3>> grid(coordinates.size());
+      for (unsigned i = 0; i < n[0]; ++i)
+        for (unsigned j = 0; j < n[1]; ++j)
+          for (unsigned k = 0; k < n[2]; ++k) {
+            const auto v = coordinates.forward(i, j, k);
+            grid[(i * n[1] + j) * n[2] + k] = {{ctype(v[0]), ctype(v[1]), ctype(v[2])}};
+          }
+      return grid;
+    }
+  }
+
+  // flat table indices in the order of make_grid (reference coordinates.hh:232-252)
+  template <typename Coordinates> std::vector<double> make_idx_grid(const Coordinates &coordinates)
+  {
+    static_assert(Coordinates::dim >= 1 && Coordinates::dim <= 3, "make_idx_grid: 1D, 2D and 3D coordinates");
+    std::vector<double> grid(coordinates.size());
+    for (unsigned i = 0; i < grid.size(); ++i) grid[i] = i;
     return grid;
   }
 } // namespace DiFfRG

@@ -1,0 +1,112 @@
+// DiFfRG/physics/interpolation/common.hh -- shared pieces of the interpolators: the return type of
+// the FP32 ("Tex") tables (reference interpolation/common.hh:8-25) and the host + HBM table
+// storage behind every interpolator.
+//
+// Copies of an interpolator share one storage and are non-owning views: the copy an integrator
+// passes by value into a kernel reads the HBM table, and update() on a view throws, as in the
+// reference (linear_interpolation_1D.cu:41).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+
+#include <DiFfRG/common/hip_prefix.hh>
+
+namespace DiFfRG
+{
+  namespace internal
+  {
+    // value type of the FP32 tables; autodiff values are not available in this build
+    template <typename NT> struct __TLITypes;
+    template <> struct __TLITypes<float> {
+      using ReturnType = float;
+    };
+    template <> struct __TLITypes<double> {
+      using ReturnType = float;
+    };
+
+    // n values of type V on the host and in HBM
+    template <typename V> class TableStorage
+    {
+    public:
+      explicit TableStorage(unsigned n) : n(n)
+      {
+        if (n == 0) throw std::runtime_error("interpolator: empty table");
+        ensure_device();
+        host = std::shared_ptr<V[]>(new V[n]());
+        void *p = nullptr;
+        dfrg_check(dfrg_malloc(&p, sizeof(V) * n), "interpolator table");
+        dev = std::shared_ptr<void>(p, [](void *q) { dfrg_free(q); });
+      }
+      V *host_ptr() const { return host.get(); }
+      const V *device_ptr() const { return (const V *)dev.get(); }
+      void upload() const
+      {
+        dfrg_check(dfrg_memcpy_h2d(dev.get(), host.get(), sizeof(V) * n, nullptr), "interpolator upload");
+        dfrg_check(dfrg_stream_sync(nullptr), "interpolator upload");
+      }
+      unsigned size() const { return n; }
+
+    private:
+      unsigned n;
+      std::shared_ptr<V[]> host;
+      std::shared_ptr<void> dev;
+    };
+
+    // Neighbours and weights of a clamped index on an axis of n >= 2 points: i1 = min(ceil(idx +
+    // 1e-16), n - 1), i0 = i1 - 1, w0 = i1 - idx, w1 = idx - i0 (the corner rule of the reference's
+    // multilinear interpolators, linear_interpolation_2D.hh:93-106).  The clamp is fmax(0, fmin(idx,
+    // n - 1)), the reference's device semantics: a NaN index (a point the coordinates cannot map,
+    // such as log1p below -1) clamps to the last point, on the host as on the device.
+    template <typename C> struct Axis {
+      unsigned i0, i1;
+      C w0, w1;
+      __forceinline__ __host__ __device__ Axis(C idx, unsigned n)
+      {
+        idx = fmax(C(0), fmin(idx, C(n - 1)));
+        const C top = fmin(ceil(idx + C(1e-16)), C(n - 1));
+        i1 = top < C(1) ? 1u : unsigned(top);
+        i0 = i1 - 1;
+        w0 = C(i1) - idx;
+        w1 = idx - C(i0);
+      }
+    };
+
+    // Clamped 1D lerp between floor and ceil (reference linear_interpolation_1D.hh:75-92), same clamp.
+    template <typename V, typename C> __forceinline__ __host__ __device__ V lerp1(const V *d, C idx, unsigned n)
+    {
+      idx = fmax(C(0), fmin(idx, C(n - 1)));
+      const C fl = floor(idx);
+      return d[unsigned(fl)] * (C(1) - idx + fl) + d[unsigned(ceil(idx))] * (idx - fl);
+    }
+
+    template <typename V, typename C>
+    __forceinline__ __host__ __device__ V lerp2(const V *d, C ix, C iy, unsigned nx, unsigned ny)
+    {
+      const Axis<C> x(ix, nx), y(iy, ny);
+      const V *r0 = d + (size_t)x.i0 * ny, *r1 = d + (size_t)x.i1 * ny;
+      return r0[y.i0] * x.w0 * y.w0 + r0[y.i1] * x.w0 * y.w1 + r1[y.i0] * x.w1 * y.w0 + r1[y.i1] * x.w1 * y.w1;
+    }
+
+    template <typename V, typename C>
+    __forceinline__ __host__ __device__ V lerp3(const V *d, C ix, C iy, C iz, unsigned nx, unsigned ny, unsigned nz)
+    {
+      const Axis<C> x(ix, nx), y(iy, ny), z(iz, nz);
+      auto at = [&](unsigned a, unsigned b, unsigned c) { return d[((size_t)a * ny + b) * nz + c]; };
+      return at(x.i0, y.i0, z.i0) * x.w0 * y.w0 * z.w0 + at(x.i0, y.i0, z.i1) * x.w0 * y.w0 * z.w1 +
+             at(x.i0, y.i1, z.i0) * x.w0 * y.w1 * z.w0 + at(x.i0, y.i1, z.i1) * x.w0 * y.w1 * z.w1 +
+             at(x.i1, y.i0, z.i0) * x.w1 * y.w0 * z.w0 + at(x.i1, y.i0, z.i1) * x.w1 * y.w0 * z.w1 +
+             at(x.i1, y.i1, z.i0) * x.w1 * y.w1 * z.w0 + at(x.i1, y.i1, z.i1) * x.w1 * y.w1 * z.w1;
+    }
+
+    inline void require_axes(const unsigned *n, int dim, const char *who)
+    {
+      for (int i = 0; i < dim; ++i)
+        if (n[i] < 2) throw std::runtime_error(std::string(who) + ": every axis needs at least 2 points");
+    }
+  } // namespace internal
+} // namespace DiFfRG

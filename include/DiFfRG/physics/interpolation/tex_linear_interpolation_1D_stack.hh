@@ -1,0 +1,94 @@
+// DiFfRG/physics/interpolation/tex_linear_interpolation_1D_stack.hh --
+// TexLinearInterpolator1DStack<NT, Coordinates, max_stack_size> (reference
+// tex_linear_interpolation_1D_stack.hh:20-150): a stack of FP32 1D tables, one per Matsubara
+// frequency (or index) of 2D stack coordinates such as BosonicCoordinates1DFiniteT.  operator()(m,
+// p) takes the stack entry the coordinates select for m and lerps the momentum index, clamped.
+// The stack is one row-major FP32 array in HBM (row s at s * p_size); FP32 lerp on device and host.
+#pragma once
+
+#include <vector>
+
+#include <DiFfRG/common/math.hh>
+#include <DiFfRG/discretization/grid/combined_coordinates.hh>
+#include <DiFfRG/physics/interpolation/common.hh>
+
+namespace DiFfRG
+{
+  template <typename NT, typename Coordinates, unsigned max_stack_size = 64> class TexLinearInterpolator1DStack
+  {
+    static_assert(Coordinates::dim == 2, "TexLinearInterpolator1DStack requires 2D coordinates");
+
+  public:
+    using ReturnType = typename internal::__TLITypes<NT>::ReturnType;
+
+    TexLinearInterpolator1DStack(const std::vector<NT> &data, const Coordinates &coordinates)
+        : TexLinearInterpolator1DStack(data.data(), coordinates)
+    {
+      if (data.size() != coordinates.size()) throw std::runtime_error("Data and coordinates must have the same size");
+    }
+    TexLinearInterpolator1DStack(const NT *data, const Coordinates &coordinates)
+        : TexLinearInterpolator1DStack(coordinates)
+    {
+      update(data);
+    }
+    TexLinearInterpolator1DStack(const Coordinates &coordinates)
+        : stack_size(coordinates.sizes()[0]), p_size(coordinates.sizes()[1]), coordinates(coordinates),
+          table(std::make_shared<internal::TableStorage<float>>(coordinates.size())), owner(true)
+    {
+      if (stack_size > max_stack_size)
+        throw std::runtime_error("TexLinearInterpolator1DStack: stack size exceeds max_stack_size");
+      if (p_size < 2) throw std::runtime_error("TexLinearInterpolator1DStack: every table needs at least 2 points");
+      d_table = table->device_ptr();
+      h_table = table->host_ptr();
+      table->upload();
+    }
+    template <typename Coordinates2>
+    TexLinearInterpolator1DStack(const Coordinates2 &coordinates, const unsigned m_start, const unsigned m_end,
+                                 const double T)
+        : TexLinearInterpolator1DStack(Coordinates(coordinates, m_start, m_end, T))
+    {
+    }
+    TexLinearInterpolator1DStack(const TexLinearInterpolator1DStack &o)
+        : stack_size(o.stack_size), p_size(o.p_size), coordinates(o.coordinates), table(o.table), d_table(o.d_table),
+          h_table(o.h_table), owner(false)
+    {
+    }
+
+    template <typename NT2> void update(const NT2 *data)
+    {
+      if (!owner) throw std::runtime_error("Cannot update data of non-owner interpolator");
+      for (unsigned i = 0; i < stack_size * p_size; ++i) h_table[i] = static_cast<float>(data[i]);
+      update();
+    }
+    void update()
+    {
+      if (!owner) throw std::runtime_error("Cannot update data of non-owner interpolator");
+      table->upload();
+    }
+    float *data(const unsigned s) const { return h_table + (size_t)s * p_size; }
+    ReturnType &operator[](const unsigned i) { return h_table[i]; }
+    const ReturnType &operator[](const unsigned i) const { return h_table[i]; }
+    const Coordinates &get_coordinates() const { return coordinates; }
+
+    __host__ __device__ ReturnType operator()(const float m, const float p) const
+    {
+      const auto mp = coordinates.backward(m, p);
+      int s = int(std::get<0>(mp));
+      s = s < 0 ? 0 : (s >= int(stack_size) ? int(stack_size) - 1 : s);
+#ifdef __HIP_DEVICE_COMPILE__
+      const float *d = d_table;
+#else
+      const float *d = h_table;
+#endif
+      return internal::lerp1(d + (size_t)s * p_size, float(std::get<1>(mp)), p_size);
+    }
+
+  private:
+    unsigned stack_size, p_size;
+    Coordinates coordinates;
+    std::shared_ptr<internal::TableStorage<float>> table;
+    const float *d_table = nullptr;
+    float *h_table = nullptr;
+    bool owner;
+  };
+} // namespace DiFfRG

@@ -59,6 +59,8 @@ int dfrg_malloc(void **d_ptr, size_t bytes);
 int dfrg_free(void *d_ptr);
 int dfrg_host_alloc(void **h_ptr, size_t bytes); /* pinned host memory */
 int dfrg_host_free(void *h_ptr);
+/* Copies are enqueued on `stream` (NULL: the calling thread's stream, below); synchronise that
+ * stream before reusing or reading the host buffer. */
 int dfrg_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes, void *stream);
 int dfrg_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes, void *stream);
 int dfrg_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);

@@ -21,7 +21,7 @@
 #include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_cpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_gpu.hh>
-#include <DiFfRG/physics/interpolation/linear_interpolation_1D.hh>
+#include <DiFfRG/physics/interpolation.hh>
 #include <DiFfRG/physics/regulators.hh>
 #include <DiFfRG/physics/utils.hh>
 
@@ -619,6 +619,159 @@ int kat_variant_requests(int which, int nx, int na, double x_extent, double T, c
            [&](auto &I, double k) { return I.get(k, 0.0, c0, c1, c2, 0., 0., 0., a0, a1, a2, 0., T * T, 0., 1.1, 0.); });
     } else
       throw std::invalid_argument("unknown variant");
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+// interpolators: one evaluation per point inside a kernel (HBM table through the by-value copy)
+// and on the host
+// ---------------------------------------------------------------------------------------------
+} // extern "C"
+
+namespace
+{
+  template <int DIM, class I>
+  __global__ void interp_eval_kernel(I interp, const double *pts, int n, double *out)
+  {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *x = pts + (size_t)i * DIM;
+    if constexpr (DIM == 1)
+      out[i] = double(interp(x[0]));
+    else if constexpr (DIM == 2)
+      out[i] = double(interp(x[0], x[1]));
+    else
+      out[i] = double(interp(x[0], x[1], x[2]));
+  }
+
+  template <int DIM, class I> void interp_eval(const I &interp, const double *pts, int n, double *out_dev, double *out_host)
+  {
+    void *d_pts = nullptr, *d_out = nullptr;
+    dfrg_check(dfrg_malloc(&d_pts, sizeof(double) * DIM * n), "pts");
+    dfrg_check(dfrg_malloc(&d_out, sizeof(double) * n), "out");
+    // the library's copies run on this thread's stream: launch there too
+    void *s = nullptr;
+    dfrg_check(dfrg_thread_stream(&s), "thread stream");
+    dfrg_check(dfrg_memcpy_h2d(d_pts, pts, sizeof(double) * DIM * n, s), "pts");
+    hipLaunchKernelGGL((interp_eval_kernel<DIM, I>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)s, interp,
+                       (const double *)d_pts, n, (double *)d_out);
+    check_hip("interp_eval_kernel");
+    dfrg_check(dfrg_memcpy_d2h(out_dev, d_out, sizeof(double) * n, s), "out");
+    dfrg_check(dfrg_stream_sync(s), "out");
+    dfrg_free(d_pts);
+    dfrg_free(d_out);
+    for (int i = 0; i < n; ++i) {
+      const double *x = pts + (size_t)i * DIM;
+      if constexpr (DIM == 1)
+        out_host[i] = double(interp(x[0]));
+      else if constexpr (DIM == 2)
+        out_host[i] = double(interp(x[0], x[1]));
+      else
+        out_host[i] = double(interp(x[0], x[1], x[2]));
+    }
+  }
+} // namespace
+
+extern "C" {
+
+// kind: 0 LinearInterpolator2D<double>, 1 LinearInterpolator3D<double>, 2 TexLinearInterpolator1D
+// (log coordinates), 3 TexLinearInterpolator2D, 4 TexLinearInterpolator3D, 5 TexLinearInterpolator1DStack
+// (bosonic), 6 TexLinearInterpolator1DStack (fermionic), 7 LinearInterpolator2D<float>.
+// cp: linear axes (n, start, stop) per axis; log (n, start, stop, bias); stacks (m_start, m_stop, T,
+// n, start, stop, bias).  pts: n_pts points of the interpolator's dimension.
+int kat_interp(int kind, const double *cp, const double *data, int n_data, const double *pts, int n_pts,
+               double *out_dev, double *out_host)
+{
+  return guard([&] {
+    using LD = LinearCoordinates1D<double>;
+    using LF = LinearCoordinates1D<float>;
+    auto need = [&](unsigned n) {
+      if ((unsigned)n_data != n) throw std::invalid_argument("data size does not match the coordinates");
+    };
+    switch (kind) {
+    case 0: {
+      CoordinatePackND<LD, LD> c(LD(cp[0], cp[1], cp[2]), LD(cp[3], cp[4], cp[5]));
+      need(c.size());
+      LinearInterpolator2D<double, decltype(c)> I(c);
+      I.update(data);
+      interp_eval<2>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 1: {
+      CoordinatePackND<LD, LD, LD> c(LD(cp[0], cp[1], cp[2]), LD(cp[3], cp[4], cp[5]), LD(cp[6], cp[7], cp[8]));
+      need(c.size());
+      LinearInterpolator3D<double, decltype(c)> I(std::vector<double>(data, data + n_data), c);
+      interp_eval<3>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 2: {
+      LogarithmicCoordinates1D<float> c((unsigned)cp[0], cp[1], cp[2], cp[3]);
+      need(c.size());
+      TexLinearInterpolator1D<float, decltype(c)> I(c);
+      I.update(data);
+      interp_eval<1>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 3: {
+      CoordinatePackND<LF, LF> c(LF(cp[0], cp[1], cp[2]), LF(cp[3], cp[4], cp[5]));
+      need(c.size());
+      TexLinearInterpolator2D<double, decltype(c)> I(data, c);
+      interp_eval<2>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 4: {
+      CoordinatePackND<LF, LF, LF> c(LF(cp[0], cp[1], cp[2]), LF(cp[3], cp[4], cp[5]), LF(cp[6], cp[7], cp[8]));
+      need(c.size());
+      TexLinearInterpolator3D<float, decltype(c)> I(c);
+      I.update(data);
+      interp_eval<3>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 5: {
+      BosonicCoordinates1DFiniteT<int, float> c((int)cp[0], (int)cp[1], cp[2], (unsigned)cp[3], cp[4], cp[5], cp[6]);
+      need(c.size());
+      TexLinearInterpolator1DStack<float, decltype(c)> I(c);
+      I.update(data);
+      interp_eval<2>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 6: {
+      FermionicCoordinates1DFiniteT<int, float> c((int)cp[0], (int)cp[1], cp[2], (unsigned)cp[3], cp[4], cp[5], cp[6]);
+      need(c.size());
+      TexLinearInterpolator1DStack<float, decltype(c)> I(c);
+      I.update(data);
+      interp_eval<2>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 7: {
+      CoordinatePackND<LF, LF> c(LF(cp[0], cp[1], cp[2]), LF(cp[3], cp[4], cp[5]));
+      need(c.size());
+      LinearInterpolator2D<float, decltype(c)> I(std::vector<float>(c.size(), 0.f), c);
+      I.update(data);
+      interp_eval<2>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    default: throw std::invalid_argument("unknown interpolator kind");
+    }
+  });
+}
+
+// views: update() on a copy throws; stack coordinates reject start > stop
+int kat_interp_errors(int which)
+{
+  return guard([&] {
+    using LD = LinearCoordinates1D<double>;
+    if (which == 0) {
+      CoordinatePackND<LD, LD> c(LD(4, 0., 1.), LD(5, 0., 1.));
+      LinearInterpolator2D<double, decltype(c)> I(c);
+      auto view = I;
+      view.update();
+    } else if (which == 1) {
+      BosonicMatsubaraValues<int, double> m(3, 1, 0.1);
+    } else {
+      CoordinatePackND<LD, LD> c(LD(1, 0., 1.), LD(5, 0., 1.));
+      LinearInterpolator2D<double, decltype(c)> I(c);
+    }
   });
 }
 

@@ -1,0 +1,183 @@
+"""The 2D/3D, FP32 ("Tex") and stacked interpolators (include/DiFfRG/physics/interpolation/) inside
+kernels and on the host, through tests/native/libdfrg_kat.so.
+
+Pinned by what the mathematics fixes: multilinear interpolation reproduces any table that is
+multilinear in the indices exactly (a wrong corner, weight or clamp fails this), at clamped
+indices for points outside the table; plus the reference tests' own data and expected values
+(tests/physics/interpolation/*_gpu.cu) and the device == host agreement they check.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KAT = os.path.join(ROOT, "tests", "native", "libdfrg_kat.so")
+_dp = C.POINTER(C.c_double)
+
+
+@pytest.fixture(scope="module")
+def kat(gpu):
+    lib = C.CDLL(KAT)
+    lib.kat_last_error.restype = C.c_char_p
+    return lib
+
+
+def P(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_dp)
+
+
+def run(kat, kind, cp, data, pts):
+    cp, data, pts = (np.ascontiguousarray(v, dtype=np.float64) for v in (cp, data, pts))
+    n = len(pts)
+    dev, host = np.empty(n), np.empty(n)
+    print(f"[case] interpolator kind {kind}, {len(data)} table values, {n} points", flush=True)
+    rc = kat.kat_interp(kind, P(cp), P(data), len(data), P(pts), n, P(dev), P(host))
+    assert rc == 0, kat.kat_last_error()
+    return dev, host
+
+
+def lin_axis(rng, dtype):
+    n = int(rng.integers(10, 100))
+    start = dtype(rng.uniform(1e-6, 1e-1))
+    stop = dtype(rng.uniform(1, 100) + start)
+    return n, start, stop
+
+
+def lin_backward(x, n, start, stop, dtype):
+    a = dtype((dtype(stop) - dtype(start)) / dtype(n - 1))
+    return (dtype(x) - dtype(start)) / a
+
+
+def points(rng, axes, m, dtype):
+    # mostly inside, some beyond either end of every axis (clamping)
+    cols = []
+    for n, start, stop in axes:
+        u = rng.uniform(-0.1, 1.1, m)
+        cols.append((start + u * (stop - start)).astype(dtype).astype(np.float64))
+    return np.stack(cols, axis=1)
+
+
+def multilinear(coef, idx):
+    # coef over the subsets of axes: sum_S c_S prod_{a in S} idx_a
+    out = np.zeros(len(idx))
+    d = idx.shape[1]
+    for mask in range(1 << d):
+        term = np.full(len(idx), coef[mask])
+        for a in range(d):
+            if mask >> a & 1:
+                term = term * idx[:, a]
+        out += term
+    return out
+
+
+@pytest.mark.parametrize("kind,dim,dtype,tol", [(0, 2, np.float64, 1e-12), (1, 3, np.float64, 1e-12),
+                                                (3, 2, np.float32, 2e-5), (4, 3, np.float32, 2e-5),
+                                                (7, 2, np.float32, 2e-5)])
+def test_multilinear_tables_are_reproduced(kat, kind, dim, dtype, tol):
+    rng = np.random.default_rng(200 + kind)
+    for _ in range(3):
+        axes = [lin_axis(rng, dtype) for _ in range(dim)]
+        shape = [a[0] for a in axes]
+        grids = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+        grid_idx = np.stack([g.ravel() for g in grids], axis=1)
+        coef = rng.uniform(-1, 1, 1 << dim)
+        data = multilinear(coef, grid_idx)
+        pts = points(rng, axes, 300, dtype)
+        dev, host = run(kat, kind, [v for a in axes for v in a], data, pts)
+        idx = np.stack([np.clip(lin_backward(pts[:, a].astype(dtype), *axes[a], dtype).astype(np.float64), 0,
+                                shape[a] - 1) for a in range(dim)], axis=1)
+        want = multilinear(coef, idx)
+        scale = np.sum(np.abs(coef)) * np.prod(shape)
+        assert np.max(np.abs(dev - want)) <= tol * scale
+        assert np.max(np.abs(host - want)) <= tol * scale
+        assert np.max(np.abs(dev - host)) <= tol * scale
+
+
+def test_reference_2D_3D_cases(kat):
+    # linear_interpolation_2D_gpu.cu:24-56 (data = j -> clamped j index), 3D :36-73 (data = k)
+    rng = np.random.default_rng(7)
+    for kind, dim in ((7, 2), (3, 2), (4, 3)):
+        axes = [lin_axis(rng, np.float32) for _ in range(dim)]
+        shape = [a[0] for a in axes]
+        grids = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+        data = grids[-1].ravel()
+        pts = np.stack([(a[1] + rng.uniform(0, 1, 50) * (a[2] - a[1])).astype(np.float32).astype(np.float64)
+                        for a in axes], axis=1)
+        dev, host = run(kat, kind, [v for a in axes for v in a], data, pts)
+        last = np.clip(lin_backward(pts[:, -1].astype(np.float32), *axes[-1], np.float32), 0, shape[-1])
+        for res in (dev, host):
+            assert np.all(np.abs(res - last) <= 1e-6 * np.maximum(1, np.abs(last)) + 1e-5)
+
+
+def log_backward(y, n, start, stop, bias):
+    f = np.float32
+    a, b, c = f(bias), f((f(stop) - f(start)) / f(math.expm1(f(bias)))), f(start)
+    with np.errstate(invalid="ignore"):  # below the table start: NaN, clamped by the caller
+        return np.log1p((f(y) - c) / b) * f(n - 1) / a
+
+
+def test_tex_1D_log_coordinates(kat):
+    # tex_linear_interpolation_1D_gpu.cu:19-57: data = j on log coordinates
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        n = int(rng.integers(10, 100))
+        start = rng.uniform(1e-6, 1e-1)
+        stop = rng.uniform(1, 100) + start
+        bias = rng.uniform(1, 10)
+        cp = [n, np.float32(start), np.float32(stop), np.float32(bias)]
+        data = np.arange(n, dtype=np.float64) * 0.75 - 3.0
+        pts = (start + rng.uniform(-0.05, 1.05, 200) * (stop - start)).astype(np.float32).astype(np.float64)
+        dev, host = run(kat, 2, cp, data, pts)
+        # far below the table start log1p has no real value: the NaN index clamps to the last point
+        # (fmax(0, fmin(NaN, n - 1)), the reference's device clamp)
+        idx = np.clip(np.nan_to_num(log_backward(pts, n, *cp[1:]).astype(np.float64), nan=n - 1), 0, n - 1)
+        want = idx * 0.75 - 3.0
+        assert np.max(np.abs(dev - want)) <= 2e-5 * n
+        assert np.max(np.abs(dev - host)) <= 2e-5 * n
+
+
+@pytest.mark.parametrize("kind,fermionic", [(5, False), (6, True)])
+def test_tex_1D_stack(kat, kind, fermionic):
+    # tex_linear_interpolation_1D_stack_gpu.cu:16-55: data = i p_size + j; frequencies on the stack
+    rng = np.random.default_rng(20 + kind)
+    for _ in range(4):
+        m_start = int(rng.integers(-50, 5))
+        m_size = int(rng.integers(2, 64))
+        T = float(np.float32(rng.uniform(1e-3, 1e-1)))
+        n = int(rng.integers(10, 100))
+        start, stop, bias = (float(np.float32(v)) for v in (rng.uniform(1e-6, 1e-1), rng.uniform(1, 100),
+                                                            rng.uniform(1, 10)))
+        stop += start
+        cp = [m_start, m_start + m_size, T, n, start, stop, bias]
+        data = np.arange(m_size * n, dtype=np.float64)
+        m_idx = rng.integers(0, m_size, 100)
+        shift = 0.5 if fermionic else 0.0
+        m = (m_idx + m_start + shift) * 2 * math.pi * T
+        p = start + rng.uniform(-0.05, 1.05, 100) * (stop - start)
+        pts = np.stack([m, p], axis=1).astype(np.float32).astype(np.float64)
+        dev, host = run(kat, kind, cp, data, pts)
+        pidx = np.clip(np.nan_to_num(log_backward(pts[:, 1], n, start, stop, bias).astype(np.float64), nan=n - 1),
+                       0, n - 1)
+        want = m_idx * n + pidx
+        assert np.max(np.abs(dev - want) / np.maximum(1, want)) <= 2e-6
+        assert np.max(np.abs(dev - host) / np.maximum(1, want)) <= 2e-6
+        # beyond the last frequency: last table, |p| continued by the frequency excess
+        mo = (m_start + m_size + 2 + shift) * 2 * math.pi * T
+        po = 0.5 * (start + stop)
+        dev, _ = run(kat, kind, cp, data, np.array([[np.float32(mo), np.float32(po)]], dtype=np.float64))
+        dm = np.float32(mo) - np.float32((m_start + m_size - 1 + shift) * 2 * math.pi * T)
+        pe = np.sqrt(np.float32(po) ** 2 + dm ** 2)
+        want = (m_size - 1) * n + np.clip(log_backward(pe, n, start, stop, bias), 0, n - 1)
+        assert abs(dev[0] - want) <= 2e-6 * want
+
+
+def test_interpolator_errors(kat):
+    print("[case] interpolator errors", flush=True)
+    assert kat.kat_interp_errors(0) == -2 and b"non-owner" in kat.kat_last_error()
+    assert kat.kat_interp_errors(1) == -1 and b"start must be smaller than stop" in kat.kat_last_error()
+    assert kat.kat_interp_errors(2) == -2 and b"at least 2 points" in kat.kat_last_error()
