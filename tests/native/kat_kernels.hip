@@ -24,6 +24,8 @@
 #include <DiFfRG/physics/interpolation.hh>
 #include <DiFfRG/physics/regulators.hh>
 #include <DiFfRG/physics/utils.hh>
+#include <dfrg/explicit_rk.hh>
+#include <dfrg/x_extent.hh>
 
 #include <chrono>
 #include <memory>
@@ -772,6 +774,124 @@ int kat_interp_errors(int which)
       CoordinatePackND<LD, LD> c(LD(1, 0., 1.), LD(5, 0., 1.));
       LinearInterpolator2D<double, decltype(c)> I(c);
     }
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+// time stepping of variables systems and the x_extent choice
+// ---------------------------------------------------------------------------------------------
+} // extern "C"
+
+namespace
+{
+  // y' = A y for a 2x2 matrix per system (A row-major in a device array), plus a decaying
+  // integral term c_s * int d^4q/(2pi)^4 RBdot(q^2)/(q^2 + RB(q^2) + y0^2)^2 on GL nodes when nx > 0
+  struct LinearRHS {
+    const double *A; // n_systems x 4
+    const double *xq, *wq;
+    int nx;
+    double c;
+    __device__ void operator()(double, const double *y, double *f, int s) const
+    {
+      const double *a = A + 4 * s;
+      f[0] = a[0] * y[0] + a[1] * y[1];
+      f[1] = a[2] * y[0] + a[3] * y[1];
+      if (nx > 0) {
+        double I = 0.0;
+        for (int i = 0; i < nx; ++i) {
+          const double q2 = xq[i] * 20.0;
+          I += wq[i] * 20.0 * (0.5 * q2) * Reg::RBdot(1.0, q2) / powr<2>(q2 + Reg::RB(1.0, q2) + y[0] * y[0]);
+        }
+        f[0] += c * I * (2.0 * M_PI * M_PI / powr<4>(2.0 * M_PI));
+      }
+    }
+  };
+} // namespace
+
+extern "C" {
+
+// host stepper: 2-variable linear systems (no integral term); y: n x 2 in/out; snapshots of the
+// first system: n_out x 2
+int kat_rk_host(int method, const double *A, double *y, int n, double t0, double t1, double dt, double output_dt,
+                double *snap, int n_snap, int *n_written)
+{
+  return guard([&] {
+    int w = 0;
+    for (int s = 0; s < n; ++s) {
+      std::vector<double> v{y[2 * s], y[2 * s + 1]};
+      const double *a = A + 4 * s;
+      dfrg::tmpl::run_variables(
+          v, t0, t1, dt, output_dt, method,
+          [&](double, const std::vector<double> &Y, std::vector<double> &f) {
+            f[0] = a[0] * Y[0] + a[1] * Y[1];
+            f[1] = a[2] * Y[0] + a[3] * Y[1];
+          },
+          [&](double, const std::vector<double> &Y) {
+            if (s == 0 && w < n_snap) {
+              snap[2 * w] = Y[0];
+              snap[2 * w + 1] = Y[1];
+              ++w;
+            }
+          });
+      y[2 * s] = v[0];
+      y[2 * s + 1] = v[1];
+    }
+    *n_written = w;
+  });
+}
+
+// device ensemble of the same systems; with nx > 0 the RHS adds the GL threshold integral
+// (c * I(y0)); snap: n_snap x n x 2; status per system
+int kat_rk_device(int method, const double *A, double *y, int n, double t0, double t1, double dt, double output_dt,
+                  int nx, double c, double *snap, int n_snap, int *status)
+{
+  return guard([&] {
+    void *s = nullptr;
+    dfrg_check(dfrg_thread_stream(&s), "thread stream");
+    void *dA, *dy, *dsnap = nullptr, *dst;
+    dfrg_check(dfrg_malloc(&dA, sizeof(double) * 4 * n), "A");
+    dfrg_check(dfrg_malloc(&dy, sizeof(double) * 2 * n), "y");
+    dfrg_check(dfrg_malloc(&dst, sizeof(int) * n), "status");
+    if (n_snap > 0) dfrg_check(dfrg_malloc(&dsnap, sizeof(double) * 2 * n * n_snap), "snap");
+    dfrg_check(dfrg_memcpy_h2d(dA, A, sizeof(double) * 4 * n, s), "A");
+    dfrg_check(dfrg_memcpy_h2d(dy, y, sizeof(double) * 2 * n, s), "y");
+    LinearRHS rhs{(const double *)dA, nullptr, nullptr, 0, c};
+    if (nx > 0) {
+      rhs.xq = provider().get_device_points<double>(nx);
+      rhs.wq = provider().get_device_weights<double>(nx);
+      rhs.nx = nx;
+    }
+    dfrg::tmpl::rk_ensemble<2>(rhs, method, (double *)dy, n, t0, t1, dt, output_dt, (double *)dsnap, n_snap,
+                               (int *)dst, (hipStream_t)s);
+    dfrg_check(dfrg_memcpy_d2h(y, dy, sizeof(double) * 2 * n, s), "y");
+    dfrg_check(dfrg_memcpy_d2h(status, dst, sizeof(int) * n, s), "status");
+    if (n_snap > 0) dfrg_check(dfrg_memcpy_d2h(snap, dsnap, sizeof(double) * 2 * n * n_snap, s), "snap");
+    dfrg_check(dfrg_stream_sync(s), "sync");
+    dfrg_free(dA);
+    dfrg_free(dy);
+    dfrg_free(dst);
+    if (dsnap) dfrg_free(dsnap);
+  });
+}
+
+int kat_rk_snapshots(double t0, double t1, double dt, double output_dt, int *n)
+{
+  return guard([&] { *n = dfrg::tmpl::rk_snapshots(t0, t1, dt, output_dt); });
+}
+
+// x_extent for f(q^2) = exp(-a q^2) (which = 0), f = 0 (which = 1), or a slowly decaying
+// 1/(1 + q^2)^3 (which = 2)
+int kat_x_extent(int which, double a, unsigned order, double tol, double *x_extent)
+{
+  return guard([&] {
+    std::function<double(double)> f;
+    if (which == 0)
+      f = [a](double q2) { return std::exp(-a * q2); };
+    else if (which == 1)
+      f = [](double) { return 0.0; };
+    else
+      f = [](double q2) { return 1.0 / powr<3>(1.0 + q2); };
+    *x_extent = dfrg::tmpl::optimize_x_extent(provider(), f, order, tol);
   });
 }
 
