@@ -265,16 +265,17 @@ namespace
   // FP32 log2; the in-cell offset y - y_i = (|p-q|^2 - y_i^2)/(y + y_i) takes the difference in
   // FP64 (no cancellation error) and the rest in FP32; the in-cell position s*log1p(u)
   // (|u| <~ 0.05) in FP32 with the rounding of 1+u restored, Z assembled in FP64 from the FP64 node
-  // value; R = k^2 2^t with t split on a 1/64 grid (table of k^2 2^(j/64)) and a degree-4
-  // polynomial remainder in FP64 (2^t for t below -2^25/64 underflows to 0 through the exponent);
+  // value; R = k^2 2^t with t split on a 1/64 grid (table of k^2 2^(j/64)) and a degree-3
+  // polynomial remainder in FP64 (|r| <= ln2/128: truncation below 4e-11 relative; 2^t for t
+  // below -2^25/64 underflows to 0 through the exponent);
   // the denominator in FP64; the reciprocal from an FP32 seed refined in FP64.  Per-evaluation
   // error ~1e-9 relative.
   __device__ __forceinline__ double se_G_mixed(const SEMixed &m, double pq2d)
   {
     const float ys = __builtin_amdgcn_sqrtf((float)pq2d);
-    // cell index: clamp to [0, n-2] first (NaN-safe), then truncation is floor
+    // cell index: clamp to [0, n-2] first (med3 of a NaN gives 0), then truncation is floor
     const float coarse = __builtin_amdgcn_logf(fmaf(ys - m.cf, m.ibf, 1.0f)) * m.scl;
-    const int i0 = (int)fminf(fmaxf(coarse, 0.0f), m.imaxf);
+    const int i0 = (int)__builtin_amdgcn_fmed3f(coarse, 0.0f, m.imaxf); // NaN gives 0
     const SECellY ca = m.tabA[i0];
     const SECellZ cz = m.tabB[i0];
     const float num = (float)(pq2d - ca.y2);
@@ -282,13 +283,13 @@ namespace
     const float w1 = 1.0f + u;
     const float ru = u - (w1 - 1.0f); // rounding error of 1 + u
     float fr = fmaf(__builtin_amdgcn_logf(w1), m.scl, ru * m.sf);
-    fr = fmaxf(0.0f, fminf(fr, 1.0f)); // NaN-safe; realises the [0, n-1] clamp
+    fr = __builtin_amdgcn_fmed3f(fr, 0.0f, 1.0f); // realises the [0, n-1] clamp
     const double Z = cz.z + (double)(fr * cz.dz);
     const double t64 = pq2d * m.ek64; // 64 t, t = -|p-q|^2/k^2 log2(e)
     const double tn = rint(t64);
     const int jn = (int)tn;
     const double r = (t64 - tn) * (kLn2 / 64.0);
-    const double pe = fma(fma(fma(fma(r, 1.0 / 24.0, 1.0 / 6.0), r, 0.5), r, 1.0), r, 1.0);
+    const double pe = fma(fma(fma(r, 1.0 / 6.0, 0.5), r, 1.0), r, 1.0);
     const double R = ldexp(pe * m.exp2t[jn & 63], jn >> 6);
     return dev::rcp_seeded(fma(Z, pq2d, R));
   }

@@ -7,7 +7,7 @@ The FLOP-per-evaluation figures of the roofline come from here: the source is co
 gfx950 with the library's own flags to assembly (-S, device only, into a temporary folder), the
 innermost loops of each kernel are found from their backward branches, and their instructions
 are counted with FMA = 2 FLOP; add, sub, mul, min/max, floor/rint/fract, ldexp and the hardware
-transcendentals (exp, log, rcp, rsq, sqrt) = 1; moves, compares, selects, conversions,
+transcendentals (exp, log, rcp, rsq, sqrt) and med3 (a clamp) = 1; moves, compares, selects, conversions,
 integer and LDS instructions = 0.  VALU counts every vector instruction, `trans` the
 quarter-rate transcendental ones.
 """
@@ -42,7 +42,7 @@ def flop(op: str) -> int:
     if any(t in op for t in ("fma", "fmac", "mad")):
         return 2
     if any(t in op for t in ("add", "sub", "mul", "exp", "log", "rcp", "rsq", "sqrt", "ldexp", "div_", "fract",
-                             "rndne", "floor", "ceil", "trunc", "max", "min")):
+                             "rndne", "floor", "ceil", "trunc", "max", "min", "med3")):
         return 1
     return 0
 
