@@ -76,8 +76,8 @@ def parse():
                     help="fixed launch shape instead of the autotuner for the headline (MODE fp64|mixed); "
                          "for counter runs, where serialised dispatches distort the tuner's timings")
     ap.add_argument("--eager-only", action="store_true",
-                    help="headline: time eager launches only (no hipGraph or multi-stream runs), so a kernel trace "
-                         "ends with exactly the timed launches")
+                    help="headline: time eager launches only (no hipGraph, multi-stream or batched runs), so a "
+                         "kernel trace ends with exactly the timed launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -230,6 +230,25 @@ def selfenergy(D, qp, args):
         log(f"selfenergy {name}: eager {ms_eager * 1e3:.1f} us, graph {ms_graph * 1e3:.1f} us, "
             f"{N_STREAMS} streams {ms_streams * 1e3:.1f} us per call set, "
             f"err {err:.2e}, shape {shape}")
+    if args.eager_only:
+        return {"n": n, "evals": b.evals, "na": c["na"], "res": res, "ref_fn": ref_fn}
+    # BATCH call sets in one launch (the 64 integrals repeated): the kernel's throughput on C1's
+    # integrand without the fixed per-launch cost
+    nb = BATCH * n
+    bb = D.SEBatch(qp, np.tile(c["p"], BATCH), c["nq"], c["na"], [c["Z"]] * nb, *[[v] * nb for v in c["coord"]],
+                   x_extent=c["x_extent"], k=c["k"])
+    outb = D.DeviceArray(nb)
+    refb = np.tile(ref, BATCH)
+    for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
+        shape = bb.autotune(mode, reps=3)
+        ms, _ = time_steps(D, lambda: bb.run(outb, mode), max(1, args.steps // BATCH), max(1, args.warmup // BATCH))
+        fl, va = se_ops(mode, c["na"], shape["block"])
+        res[name][f"batched_x{BATCH}"] = {
+            "integrals_per_s": nb / (ms * 1e-3), "us_per_launch": ms * 1e3, "shape": shape,
+            "max_rel_err": rel_err(outb.to_host(), refb),
+            "roofline_frac": fl * bb.evals / (ms * 1e-3) / PEAK_FP64_VALU,
+            "valu_issue_frac": valu_issue_frac(va, bb.evals, ms * 1e-3)}
+        log(f"selfenergy {name} x{BATCH}: {ms * 1e3:.1f} us per launch, shape {shape}")
     return {"n": n, "evals": b.evals, "na": c["na"], "res": res, "ref_fn": ref_fn}
 
 
@@ -455,12 +474,17 @@ def main():
         "mixed": {"integrals_per_s": n / (mx["ms"] * 1e-3), "evals_per_s": evals / (mx["ms"] * 1e-3),
                   "max_rel_err_vs_fp64_oracle": mx["err"], "shape": mx["shape"], "launch": mx["launch"],
                   "us_eager": mx["ms_eager"] * 1e3, "us_graph": finite(mx["ms_graph"] * 1e3),
-                  f"integrals_per_s_{N_STREAMS}_streams": finite(n / (mx["ms_streams"] * 1e-3))},
+                  f"integrals_per_s_{N_STREAMS}_streams": finite(n / (mx["ms_streams"] * 1e-3)),
+                  f"batched_x{BATCH}": mx.get(f"batched_x{BATCH}")},
         "plain_fp64": {"integrals_per_s": n / (fp["ms"] * 1e-3), "evals_per_s": evals / (fp["ms"] * 1e-3),
                        "ms_per_step": fp["ms"], "max_rel_err_vs_oracle": fp["err"], "shape": fp["shape"],
                        "launch": fp["launch"], "us_eager": fp["ms_eager"] * 1e3, "us_graph": finite(fp["ms_graph"] * 1e3),
-                       f"integrals_per_s_{N_STREAMS}_streams": finite(n / (fp["ms_streams"] * 1e-3))},
+                       f"integrals_per_s_{N_STREAMS}_streams": finite(n / (fp["ms_streams"] * 1e-3)),
+                       f"batched_x{BATCH}": fp.get(f"batched_x{BATCH}")},
         "mixed_over_plain": fp["ms"] / mx["ms"],
+        f"mixed_over_plain_batched_x{BATCH}": (mx[f"batched_x{BATCH}"]["integrals_per_s"] /
+                                               fp[f"batched_x{BATCH}"]["integrals_per_s"]
+                                               if f"batched_x{BATCH}" in mx else None),
     }
     na = se["na"]
     fl1, va1 = se_ops(1, na, mx["shape"]["block"])
