@@ -11,8 +11,9 @@
 //
 // Device layout: every resident flow owns 4 n doubles in one HBM pool (u, RK accumulator, two
 // stage buffers) and a slot in SoA descriptor arrays.  A point table lists (slot, i) of every
-// resident grid point; one wave evaluates one point (one Gauss-Legendre node per lane, wave64 DPP
-// sums), so all resident flows advance together in each launch.  One RK step is four launches,
+// resident grid point; one wave evaluates one point at a time (one Gauss-Legendre node per lane,
+// wave64 DPP sums) over a run of consecutive points, so all resident flows advance together in
+// each launch.  One RK step is four launches,
 // one per stage; steps_per_launch steps are captured into one hipGraph.  A flow's step index is
 // the launch's base step (written by the host before each graph launch) plus the step within the
 // graph minus the base step at its admission, so flows admitted at different times share the
@@ -41,7 +42,7 @@ using namespace dfrg;
 namespace
 {
   constexpr int kFlowBlock = 256;
-  constexpr int kFlowGrid = 1024; // waves = kFlowGrid * kFlowBlock / 64, grid-stride over points
+  constexpr int kFlowGrid = 1024; // waves = kFlowGrid * kFlowBlock / 64, grid-stride over chunks
   constexpr double kPi = 3.14159265358979323846;
 
   struct FlowDev {
@@ -57,62 +58,97 @@ namespace
     double dt;
   };
 
-  // S2(E) = coth(E/2T)/(4E^3) + csch^2(E/2T)/(8 T E^2) with em1 = expm1(-E/T), e = 1 + em1,
-  // coth = (1+e)/(1-e), csch^2 = 4e/(1-e)^2, put over one denominator:
-  //   S2 = (2 T (1+e)(1-e) + 4 e E) / (8 T E^3 (1-e)^2)
-  __device__ __forceinline__ double bose_S2(double E, double T, double inv_T)
+  // S2(E) = coth(E/2T)/(4E^3) + csch^2(E/2T)/(8 T E^2) with e = exp(-E/T), coth = (1+e)/(1-e),
+  // csch^2 = 4e/(1-e)^2, put over one denominator:
+  //   S2 = (2 T (1+e)(1-e) + 4 e E) / (8 T E^3 (1-e)^2),   E = sqrt(E2).
+  // FP64-accurate seeded forms instead of library calls: y = 1/E from an FP32 rsqrt seed and one
+  // Newton step, E = E2 y; e from the 1/64-grid table and a degree-6 polynomial; the denominator as
+  // the product of 1/(8T), y^3 and the seeded reciprocal of (1-e), squared, so no factor leaves the
+  // FP32 range of the seeds.  1 - e loses relative accuracy only as E/T -> 0 (absolute error
+  // ~1e-16 against 1 - e ~ E/T).  Relative error of S2 ~1e-14.
+  __device__ __forceinline__ double bose_S2(double E2, double T, double inv_8T, double inv_T,
+                                            const double *exp2t)
   {
-    const double em1 = expm1(-E * inv_T);
-    const double e = 1.0 + em1, ome = -em1;
+    const double y = dev::rsqrt_seeded(E2);
+    const double E = E2 * y;
+    const double e = dev::exp_neg_tab(E * inv_T, exp2t), ome = 1.0 - e;
     const double num = fma(2.0 * T * (1.0 + e), ome, 4.0 * e * E);
-    return num / (8.0 * T * (E * E * E) * (ome * ome));
+    const double r = dev::rcp_seeded(ome);
+    return num * inv_8T * (y * y * y) * (r * r);
   }
 
   __global__ __launch_bounds__(kFlowBlock) void flow_stage_kernel(FlowDev F, int stage, int local)
   {
+    __shared__ double s_exp2[64]; // 2^(j/64)
+    for (int j = threadIdx.x; j < 64; j += kFlowBlock) s_exp2[j] = exp2(j * (1.0 / 64.0));
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave_id = (blockIdx.x * kFlowBlock + threadIdx.x) >> 6;
     const int n_waves = (gridDim.x * kFlowBlock) >> 6;
     const int P = *F.n_points;
     const int base = *F.base + local;
-    for (int w = wave_id; w < P; w += n_waves) {
-      const int2 pt = F.points[w];
-      const int s = pt.x, i = pt.y;
-      const int step = base - F.admit[s];
-      if (F.status[s] != 1 || step < 0 || step >= F.n_steps) continue; // wave-uniform
-      const int n = F.n[s];
-      double *u = F.pool + F.off[s];
-      double *acc = u + n, *y0 = u + 2 * n, *y1 = u + 3 * n;
-      const double *in = stage == 1 ? u : (stage == 3 ? y1 : y0);
-      const double h = F.h[s], T = F.T[s], Nn = F.N[s];
-      const double k = F.kt[3 * step + (stage == 1 ? 0 : (stage == 4 ? 2 : 1))];
-      double up, upp;
-      if (i == 0) {
-        up = (-3.0 * in[0] + 4.0 * in[1] - in[2]) / (2.0 * h);
-        upp = (in[0] - 2.0 * in[1] + in[2]) / (h * h);
-      } else if (i == n - 1) {
-        up = (3.0 * in[n - 1] - 4.0 * in[n - 2] + in[n - 3]) / (2.0 * h);
-        upp = (in[n - 1] - 2.0 * in[n - 2] + in[n - 3]) / (h * h);
-      } else {
-        up = (in[i + 1] - in[i - 1]) / (2.0 * h);
-        upp = (in[i + 1] - 2.0 * in[i] + in[i - 1]) / (h * h);
+    // A wave takes a chunk of consecutive points, sized so that all waves get work: one point each
+    // while points are fewer than waves, up to 64 when they are many.  Lane l gathers the data of
+    // point w0 + l (slot, neighbours in u, couplings) in one round of loads; the wave then
+    // evaluates the points' right-hand sides one after the other with the point's values
+    // broadcast from its lane (lanes over Gauss-Legendre nodes, wave64 DPP sums), and lane l
+    // applies the RK stage update of its point.
+    const int chunk = min(64, max(1, (P + n_waves - 1) / n_waves));
+    for (int w0 = wave_id * chunk; w0 < P; w0 += n_waves * chunk) {
+      const int cnt = min(chunk, P - w0);
+      int s = 0, i = 0, n = 0, active = 0;
+      double up = 0.0, upp = 0.0, rho = 0.0, m2pi = 0.0, m2sig = 0.0, T = 1.0, Nn = 0.0, k = 0.0;
+      if (lane < cnt) {
+        const int2 pt = F.points[w0 + lane];
+        s = pt.x;
+        i = pt.y;
+        const int step = base - F.admit[s];
+        active = F.status[s] == 1 && step >= 0 && step < F.n_steps;
+        if (active) {
+          n = F.n[s];
+          const double *u = F.pool + F.off[s];
+          const double *in = stage == 1 ? u : (stage == 3 ? u + 3 * n : u + 2 * n);
+          const double h = F.h[s];
+          T = F.T[s];
+          Nn = F.N[s];
+          k = F.kt[3 * step + (stage == 1 ? 0 : (stage == 4 ? 2 : 1))];
+          if (i == 0) {
+            up = (-3.0 * in[0] + 4.0 * in[1] - in[2]) / (2.0 * h);
+            upp = (in[0] - 2.0 * in[1] + in[2]) / (h * h);
+          } else if (i == n - 1) {
+            up = (3.0 * in[n - 1] - 4.0 * in[n - 2] + in[n - 3]) / (2.0 * h);
+            upp = (in[n - 1] - 2.0 * in[n - 2] + in[n - 3]) / (h * h);
+          } else {
+            up = (in[i + 1] - in[i - 1]) / (2.0 * h);
+            upp = (in[i + 1] - 2.0 * in[i] + in[i - 1]) / (h * h);
+          }
+          rho = i * h;
+          m2pi = in[i];
+          m2sig = in[i] + 2.0 * rho * up;
+        }
       }
-      const double rho = i * h;
-      const double m2pi = in[i], m2sig = in[i] + 2.0 * rho * up;
-      // two threshold integrals, lane j evaluates node j (d = 3 radial element)
-      const double k2 = k * k, inv_T = 1.0 / T;
-      double ipi = 0.0, isg = 0.0;
-      for (int j = lane; j < F.nx; j += 64) {
-        const double a = F.A[j], dd = F.D[j];
-        ipi = fma(dd, bose_S2(sqrt(fma(k2, a, m2pi)), T, inv_T), ipi);
-        isg = fma(dd, bose_S2(sqrt(fma(k2, a, m2sig)), T, inv_T), isg);
+      double f_lane = 0.0;
+      for (int p = 0; p < cnt; ++p) {
+        if (!__builtin_amdgcn_readlane(active, p)) continue; // wave-uniform
+        const double pT = dev::lane_value(T, p), pk = dev::lane_value(k, p);
+        const double pm2pi = dev::lane_value(m2pi, p), pm2sig = dev::lane_value(m2sig, p);
+        // two threshold integrals, lane j evaluates node j (d = 3 radial element)
+        const double k2 = pk * pk, inv_T = 1.0 / pT, inv_8T = 0.125 * inv_T;
+        double ipi = 0.0, isg = 0.0;
+        for (int j = lane; j < F.nx; j += 64) {
+          const double a = F.A[j], dd = F.D[j];
+          ipi = fma(dd, bose_S2(fma(k2, a, pm2pi), pT, inv_8T, inv_T, s_exp2), ipi);
+          isg = fma(dd, bose_S2(fma(k2, a, pm2sig), pT, inv_8T, inv_T, s_exp2), isg);
+        }
+        const double k5 = k2 * k2 * pk;
+        ipi = k5 * dev::wave_sum(ipi);
+        isg = k5 * dev::wave_sum(isg);
+        if (lane == p) f_lane = 0.5 * ((Nn - 1.0) * up * ipi + (3.0 * up + 2.0 * rho * upp) * isg);
       }
-      const double k5 = k2 * k2 * k;
-      ipi = k5 * dev::wave_sum(ipi);
-      isg = k5 * dev::wave_sum(isg);
-      if (lane == 0) {
-        const double f = 0.5 * ((Nn - 1.0) * up * ipi + (3.0 * up + 2.0 * rho * upp) * isg);
-        const double dt = F.dt;
+      if (lane < cnt && active) {
+        double *u = F.pool + F.off[s];
+        double *acc = u + n, *y0 = u + 2 * n, *y1 = u + 3 * n;
+        const double f = f_lane, dt = F.dt;
         if (stage == 1) {
           acc[i] = f;
           y0[i] = u[i] + 0.5 * dt * f;

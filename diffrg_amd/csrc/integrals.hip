@@ -294,23 +294,6 @@ namespace
     return dev::rcp_seeded(fma(Z, pq2d, R));
   }
 
-  // exp(-t), t >= 0, to FP64 accuracy: t on a 1/64 grid (2^(j/64) from the LDS table) and a degree-6
-  // polynomial of the remainder |r| <= ln2/128 (truncation below 1e-16 relative).
-  __device__ __forceinline__ double exp_neg_tab(double t, const double *exp2t)
-  {
-    const double t64 = fmax(-t * (64.0 * kLog2e), -64.0 * 1020.0);
-    const double tn = rint(t64);
-    const int jn = (int)tn;
-    const double r = (t64 - tn) * (kLn2 / 64.0);
-    double pe = fma(r, 1.0 / 720.0, 1.0 / 120.0);
-    pe = fma(pe, r, 1.0 / 24.0);
-    pe = fma(pe, r, 1.0 / 6.0);
-    pe = fma(pe, r, 0.5);
-    pe = fma(pe, r, 1.0);
-    pe = fma(pe, r, 1.0);
-    return ldexp(pe * exp2t[jn & 63], jn >> 6);
-  }
-
   // log1p(t) to FP64 accuracy from the FP32 hardware log2 and one Newton step on e^l = 1 + t,
   // with 1 + t carried exactly as w + err (TwoSum).  Quadratic convergence from the ~1e-7 seed
   // leaves ~1e-14 absolute.  t <= -1 gives NaN, which the callers' clamps map to index 0.
@@ -320,7 +303,7 @@ namespace
     const double bv = w - 1.0;
     const double err = (1.0 - (w - bv)) + (t - bv);
     const double l0 = (double)(__builtin_amdgcn_logf((float)w) * (float)kLn2);
-    const double e = exp_neg_tab(l0, exp2t);
+    const double e = dev::exp_neg_tab(l0, exp2t);
     return l0 + fma(w, e, -1.0) + err * e;
   }
 
@@ -450,7 +433,7 @@ namespace
           qt = el * qw.y * (2.0 * ex * k2 * (1.0 + xr)) * Gq * Gq;
         } else {
           const double xr = q2 * dev::rcp_seeded(k2);
-          const double ex = exp_neg_tab(xr, s_exp2);
+          const double ex = dev::exp_neg_tab(xr, s_exp2);
           double idx = log1p_seeded((q - T.c) * T.inv_b, s_exp2) * T.scale;
           idx = fmax(0.0, fmin(idx, (double)(T.ntab - 1)));
           const double fl = floor(idx);

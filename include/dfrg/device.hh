@@ -221,5 +221,32 @@ namespace dfrg
       return fma(r, fma(-d, r, 1.0), r);
     }
     __device__ __forceinline__ float fast_sqrtf(float v) { return __builtin_amdgcn_sqrtf(v); }
+
+    // exp(-t), t >= 0, to FP64 accuracy: t on a 1/64 grid (exp2t[j] = 2^(j/64), j < 64, a table the
+    // caller keeps in LDS) and a degree-6 polynomial of the remainder |r| <= ln2/128 (truncation
+    // below 1e-16 relative); t beyond 1020 ln2 gives 0 through the exponent.
+    __device__ __forceinline__ double exp_neg_tab(double t, const double *exp2t)
+    {
+      constexpr double kLog2e = 1.4426950408889634074, kLn2 = 0.69314718055994530942;
+      const double t64 = fmax(-t * (64.0 * kLog2e), -64.0 * 1020.0);
+      const double tn = rint(t64);
+      const int jn = (int)tn;
+      const double r = (t64 - tn) * (kLn2 / 64.0);
+      double pe = fma(r, 1.0 / 720.0, 1.0 / 120.0);
+      pe = fma(pe, r, 1.0 / 24.0);
+      pe = fma(pe, r, 1.0 / 6.0);
+      pe = fma(pe, r, 0.5);
+      pe = fma(pe, r, 1.0);
+      pe = fma(pe, r, 1.0);
+      return ldexp(pe * exp2t[jn & 63], jn >> 6);
+    }
+
+    // 1/sqrt(s) for s > 0 in the FP32 range: FP32 hardware seed and one FP64 Newton step on
+    // f(y) = 1/y^2 - s, y <- y + y (1 - s y^2)/2 (relative error about the square of the seed's).
+    __device__ __forceinline__ double rsqrt_seeded(double s)
+    {
+      const double y = (double)__builtin_amdgcn_rsqf((float)s);
+      return fma(0.5 * y, fma(-s * y, y, 1.0), y);
+    }
   } // namespace dev
 } // namespace dfrg
