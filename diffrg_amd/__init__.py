@@ -97,6 +97,11 @@ SIGNATURES = {
     "dfrg_fs_poll": [_vp, C.c_longlong, _ip],
     "dfrg_fs_result": [_vp, C.c_longlong, _dp, C.c_int, _dp],
     "dfrg_fs_stats": [_vp, _ip, _dp, _dp, _dp],
+    "dfrg_table_create": [C.POINTER(_vp), C.c_int, C.c_double, C.c_double, C.c_double, _dp],
+    "dfrg_table_update": [_vp, _dp, _vp],
+    "dfrg_table_view": [_vp, C.POINTER(_vp), _ip],
+    "dfrg_table_eval": [_vp, _vp, C.c_int, _vp, _vp],
+    "dfrg_table_destroy": [_vp],
 }
 
 
@@ -250,6 +255,43 @@ def matsubara_generate(T, E, step=2, min_size=0, max_size=1024, vacuum_quad_size
     check(lib().dfrg_matsubara_generate(T, E, step, min_size, max_size, vacuum_quad_size, precision_factor, x.ptr,
                                         w.ptr, cap, C.byref(n), C.byref(Tr), None))
     return x.to_host()[: n.value], w.to_host()[: n.value], Tr.value
+
+
+class Table:
+    """Interpolation table on logarithmic coordinates in HBM (dfrg_table_*; the data of a
+    LinearInterpolator1D<double, LogarithmicCoordinates1D<double>>)."""
+
+    def __init__(self, values, start: float, stop: float, bias: float):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        h = _vp()
+        check(lib().dfrg_table_create(C.byref(h), len(v), start, stop, bias, v.ctypes.data_as(_dp)))
+        self.ptr, self.n = h, len(v)
+
+    def update(self, values, stream=None) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if len(v) != self.n:
+            raise ValueError("update needs as many values as the table holds")
+        check(lib().dfrg_table_update(self.ptr, v.ctypes.data_as(_dp), stream))
+        check(lib().dfrg_stream_sync(stream))
+
+    def view(self) -> int:
+        d, n = _vp(), C.c_int()
+        check(lib().dfrg_table_view(self.ptr, C.byref(d), C.byref(n)))
+        return d.value
+
+    def eval(self, d_y: "DeviceArray", d_out: "DeviceArray", stream=None) -> None:
+        check(lib().dfrg_table_eval(self.ptr, d_y.ptr, d_y.n, d_out.ptr, stream))
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().dfrg_table_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class QuadratureProvider:

@@ -116,6 +116,24 @@ int dfrg_qp_matsubara(dfrg_qp *qp, double T, double E, int fp32, int *size, doub
                       const void **d_w, const void **h_x, const void **h_w);
 
 /* ------------------------------------------------------------------------------------------ */
+/* interpolation tables (LinearInterpolator1D<double, LogarithmicCoordinates1D<double>>,       */
+/* linear_interpolation_1D.hh:22-113, coordinates.hh:141-196)                                   */
+/* ------------------------------------------------------------------------------------------ */
+/* n FP64 values on y(i) = b expm1(bias i/(n-1)) + start, b = (stop-start)/expm1(bias), held in
+ * HBM.  Evaluation is the template's: idx = log1p((y-start)/b)(n-1)/bias clamped to [0, n-1]
+ * (fmax/fmin: a NaN index gives the last value), linear between floor and ceil.  Errors:
+ * n < 2, stop <= start or bias == 0 give DFRG_ERR_INVALID. */
+typedef struct dfrg_table dfrg_table;
+int dfrg_table_create(dfrg_table **t, int n, double start, double stop, double bias, const double *values);
+/* copy n new host values (the reference's update(); enqueued on `stream`) */
+int dfrg_table_update(dfrg_table *t, const double *values, void *stream);
+/* device pointer to the n values (a non-owning view, as the reference's copied interpolator) */
+int dfrg_table_view(dfrg_table *t, const double **d_values, int *n);
+/* d_out[i] = table(d_y[i]) for n_points device values */
+int dfrg_table_eval(dfrg_table *t, const double *d_y, int n_points, double *d_out, void *stream);
+int dfrg_table_destroy(dfrg_table *t);
+
+/* ------------------------------------------------------------------------------------------ */
 /* batched integrals with the built-in integrands                                             */
 /* ------------------------------------------------------------------------------------------ */
 /* LPA threshold integrals, radial d-dim form of IntegratorGPU<d> (integrator_gpu.hh:34-60):
@@ -129,8 +147,8 @@ int dfrg_threshold_batch(int d, const double *d_m2, int n_int, const double *d_x
  *   f = RBdot(q) G(q)^2 G(|p_i - q|),  G(y) = 1/(Z_i(y) y^2 + RB(k^2,y^2)),
  * each integral with its own radial order nq[i], angular order na[i] (both angles), external
  * momentum p[i] and dressing table Z_i (tables[tab_off[i] .. + tab_n[i]]) on
- * LogarithmicCoordinates1D(tab_n[i], start[i], stop[i], bias[i]).  Orders must be in
- * {4,8,16,32,48,64,96,128} (radial) and {4,8,16,32,64} (angular). Host arrays; copied to HBM once. */
+ * LogarithmicCoordinates1D(tab_n[i], start[i], stop[i], bias[i]).  Radial orders 1..1024, angular
+ * orders powers of two in [2, 64], tables of 2..256 values.  Host arrays; copied to HBM once. */
 typedef struct dfrg_se_batch dfrg_se_batch;
 int dfrg_se_batch_create(dfrg_se_batch **b, dfrg_qp *qp, int n_int, const double *p, const int *nq,
                          const int *na, const int *tab_off, const int *tab_n, const double *start,

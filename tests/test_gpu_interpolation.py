@@ -181,3 +181,38 @@ def test_interpolator_errors(kat):
     assert kat.kat_interp_errors(0) == -2 and b"non-owner" in kat.kat_last_error()
     assert kat.kat_interp_errors(1) == -1 and b"start must be smaller than stop" in kat.kat_last_error()
     assert kat.kat_interp_errors(2) == -2 and b"at least 2 points" in kat.kat_last_error()
+
+
+# ------------------------------------------------------------------ C-ABI tables (dfrg_table_*)
+
+def test_c_abi_table_matches_oracle(gpu):
+    from oracle import oracle as O
+    rng = np.random.default_rng(5)
+    n, start, stop, bias = 128, 1e-3, 40.0, 6.0
+    vals = np.cumsum(rng.uniform(0.0, 0.1, n)) + 1.0
+    t = gpu.Table(vals, start, stop, bias)
+    coord = O.logcoord(n, start, stop, bias)
+    y = np.concatenate([rng.uniform(0, 45, 500), [start, stop, 0.0, 50.0, -1.0]])
+    print("[case] dfrg_table_eval on 505 points", flush=True)
+    d_y, d_out = gpu.DeviceArray.from_host(y), gpu.DeviceArray(len(y))
+    t.eval(d_y, d_out)
+    got = d_out.to_host()
+    want = np.array([O.lerp_log(vals, coord, v) for v in y])
+    assert np.allclose(got, want, rtol=1e-13, atol=0)
+    # update() replaces the values the view points to
+    t.update(2.0 * vals)
+    t.eval(d_y, d_out)
+    assert np.allclose(d_out.to_host(), 2.0 * want, rtol=1e-13, atol=0)
+    assert t.view() != 0
+    t.close()
+
+
+def test_c_abi_table_errors(gpu):
+    L = gpu.lib()
+    h = C.c_void_p()
+    v = np.ones(4)
+    pv = v.ctypes.data_as(_dp)
+    assert L.dfrg_table_create(C.byref(h), 1, 0.0, 1.0, 1.0, pv) == gpu.ERR_INVALID
+    assert L.dfrg_table_create(C.byref(h), 4, 1.0, 1.0, 1.0, pv) == gpu.ERR_INVALID
+    assert L.dfrg_table_create(C.byref(h), 4, 0.0, 1.0, 0.0, pv) == gpu.ERR_INVALID
+    assert b"bias" in L.dfrg_last_error()
