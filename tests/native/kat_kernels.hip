@@ -88,6 +88,20 @@ namespace
       return complex<double>(c, 0.0);
     }
   };
+  // FP32 variant of PolyRadial (NT = ctype = float)
+  struct PolyRadialF {
+    static __forceinline__ __host__ __device__ float kernel(const float q, const float, const float, const float x0,
+                                                            const float x1, const float x2, const float x3,
+                                                            const float x4, const float x5)
+    {
+      return x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q);
+    }
+    static __forceinline__ __host__ __device__ float constant(const float, const float c, const float, const float,
+                                                              const float, const float, const float, const float)
+    {
+      return c;
+    }
+  };
   // integrator_4D_2ang_cpu.cc:14-36
   struct Poly2Ang {
     static __forceinline__ __host__ __device__ auto kernel(const double q, const double cos1, const double cos2,
@@ -892,6 +906,18 @@ int kat_x_extent(int which, double a, unsigned order, double tol, double *x_exte
     else
       f = [](double q2) { return 1.0 / powr<3>(1.0 + q2); };
     *x_extent = dfrg::tmpl::optimize_x_extent(provider(), f, order, tol);
+  });
+}
+
+// NT = float: the radial integrator on FP32 nodes (get and a batched request)
+int kat_radial_float(int nx, float x_extent, float k, const double *c, float constant, double *out_get,
+                     double *out_req)
+{
+  return guard([&] {
+    IntegratorGPU<4, float, PolyRadialF> integ(provider(), nx, x_extent);
+    const float cf[6] = {(float)c[0], (float)c[1], (float)c[2], (float)c[3], (float)c[4], (float)c[5]};
+    *out_get = integ.get(k, constant, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5]);
+    *out_req = integ.request(k, constant, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5]).get();
   });
 }
 

@@ -100,6 +100,18 @@ def test_radial_complex_and_requests(kat, qp):
         assert out[i] == pytest.approx(O.kat_radial(4, x, w, 1.5, k, c, 0.0), rel=TOL)
 
 
+def test_radial_template_float(kat, qp):
+    # NT = ctype = float: FP32 nodes and arithmetic, against the FP64 oracle on the same order
+    x, w = qp.get_points(64), qp.get_weights(64)
+    c = arr([0.3, -0.2, 0.5, 0.1, -0.4, 0.2])
+    g, r = D(), D()
+    print("[case] radial float", flush=True)
+    call(kat, "kat_radial_float", 64, C.c_float(1.5), C.c_float(0.7), P(c), C.c_float(0.25), C.byref(g), C.byref(r))
+    ref = 0.25 + O.kat_radial(4, x, w, 1.5, 0.7, c, 0.0)
+    assert g.value == pytest.approx(ref, rel=2e-6)
+    assert r.value == g.value
+
+
 def test_two_angle_template(kat, qp):
     rng = np.random.default_rng(7)
     xq, wq = qp.get_points(64), qp.get_weights(64)
