@@ -213,8 +213,7 @@ namespace
   };
 
   struct SECellZ {
-    double z;
-    float dz, pad;
+    double z, dz; // Z_i and Z_{i+1} - Z_i
   };
 
   // mixed-path cell data of node i: y_i^2 (FP64), 1/(b + y_i - c) and y_i (FP32)
@@ -284,7 +283,7 @@ namespace
     const float ru = u - (w1 - 1.0f); // rounding error of 1 + u
     float fr = fmaf(__builtin_amdgcn_logf(w1), m.scl, ru * m.sf);
     fr = __builtin_amdgcn_fmed3f(fr, 0.0f, 1.0f); // realises the [0, n-1] clamp
-    const double Z = cz.z + (double)(fr * cz.dz);
+    const double Z = fma((double)fr, cz.dz, cz.z);
     const double t64 = pq2d * m.ek64; // 64 t, t = -|p-q|^2/k^2 log2(e)
     const double tn = rint(t64);
     const int jn = (int)tn;
@@ -788,7 +787,7 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
       const double v = tables[tab_off[i] + j];
       const double nv = (j + 1 < n) ? tables[tab_off[i] + j + 1] : v;
       tabA[tab_off[i] + j] = SECellY{yj * yj, (float)(1.0 / (bb + yj - cc)), (float)yj};
-      tabB[tab_off[i] + j] = SECellZ{v, (float)(nv - v), 0.0f};
+      tabB[tab_off[i] + j] = SECellZ{v, nv - v};
     }
   }
   auto fail = [&](int rc) {
