@@ -42,7 +42,7 @@ GRAPH_SETS = 10  # call sets per hipGraph launch
 # (na^2 >= 2 BLOCK), "single" when it owns one pair and two radial nodes per iteration.
 # tests/test_isa_counts.py checks these figures against the current source.
 SE_OPS = {0: {"pairs": (223.5, 203.0), "single": (223.5, 205.0)},
-          1: {"pairs": (47.0, 44.5), "single": (47.0, 46.5)}}
+          1: {"pairs": (46.0, 44.0), "single": (46.0, 46.0)}}
 SE_OPS_BLOCK = {}  # (mode, block, loop) whose schedule differs from the others
 
 

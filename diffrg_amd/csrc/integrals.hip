@@ -287,8 +287,10 @@ namespace
     const double t64 = pq2d * m.ek64; // 64 t, t = -|p-q|^2/k^2 log2(e)
     const double tn = rint(t64);
     const int jn = (int)tn;
-    const double r = (t64 - tn) * (kLn2 / 64.0);
-    const double pe = fma(fma(fma(r, 1.0 / 6.0, 0.5), r, 1.0), r, 1.0);
+    // 2^(f/64) = exp(f ln2/64) for f = 64t - jn in [-1/2, 1/2], as a cubic in f
+    constexpr double c1 = kLn2 / 64.0, c2 = c1 * c1 / 2.0, c3 = c1 * c1 * c1 / 6.0;
+    const double f = t64 - tn;
+    const double pe = fma(fma(fma(f, c3, c2), f, c1), f, 1.0);
     const double R = ldexp(pe * m.exp2t[jn & 63], jn >> 6);
     return dev::rcp_seeded(fma(Z, pq2d, R));
   }
