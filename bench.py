@@ -41,8 +41,11 @@ GRAPH_SETS = 10  # call sets per hipGraph launch
 # `python -m diffrg_amd.isa_count`: "pairs" when a lane owns two angular pairs
 # (na^2 >= 2 BLOCK), "single" when it owns one pair and two radial nodes per iteration.
 # tests/test_isa_counts.py checks these figures against the current source.
+# Keyed by the integrand form (SEBatch.variants): 0 FP64, 1 mixed, 2 mixed on an integral that
+# admits the narrow exponent rounding.
 SE_OPS = {0: {"pairs": (223.5, 203.0), "single": (223.5, 205.0)},
-          1: {"pairs": (46.0, 44.0), "single": (46.0, 46.0)}}
+          1: {"pairs": (46.0, 44.0), "single": (46.0, 46.0)},
+          2: {"pairs": (46.0, 42.0), "single": (46.0, 44.0)}}
 SE_OPS_BLOCK = {}  # (mode, block, loop) whose schedule differs from the others
 
 
@@ -50,10 +53,18 @@ def se_loop(na, block):
     return "pairs" if 2 * min(na * na, block) <= na * na else "single"
 
 
-def se_ops(mode, na, block):
-    """(FLOP, VALU) per evaluation of the loop se_kernel<mode, block> runs for na angular nodes."""
+def se_ops(variant, na, block):
+    """(FLOP, VALU) per evaluation of the loop se_kernel<., block> runs with integrand form
+    `variant` for na angular nodes."""
     loop = se_loop(na, block)
-    return SE_OPS_BLOCK.get((mode, block, loop), SE_OPS[mode][loop])
+    return SE_OPS_BLOCK.get((variant, block, loop), SE_OPS[variant][loop])
+
+
+def se_variant(b, mode):
+    """The integrand form of a batch whose integrals all use the same one."""
+    v = np.unique(b.variants(mode))
+    assert len(v) == 1, v
+    return int(v[0])
 
 
 def valu_issue_frac(valu_per_eval, evals, seconds):
@@ -231,7 +242,8 @@ def selfenergy(D, qp, args):
             f"{N_STREAMS} streams {ms_streams * 1e3:.1f} us per call set, "
             f"err {err:.2e}, shape {shape}")
     if args.eager_only:
-        return {"n": n, "evals": b.evals, "na": c["na"], "res": res, "ref_fn": ref_fn}
+        return {"n": n, "evals": b.evals, "na": c["na"], "res": res, "ref_fn": ref_fn,
+                "variant": {D.FP64: se_variant(b, D.FP64), D.MIXED: se_variant(b, D.MIXED)}}
     # BATCH call sets in one launch (the 64 integrals repeated): the kernel's throughput on C1's
     # integrand without the fixed per-launch cost
     nb = BATCH * n
@@ -242,14 +254,15 @@ def selfenergy(D, qp, args):
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
         shape = bb.autotune(mode, reps=3)
         ms, _ = time_steps(D, lambda: bb.run(outb, mode), max(1, args.steps // BATCH), max(1, args.warmup // BATCH))
-        fl, va = se_ops(mode, c["na"], shape["block"])
+        fl, va = se_ops(se_variant(bb, mode), c["na"], shape["block"])
         res[name][f"batched_x{BATCH}"] = {
             "integrals_per_s": nb / (ms * 1e-3), "us_per_launch": ms * 1e3, "shape": shape,
             "max_rel_err": rel_err(outb.to_host(), refb),
             "roofline_frac": fl * bb.evals / (ms * 1e-3) / PEAK_FP64_VALU,
             "valu_issue_frac": valu_issue_frac(va, bb.evals, ms * 1e-3)}
         log(f"selfenergy {name} x{BATCH}: {ms * 1e3:.1f} us per launch, shape {shape}")
-    return {"n": n, "evals": b.evals, "na": c["na"], "res": res, "ref_fn": ref_fn}
+    return {"n": n, "evals": b.evals, "na": c["na"], "res": res, "ref_fn": ref_fn,
+            "variant": {D.FP64: se_variant(b, D.FP64), D.MIXED: se_variant(b, D.MIXED)}}
 
 
 def threshold(D, qp, args):
@@ -374,8 +387,9 @@ def uneven(D, qp, args):
         ms /= per_graph
         r[name] = {"integrals_per_s": n / (ms * 1e-3), "evals_per_s": b.evals / (ms * 1e-3), "ms_per_batch": ms,
                    "max_rel_err": rel_err(out.to_host(), ref), "shape": shape,
-                   "valu_issue_frac": sum(int(q) * int(a) ** 2 * se_ops(mode, int(a), shape["block"])[1]
-                                          for q, a in zip(c["nq"], c["na"])) / 64 / (ms * 1e-3)
+                   "valu_issue_frac": sum(int(q) * int(a) ** 2 * se_ops(int(v), int(a), shape["block"])[1]
+                                          for q, a, v in zip(c["nq"], c["na"], b.variants(mode)))
+                                      / 64 / (ms * 1e-3)
                                       / PEAK_VALU_WAVE_INSTR}
         log(f"uneven {name}: {ms:.3f} ms/batch, err {r[name]['max_rel_err']:.2e}, shape {shape}")
     r["mixed_over_plain"] = r["mixed"]["integrals_per_s"] / r["fp64"]["integrals_per_s"]
@@ -487,12 +501,13 @@ def main():
                                                if f"batched_x{BATCH}" in mx else None),
     }
     na = se["na"]
-    fl1, va1 = se_ops(1, na, mx["shape"]["block"])
-    fl0, va0 = se_ops(0, na, fp["shape"]["block"])
+    v1 = se["variant"][1]
+    fl1, va1 = se_ops(v1, na, mx["shape"]["block"])
+    fl0, va0 = se_ops(se["variant"][0], na, fp["shape"]["block"])
     achieved = fl1 * evals / (mx["ms"] * 1e-3) / 1e12
     peak = PEAK_FP64_VALU / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                       "frac": achieved / peak, "traffic": measured_traffic(), "kernel": "se_kernel<MIXED>",
+                       "frac": achieved / peak, "traffic": measured_traffic(), "kernel": f"se_kernel<{v1}>",
                        "flop_per_eval": fl1, "valu_per_eval": va1, "loop": se_loop(na, mx["shape"]["block"]),
                        "evals_per_launch": evals,
                        "valu_issue_frac": valu_issue_frac(va1, evals, mx["ms"] * 1e-3),

@@ -87,6 +87,7 @@ SIGNATURES = {
     "dfrg_se_batch_autotune": [_vp, C.c_int, C.c_int, C.c_char_p, _ip, _ip, _ip, C.POINTER(C.c_float)],
     "dfrg_se_batch_graph_launch": [_vp, C.c_int, _vp, C.c_int, _vp],
     "dfrg_se_batch_evals": [_vp, C.POINTER(C.c_ulonglong)],
+    "dfrg_se_batch_variants": [_vp, C.c_int, _ip],
     "dfrg_ft_set_block": [C.c_int],
     "dfrg_ft_batch": [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
                       C.c_double, C.c_double, C.c_int, _vp, _vp],
@@ -399,6 +400,12 @@ class SEBatch:
         e = C.c_ulonglong()
         check(lib().dfrg_se_batch_evals(self.ptr, C.byref(e)))
         return e.value
+
+    def variants(self, mode: int) -> np.ndarray:
+        """Integrand form a run in `mode` uses per integral (0 FP64, 1 mixed, 2 mixed narrow)."""
+        v = np.empty(self.n, dtype=np.int32)
+        check(lib().dfrg_se_batch_variants(self.ptr, mode, v.ctypes.data_as(C.POINTER(C.c_int))))
+        return v
 
     def set_shape(self, block=256, tile_evals=8192, grid=0):
         check(lib().dfrg_se_batch_set_shape(self.ptr, block, tile_evals, grid))

@@ -209,7 +209,7 @@ namespace
     int integral, q0, nqt, slot;
     int na, l2na, xq_off, xa_off; // node offsets: x at off, w at off + order
     int nq, tab_off, ntab, first_slot;
-    int n_slots, pad0, pad1, pad2;
+    int n_slots, narrow, pad1, pad2; // narrow: the mixed path may use se_G_mixed<true>
   };
 
   struct SECellZ {
@@ -269,7 +269,10 @@ namespace
   // below -2^25/64 underflows to 0 through the exponent);
   // the denominator in FP64; the reciprocal from an FP32 seed refined in FP64.  Per-evaluation
   // error ~1e-9 relative.
-  __device__ __forceinline__ double se_G_mixed(const SEMixed &m, double pq2d)
+  // NARROW (chosen per integral when its table nodes are >= 1e-30 and |64 t| < 2^30 at all its
+  // quadrature points): y + y_i needs no guard, and 64 t is rounded to an integer by adding
+  // 1.5 2^52 (the sum's low word is the integer), one FMA instead of round-and-convert.
+  template <bool NARROW> __device__ __forceinline__ double se_G_mixed(const SEMixed &m, double pq2d)
   {
     const float ys = __builtin_amdgcn_sqrtf((float)pq2d);
     // cell index: clamp to [0, n-2] first (med3 of a NaN gives 0), then truncation is floor
@@ -278,18 +281,28 @@ namespace
     const SECellY ca = m.tabA[i0];
     const SECellZ cz = m.tabB[i0];
     const float num = (float)(pq2d - ca.y2);
-    const float u = num * (ca.inv_span * __builtin_amdgcn_rcpf(fmaxf(ys + ca.y, 1e-30f)));
+    const float u = num * (ca.inv_span * __builtin_amdgcn_rcpf(NARROW ? ys + ca.y : fmaxf(ys + ca.y, 1e-30f)));
     const float w1 = 1.0f + u;
     const float ru = u - (w1 - 1.0f); // rounding error of 1 + u
     float fr = fmaf(__builtin_amdgcn_logf(w1), m.scl, ru * m.sf);
     fr = __builtin_amdgcn_fmed3f(fr, 0.0f, 1.0f); // realises the [0, n-1] clamp
     const double Z = fma((double)fr, cz.dz, cz.z);
-    const double t64 = pq2d * m.ek64; // 64 t, t = -|p-q|^2/k^2 log2(e)
-    const double tn = rint(t64);
-    const int jn = (int)tn;
-    // 2^(f/64) = exp(f ln2/64) for f = 64t - jn in [-1/2, 1/2], as a cubic in f
+    // 64 t = pq2 ek64, t = -|p-q|^2/k^2 log2(e); jn = round(64 t), f = 64 t - jn in [-1/2, 1/2]
+    int jn;
+    double f;
+    if (NARROW) {
+      constexpr double kRound = 6755399441055744.0; // 1.5 * 2^52
+      const double sh = fma(pq2d, m.ek64, kRound);
+      jn = (int)(unsigned)__double_as_longlong(sh);
+      f = fma(pq2d, m.ek64, -(sh - kRound));
+    } else {
+      const double t64 = pq2d * m.ek64;
+      const double tn = rint(t64);
+      jn = (int)tn;
+      f = t64 - tn;
+    }
+    // 2^(f/64) = exp(f ln2/64) as a cubic in f
     constexpr double c1 = kLn2 / 64.0, c2 = c1 * c1 / 2.0, c3 = c1 * c1 * c1 / 6.0;
-    const double f = t64 - tn;
     const double pe = fma(fma(fma(f, c3, c2), f, c1), f, 1.0);
     const double R = ldexp(pe * m.exp2t[jn & 63], jn >> 6);
     return dev::rcp_seeded(fma(Z, pq2d, R));
@@ -464,8 +477,13 @@ namespace
         m.sf = (float)T.scale;
         m.ek64 = -64.0 * kLog2e / k2;
         m.imaxf = (float)(T.ntab - 2);
-        const auto g = [&](double pq2) { return se_G_mixed(m, pq2); };
-        acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
+        if (T.narrow) { // tile-uniform
+          const auto g = [&](double pq2) { return se_G_mixed<true>(m, pq2); };
+          acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
+        } else {
+          const auto g = [&](double pq2) { return se_G_mixed<false>(m, pq2); };
+          acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
+        }
       }
       const double part = dev::block_sum(acc, s_red);
       double total = 0.0;
@@ -530,6 +548,7 @@ struct dfrg_se_batch {
   size_t tiles_cap = 0;
   // shape
   int block = 256, tile_evals = 8192, grid = 0, n_tiles = 0;
+  std::vector<int> narrow; // per integral: the mixed path may use se_G_mixed<true>
   std::string signature;
   // graph
   hipGraph_t graph = nullptr;
@@ -577,6 +596,7 @@ namespace
         t.ntab = b->tab_n[i];
         t.first_slot = first;
         t.n_slots = cnt;
+        t.narrow = b->narrow[i];
         tiles.push_back(t);
       }
     }
@@ -777,6 +797,17 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
     b->coord[3 * i + 2] = (tab_n[i] - 1.0) / bias[i];
     b->evals += (unsigned long long)nq[i] * na[i] * na[i];
   }
+  // the narrow mixed form, per integral (so a result does not depend on the batch around it):
+  // every table node is at least 1e-30 (the nodes increase from start, and FP32 rounding keeps
+  // them >= 1e-30f, the guard's value), and |64 t| <= (|p| + q_max)^2 64 log2(e)/k^2 < 2^30
+  {
+    const double q_max = k * std::sqrt(x_extent);
+    b->narrow.resize(n_int);
+    for (int i = 0; i < n_int; ++i) {
+      const double pq = std::fabs(p[i]) + q_max;
+      b->narrow[i] = start[i] >= 1e-30 && pq * pq * 64.0 * kLog2e / (k * k) < 1073741824.0;
+    }
+  }
   // cell data of the mixed path (coordinates.hh:141-196 LogarithmicCoordinates1D): node
   // y_i = b expm1(a i/(n-1)) + c, 1/(b + y_i - c), Z_i and Z_{i+1} - Z_i
   std::vector<SECellY> tabA(tables_len);
@@ -882,6 +913,15 @@ int dfrg_se_batch_evals(dfrg_se_batch *b, unsigned long long *evals)
 {
   DFRG_CHECK_ARG(b && evals, "NULL argument");
   *evals = b->evals;
+  return DFRG_OK;
+}
+
+int dfrg_se_batch_variants(dfrg_se_batch *b, int mode, int *variant)
+{
+  DFRG_CHECK_ARG(b && variant, "NULL argument");
+  DFRG_CHECK_ARG(mode == DFRG_FP64 || mode == DFRG_MIXED, "mode must be DFRG_FP64 or DFRG_MIXED");
+  for (int i = 0; i < b->n_int; ++i)
+    variant[i] = mode == DFRG_FP64 ? 0 : b->narrow[i] ? 2 : 1;
   return DFRG_OK;
 }
 

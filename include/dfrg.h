@@ -171,6 +171,11 @@ int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cac
 int dfrg_se_batch_graph_launch(dfrg_se_batch *b, int mode, double *d_out, int launches, void *stream);
 /* Evaluations (integrand calls) performed by one run of the batch. */
 int dfrg_se_batch_evals(dfrg_se_batch *b, unsigned long long *evals);
+/* Integrand form a run in `mode` uses for each integral (variant[0 .. n_int-1]): 0 FP64; 1 mixed;
+ * 2 mixed for an integral whose table starts at >= 1e-30 and whose |p - q|^2 64 log2(e)/k^2
+ * stays below 2^30, which rounds the exponent with one FMA and drops a guard (within the mixed
+ * tolerance of form 1).  The choice depends on the integral alone, not on the batch. */
+int dfrg_se_batch_variants(dfrg_se_batch *b, int mode, int *variant);
 
 /* Finite-T two-propagator batch (IntegratorFiniteTq0GPU form, integrator_finiteTq0_gpu.hh:20-44):
  *   I_i = sum_{j,n} el(q_j) wq_j [ wm_n (f(q_j,wn) + f(q_j,-wn)) ] + T_i sum_j el(q_j) wq_j f(q_j,0),

@@ -254,6 +254,48 @@ def test_uneven_alone_equals_in_mix(gpu, qp):
             assert o1.to_host()[0] == mix[i]  # bitwise
 
 
+def test_mixed_forms_in_one_batch(gpu, qp):
+    # The mixed path has a narrow integrand form (one-FMA exponent rounding, no guard on the table
+    # sum), chosen per integral: tables starting at >= 1e-30 and |p - q|^2 64 log2(e)/k^2 < 2^30.
+    # Integrals with start 0 or p = 1e5 take the general form; both meet the mixed tolerance, and
+    # each integral's result is the same alone as in the batch.
+    c = uneven_subset(48)
+    n = len(c["p"])
+    start = np.full(n, c["coord"][0])
+    start[1::3] = 0.0
+    p = np.array(c["p"], dtype=float)
+    p[2::6] = 1e5
+    stop, bias = c["coord"][1], c["coord"][2]
+    xq = [(qp.get_points(int(v)), qp.get_weights(int(v))) for v in c["nq"]]
+    xa = [(qp.get_points(int(v)), qp.get_weights(int(v))) for v in c["na"]]
+    coords = [O.logcoord(len(t), s0, stop, bias) for t, s0 in zip(c["tables"], start)]
+    ref = O.c1_selfenergy(p, c["tables"], coords, xq, xa, c["x_extent"], c["k"])
+
+    def batch(idx):
+        return gpu.SEBatch(qp, p[idx], c["nq"][idx], c["na"][idx], [c["tables"][i] for i in idx], start[idx],
+                           [stop] * len(idx), [bias] * len(idx), x_extent=c["x_extent"], k=c["k"])
+
+    b = batch(np.arange(n))
+    v = b.variants(gpu.MIXED)
+    narrow = (start >= 1e-30) & ((np.abs(p) + c["k"] * np.sqrt(c["x_extent"])) ** 2 * 64 / np.log(2)
+                                 / c["k"] ** 2 < 2.0 ** 30)
+    assert np.array_equal(v, np.where(narrow, 2, 1)) and 0 < narrow.sum() < n
+    assert np.array_equal(b.variants(gpu.FP64), np.zeros(n))
+    out = gpu.DeviceArray(n)
+    for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+        print(f"[case] two mixed forms, 48 integrals, mode={mode}", flush=True)
+        b.run(out, mode)
+        mix = out.to_host()
+        err = rel_err(mix, ref)
+        print(f"  max rel err narrow {err[narrow].max():.3e} general {err[~narrow].max():.3e}", flush=True)
+        assert err.max() <= tol
+        for i in (0, 1, 2):  # narrow, start 0, large p
+            b1 = batch(np.array([i]))
+            o1 = gpu.DeviceArray(1)
+            b1.run(o1, mode)
+            assert o1.to_host()[0] == mix[i]  # bitwise
+
+
 def test_uneven_graph_replay(gpu, qp):
     c = uneven_subset(128)
     b = make_uneven(gpu, qp, c)

@@ -19,14 +19,17 @@ def test_se_kernel_counts_match_bench():
     for mode in (0, 1):
         for block in (64, 128, 256, 512, 1024):
             name = f"se_kernelILi{mode}ELi{block}E"
-            # the two integrand loops: the ones with transcendental instructions and more than one
-            # table lookup; the paired loop reads the radial terms once for two evaluations
-            body = sorted((c for n, c in loops if name in n and c["trans"] >= 8), key=lambda c: c["lds"])
-            assert len(body) == 2, (name, body)
-            for loop, c in zip(("pairs", "single"), body):
+            # the integrand loops: the ones with transcendental instructions and more than one
+            # table lookup; the paired loop reads the radial terms once for two evaluations.  The
+            # mixed kernel holds both integrand forms; the narrow one has fewer FP64 instructions.
+            body = sorted((c for n, c in loops if name in n and c["trans"] >= 8), key=lambda c: (c["f64"], c["lds"]))
+            variants = (0,) if mode == 0 else (2, 1)
+            assert len(body) == 2 * len(variants), (name, body)
+            for j, c in enumerate(body):
+                variant, loop = variants[j // 2], ("pairs", "single")[j % 2]
                 na = 64 if loop == "pairs" else 8
                 assert bench.se_loop(na, block) == loop
-                assert bench.se_ops(mode, na, block) == (c["flop"] / 2, c["valu"] / 2), (name, loop, c)
+                assert bench.se_ops(variant, na, block) == (c["flop"] / 2, c["valu"] / 2), (name, variant, loop, c)
 
 
 def test_se_loop_choice():
