@@ -2,7 +2,11 @@
 // (reference DiFfRG/include/DiFfRG/common/types.hh:130-167).
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include <DiFfRG/common/complex_math.hh>
+#include <DiFfRG/common/dual_traits.hh>
 
 namespace DiFfRG
 {
@@ -14,7 +18,13 @@ namespace DiFfRG
   {
     namespace internal
     {
-      template <typename CT> struct _ctype;
+      // a dual number (DiFfRG/common/dual_traits.hh): the type of its value, as for
+      // autodiff::Real<1, T> in the reference (types.hh:89-96)
+      template <typename CT> struct _ctype {
+        static_assert(DiFfRG::internal::DualTraits<CT>::is_dual, "ctype: unsupported number type");
+        using value = std::remove_cv_t<
+            std::remove_reference_t<decltype(DiFfRG::internal::DualTraits<CT>::value(std::declval<CT>()))>>;
+      };
       template <> struct _ctype<float> { using value = float; };
       template <> struct _ctype<double> { using value = double; };
       template <> struct _ctype<complex<float>> { using value = float; };

@@ -1,6 +1,7 @@
 // DiFfRG/physics/interpolation/common.hh -- shared pieces of the interpolators: the return type of
-// the FP32 ("Tex") tables (reference interpolation/common.hh:8-25) and the host + HBM table
-// storage behind every interpolator.
+// the FP32 ("Tex") tables (reference interpolation/common.hh:8-25), their derivative channel for
+// dual numbers (DiFfRG/common/dual_traits.hh), and the host + HBM table storage behind every
+// interpolator.
 //
 // Copies of an interpolator share one storage and are non-owning views: the copy an integrator
 // passes by value into a kernel reads the HBM table, and update() on a view throws, as in the
@@ -9,19 +10,25 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 
+#include <DiFfRG/common/dual_traits.hh>
 #include <DiFfRG/common/hip_prefix.hh>
 
 namespace DiFfRG
 {
   namespace internal
   {
-    // value type of the FP32 tables; autodiff values are not available in this build
-    template <typename NT> struct __TLITypes;
+    // value type of the FP32 tables: float for float and double data, the dual type itself for
+    // dual data (reference interpolation/common.hh:10-24)
+    template <typename NT> struct __TLITypes {
+      static_assert(DualTraits<NT>::is_dual, "FP32 table interpolators hold float, double or dual-number data");
+      using ReturnType = NT;
+    };
     template <> struct __TLITypes<float> {
       using ReturnType = float;
     };
@@ -103,6 +110,57 @@ namespace DiFfRG
              at(x.i1, y.i1, z.i0) * x.w1 * y.w1 * z.w0 + at(x.i1, y.i1, z.i1) * x.w1 * y.w1 * z.w1;
     }
 
+    // The FP32 tables of a Tex interpolator over n points: values, and derivatives when NT is a
+    // dual number (zero until data with derivatives is given, as in the reference).
+    template <typename NT> class TexTables
+    {
+    public:
+      static constexpr bool dual = DualTraits<NT>::is_dual;
+      explicit TexTables(unsigned n)
+          : n(n), val(std::make_shared<TableStorage<float>>(n)),
+            der(dual ? std::make_shared<TableStorage<float>>(n) : nullptr)
+      {
+        upload();
+      }
+      // values (and derivatives, when both NT and NT2 carry them) from n entries of data
+      template <typename NT2> void fill(const NT2 *data) const
+      {
+        float *v = val->host_ptr(), *dv = der ? der->host_ptr() : nullptr;
+        for (unsigned i = 0; i < n; ++i) {
+          if constexpr (DualTraits<NT2>::is_dual) {
+            v[i] = static_cast<float>(DualTraits<NT2>::value(data[i]));
+            if (dv) dv[i] = static_cast<float>(DualTraits<NT2>::derivative(data[i]));
+          } else
+            v[i] = static_cast<float>(data[i]);
+        }
+      }
+      void upload() const
+      {
+        val->upload();
+        if (der) der->upload();
+      }
+      float *host() const { return val->host_ptr(); }
+      float *host_AD() const { return der ? der->host_ptr() : nullptr; }
+      const float *device() const { return val->device_ptr(); }
+      const float *device_AD() const { return der ? der->device_ptr() : nullptr; }
+
+    private:
+      unsigned n;
+      std::shared_ptr<TableStorage<float>> val, der;
+    };
+
+    // One interpolated value of a Tex interpolator: lerp(values), or for dual data
+    // make(lerp(values), lerp(derivatives)).  lerp maps a table pointer to the FP32 result.
+    template <typename NT, typename L>
+    __forceinline__ __host__ __device__ typename __TLITypes<NT>::ReturnType tex_value(const float *v, const float *dv,
+                                                                                       const L &lerp)
+    {
+      if constexpr (DualTraits<NT>::is_dual)
+        return DualTraits<NT>::make(double(lerp(v)), double(lerp(dv)));
+      else
+        return lerp(v);
+    }
+
     inline void require_axes(const unsigned *n, int dim, const char *who)
     {
       for (int i = 0; i < dim; ++i)
@@ -110,3 +168,4 @@ namespace DiFfRG
     }
   } // namespace internal
 } // namespace DiFfRG
+

@@ -3,7 +3,8 @@
 // linear interpolation, the reference's texture-memory interpolator.  The reference filters with
 // NVIDIA texture hardware (weights quantised to 1/256); here the table is FP32 in HBM and the lerp
 // is computed in FP32 on the device and on the host alike, which is the reference's host formula.
-// Values are returned as float (internal::__TLITypes).
+// Values are returned as float (internal::__TLITypes); dual-number data (autodiff::real) keeps a
+// second FP32 table for the derivative and returns both channels.
 #pragma once
 
 #include <vector>
@@ -31,52 +32,68 @@ namespace DiFfRG
       update(data);
     }
     TexLinearInterpolator1D(const Coordinates &coordinates)
-        : size(coordinates.size()), coordinates(coordinates),
-          table(std::make_shared<internal::TableStorage<float>>(coordinates.size())), owner(true)
+        : size(coordinates.size()), coordinates(coordinates), owner(true)
     {
       if (size < 2) throw std::runtime_error("TexLinearInterpolator1D: the table needs at least 2 points");
-      d_table = table->device_ptr();
-      h_table = table->host_ptr();
-      table->upload();
+      tables = std::make_shared<internal::TexTables<NT>>(size);
+      d_table = tables->device();
+      d_table_AD = tables->device_AD();
+      h_table = tables->host();
+      h_table_AD = tables->host_AD();
     }
     TexLinearInterpolator1D(const TexLinearInterpolator1D &o)
-        : size(o.size), coordinates(o.coordinates), table(o.table), d_table(o.d_table), h_table(o.h_table),
-          owner(false)
+        : size(o.size), coordinates(o.coordinates), tables(o.tables), d_table(o.d_table), d_table_AD(o.d_table_AD),
+          h_table(o.h_table), h_table_AD(o.h_table_AD), owner(false)
     {
     }
 
     template <typename NT2> void update(const NT2 *data)
     {
       if (!owner) throw std::runtime_error("Cannot update data of non-owner interpolator");
-      for (unsigned i = 0; i < size; ++i) h_table[i] = static_cast<float>(data[i]);
+      tables->fill(data);
       update();
     }
     void update()
     {
       if (!owner) throw std::runtime_error("Cannot update data of non-owner interpolator");
-      table->upload();
+      tables->upload();
     }
     float *data() const { return h_table; }
-    ReturnType &operator[](const unsigned i) { return h_table[i]; }
-    const ReturnType &operator[](const unsigned i) const { return h_table[i]; }
+    float *data_AD() const { return h_table_AD; }
+    ReturnType &operator[](const unsigned i)
+    {
+      if constexpr (internal::DualTraits<NT>::is_dual)
+        throw std::runtime_error("Cannot access dual-number data directly");
+      else
+        return h_table[i];
+    }
+    const ReturnType &operator[](const unsigned i) const
+    {
+      if constexpr (internal::DualTraits<NT>::is_dual)
+        throw std::runtime_error("Cannot access dual-number data directly");
+      else
+        return h_table[i];
+    }
     const Coordinates &get_coordinates() const { return coordinates; }
 
     __host__ __device__ ReturnType operator()(const float x) const
     {
       const float idx = float(coordinates.backward(x));
+      const unsigned n = size;
+      const auto lerp = [idx, n](const float *t) { return internal::lerp1(t, idx, n); };
 #ifdef __HIP_DEVICE_COMPILE__
-      return internal::lerp1(d_table, idx, size);
+      return internal::tex_value<NT>(d_table, d_table_AD, lerp);
 #else
-      return internal::lerp1((const float *)h_table, idx, size);
+      return internal::tex_value<NT>(h_table, h_table_AD, lerp);
 #endif
     }
 
   private:
     unsigned size;
     Coordinates coordinates;
-    std::shared_ptr<internal::TableStorage<float>> table;
-    const float *d_table = nullptr;
-    float *h_table = nullptr;
+    std::shared_ptr<internal::TexTables<NT>> tables;
+    const float *d_table = nullptr, *d_table_AD = nullptr;
+    float *h_table = nullptr, *h_table_AD = nullptr;
     bool owner;
   };
 } // namespace DiFfRG

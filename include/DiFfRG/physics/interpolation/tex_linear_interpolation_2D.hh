@@ -1,7 +1,7 @@
 // DiFfRG/physics/interpolation/tex_linear_interpolation_2D.hh -- TexLinearInterpolator2D<NT,
 // Coordinates> (reference tex_linear_interpolation_2D.hh:20-150): FP32 table over 2D coordinates,
 // bilinear with the reference's clamp and corner rule, computed in FP32 on the device (HBM table)
-// and on the host.  Returns float.
+// and on the host.  Returns float, or both channels of dual-number data.
 #pragma once
 
 #include <array>
@@ -30,54 +30,70 @@ namespace DiFfRG
       update(data);
     }
     TexLinearInterpolator2D(const Coordinates &coordinates)
-        : size(coordinates.size()), shape(coordinates.sizes()), coordinates(coordinates),
-          table(std::make_shared<internal::TableStorage<float>>(coordinates.size())), owner(true)
+        : size(coordinates.size()), shape(coordinates.sizes()), coordinates(coordinates), owner(true)
     {
       internal::require_axes(shape.data(), 2, "TexLinearInterpolator2D");
-      d_table = table->device_ptr();
-      h_table = table->host_ptr();
-      table->upload();
+      tables = std::make_shared<internal::TexTables<NT>>(size);
+      d_table = tables->device();
+      d_table_AD = tables->device_AD();
+      h_table = tables->host();
+      h_table_AD = tables->host_AD();
     }
     TexLinearInterpolator2D(const TexLinearInterpolator2D &o)
-        : size(o.size), shape(o.shape), coordinates(o.coordinates), table(o.table), d_table(o.d_table),
-          h_table(o.h_table), owner(false)
+        : size(o.size), shape(o.shape), coordinates(o.coordinates), tables(o.tables), d_table(o.d_table),
+          d_table_AD(o.d_table_AD), h_table(o.h_table), h_table_AD(o.h_table_AD), owner(false)
     {
     }
 
     template <typename NT2> void update(const NT2 *data)
     {
       if (!owner) throw std::runtime_error("Cannot update data of non-owner interpolator");
-      for (unsigned i = 0; i < size; ++i) h_table[i] = static_cast<float>(data[i]);
+      tables->fill(data);
       update();
     }
     void update()
     {
       if (!owner) throw std::runtime_error("Cannot update data of non-owner interpolator");
-      table->upload();
+      tables->upload();
     }
     float *data() const { return h_table; }
-    ReturnType &operator[](const unsigned i) { return h_table[i]; }
-    const ReturnType &operator[](const unsigned i) const { return h_table[i]; }
+    float *data_AD() const { return h_table_AD; }
+    ReturnType &operator[](const unsigned i)
+    {
+      if constexpr (internal::DualTraits<NT>::is_dual)
+        throw std::runtime_error("Cannot access dual-number data directly");
+      else
+        return h_table[i];
+    }
+    const ReturnType &operator[](const unsigned i) const
+    {
+      if constexpr (internal::DualTraits<NT>::is_dual)
+        throw std::runtime_error("Cannot access dual-number data directly");
+      else
+        return h_table[i];
+    }
     const Coordinates &get_coordinates() const { return coordinates; }
 
     __host__ __device__ ReturnType operator()(const float x, const float y) const
     {
       const auto idx = coordinates.backward(x, y);
+      const float ix = float(idx[0]), iy = float(idx[1]);
+      const unsigned nx = shape[0], ny = shape[1];
+      const auto lerp = [=](const float *t) { return internal::lerp2(t, ix, iy, nx, ny); };
 #ifdef __HIP_DEVICE_COMPILE__
-      const float *d = d_table;
+      return internal::tex_value<NT>(d_table, d_table_AD, lerp);
 #else
-      const float *d = h_table;
+      return internal::tex_value<NT>(h_table, h_table_AD, lerp);
 #endif
-      return internal::lerp2(d, float(idx[0]), float(idx[1]), shape[0], shape[1]);
     }
 
   private:
     unsigned size;
     std::array<unsigned, 2> shape;
     Coordinates coordinates;
-    std::shared_ptr<internal::TableStorage<float>> table;
-    const float *d_table = nullptr;
-    float *h_table = nullptr;
+    std::shared_ptr<internal::TexTables<NT>> tables;
+    const float *d_table = nullptr, *d_table_AD = nullptr;
+    float *h_table = nullptr, *h_table_AD = nullptr;
     bool owner;
   };
 } // namespace DiFfRG

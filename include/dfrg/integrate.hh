@@ -24,8 +24,10 @@
 #include <functional>
 #include <tuple>
 #include <type_traits>
+#include <utility>
 
 #include <DiFfRG/common/complex_math.hh>
+#include <DiFfRG/common/dual_traits.hh>
 #include <DiFfRG/common/hip_prefix.hh>
 #include <DiFfRG/common/math.hh>
 
@@ -40,8 +42,37 @@ namespace dfrg
     constexpr int kNodesPerLane = 4;
 
     // ------------------------------------------------------------------------------------------
-    // reductions over the number type NT (float, double, DiFfRG::complex<float|double>)
+    // reductions over the number type NT: float, double, or a two-component type reduced component
+    // by component -- DiFfRG::complex<float|double> (re, im) or a dual number (value, derivative;
+    // DiFfRG/common/dual_traits.hh)
     // ------------------------------------------------------------------------------------------
+    template <class NT> struct Parts {
+      static constexpr bool dual = DiFfRG::internal::DualTraits<NT>::is_dual;
+      static __host__ __device__ __forceinline__ auto first(const NT &v)
+      {
+        if constexpr (dual)
+          return DiFfRG::internal::DualTraits<NT>::value(v);
+        else
+          return v.re;
+      }
+      static __host__ __device__ __forceinline__ auto second(const NT &v)
+      {
+        if constexpr (dual)
+          return DiFfRG::internal::DualTraits<NT>::derivative(v);
+        else
+          return v.im;
+      }
+      template <class P> static __host__ __device__ __forceinline__ NT make(P a, P b)
+      {
+        if constexpr (dual)
+          return DiFfRG::internal::DualTraits<NT>::make(a, b);
+        else
+          return NT(a, b);
+      }
+      using P = decltype(first(std::declval<NT>()));
+    };
+    template <class NT> constexpr bool is_scalar_nt = std::is_same_v<NT, double> || std::is_same_v<NT, float>;
+
     __device__ __forceinline__ float wave_sum_f32(float v)
     {
       v += dev::dpp_f32<0xB1>(v);
@@ -59,7 +90,7 @@ namespace dfrg
       else if constexpr (std::is_same_v<NT, float>)
         return wave_sum_f32(v);
       else
-        return NT(wave_reduce(v.re), wave_reduce(v.im));
+        return Parts<NT>::make(wave_reduce(Parts<NT>::first(v)), wave_reduce(Parts<NT>::second(v)));
     }
 
     template <class NT> __device__ __forceinline__ NT block_reduce(NT v)
@@ -76,22 +107,29 @@ namespace dfrg
       return t;
     }
 
-    // Partials of type NT through device-scope atomics, component-wise (dfrg::dev::publish).
+    // Partials of type NT through device-scope atomics, component-wise (dfrg::dev::publish).  A
+    // two-component partial is stored as its two components at the start of its NT slot.
     template <class NT> __device__ __forceinline__ void publish_nt(NT *p, const NT &v)
     {
-      if constexpr (std::is_same_v<NT, double> || std::is_same_v<NT, float>)
+      if constexpr (is_scalar_nt<NT>)
         dev::publish(p, v);
       else {
-        dev::publish(&p->re, v.re);
-        dev::publish(&p->im, v.im);
+        using P = typename Parts<NT>::P;
+        static_assert(sizeof(NT) >= 2 * sizeof(P), "two-component number type smaller than its components");
+        P *raw = reinterpret_cast<P *>(p);
+        dev::publish(raw, Parts<NT>::first(v));
+        dev::publish(raw + 1, Parts<NT>::second(v));
       }
     }
     template <class NT> __device__ __forceinline__ NT fetch_nt(const NT *p)
     {
-      if constexpr (std::is_same_v<NT, double> || std::is_same_v<NT, float>)
+      if constexpr (is_scalar_nt<NT>)
         return dev::fetch(p);
-      else
-        return NT(dev::fetch(&p->re), dev::fetch(&p->im));
+      else {
+        using P = typename Parts<NT>::P;
+        const P *raw = reinterpret_cast<const P *>(p);
+        return Parts<NT>::make(dev::fetch(raw), dev::fetch(raw + 1));
+      }
     }
 
     // partial slots of a single call: one 128-byte line each (dfrg::dev::kSlotStride)
@@ -99,10 +137,10 @@ namespace dfrg
 
     template <class NT> __device__ __forceinline__ NT lane_nt(const NT &v, int i)
     {
-      if constexpr (std::is_same_v<NT, double> || std::is_same_v<NT, float>)
+      if constexpr (is_scalar_nt<NT>)
         return dev::lane_value(v, i);
       else
-        return NT(dev::lane_value(v.re, i), dev::lane_value(v.im, i));
+        return Parts<NT>::make(dev::lane_value(Parts<NT>::first(v), i), dev::lane_value(Parts<NT>::second(v), i));
     }
 
     // Publish this block's partial in slot `slot` (at partials[slot * stride]) of a group of n_slots;

@@ -24,6 +24,35 @@
 #include <DiFfRG/physics/interpolation.hh>
 #include <DiFfRG/physics/regulators.hh>
 #include <DiFfRG/physics/utils.hh>
+
+// A minimal forward-mode dual number with the interface autodiff::real offers (value, derivative,
+// construction from both, arithmetic with doubles): the derivative channel of the FP32 tables, the
+// generic-NT path of LinearInterpolator1D and dual-valued integrals, without the autodiff library.
+struct Dual1 {
+  double v = 0.0, d = 0.0;
+  __host__ __device__ Dual1() {}
+  __host__ __device__ Dual1(double x) : v(x) {}
+  __host__ __device__ Dual1(double x, double y) : v(x), d(y) {}
+  __host__ __device__ Dual1 &operator+=(const Dual1 &o)
+  {
+    v += o.v;
+    d += o.d;
+    return *this;
+  }
+};
+__host__ __device__ inline Dual1 operator*(const Dual1 &a, double s) { return Dual1(a.v * s, a.d * s); }
+__host__ __device__ inline Dual1 operator*(double s, const Dual1 &a) { return Dual1(s * a.v, s * a.d); }
+__host__ __device__ inline Dual1 operator+(const Dual1 &a, const Dual1 &b) { return Dual1(a.v + b.v, a.d + b.d); }
+
+namespace DiFfRG::internal
+{
+  template <> struct DualTraits<Dual1> {
+    static constexpr bool is_dual = true;
+    __host__ __device__ static double value(const Dual1 &x) { return x.v; }
+    __host__ __device__ static double derivative(const Dual1 &x) { return x.d; }
+    __host__ __device__ static Dual1 make(double v, double d) { return Dual1(v, d); }
+  };
+} // namespace DiFfRG::internal
 #include <dfrg/explicit_rk.hh>
 #include <dfrg/x_extent.hh>
 
@@ -100,6 +129,25 @@ namespace
                                                               const float, const float, const float, const float)
     {
       return c;
+    }
+  };
+  // dual-valued variant: (poly_f(q), poly_g(q)) with constant (f0, g0)
+  struct PolyRadialDual {
+    static __forceinline__ __host__ __device__ Dual1 kernel(const double q, const double, const double f0,
+                                                            const double f1, const double f2, const double f3,
+                                                            const double f4, const double f5, const double g0,
+                                                            const double g1, const double g2, const double g3,
+                                                            const double g4, const double g5)
+    {
+      return Dual1(f0 + f1 * powr<1>(q) + f2 * powr<2>(q) + f3 * powr<3>(q) + f4 * powr<4>(q) + f5 * powr<5>(q),
+                   g0 + g1 * powr<1>(q) + g2 * powr<2>(q) + g3 * powr<3>(q) + g4 * powr<4>(q) + g5 * powr<5>(q));
+    }
+    static __forceinline__ __host__ __device__ Dual1 constant(const double, const double f0, const double,
+                                                              const double, const double, const double, const double,
+                                                              const double g0, const double, const double, const double,
+                                                              const double, const double)
+    {
+      return Dual1(f0, g0);
     }
   };
   // integrator_4D_2ang_cpu.cc:14-36
@@ -772,6 +820,134 @@ int kat_interp(int kind, const double *cp, const double *data, int n_data, const
   });
 }
 
+} // extern "C"
+
+
+namespace
+{
+  template <int DIM, class I> __global__ void interp_dual_kernel(I interp, const double *pts, int n, double *out)
+  {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *x = pts + (size_t)i * DIM;
+    Dual1 r;
+    if constexpr (DIM == 1)
+      r = interp(x[0]);
+    else if constexpr (DIM == 2)
+      r = interp(x[0], x[1]);
+    else
+      r = interp(x[0], x[1], x[2]);
+    out[2 * i] = r.v;
+    out[2 * i + 1] = r.d;
+  }
+
+  template <int DIM, class I>
+  void interp_dual_eval(const I &interp, const double *pts, int n, double *out_dev, double *out_host)
+  {
+    void *d_pts = nullptr, *d_out = nullptr;
+    dfrg_check(dfrg_malloc(&d_pts, sizeof(double) * DIM * n), "pts");
+    dfrg_check(dfrg_malloc(&d_out, sizeof(double) * 2 * n), "out");
+    void *s = nullptr;
+    dfrg_check(dfrg_thread_stream(&s), "thread stream");
+    dfrg_check(dfrg_memcpy_h2d(d_pts, pts, sizeof(double) * DIM * n, s), "pts");
+    hipLaunchKernelGGL((interp_dual_kernel<DIM, I>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)s, interp,
+                       (const double *)d_pts, n, (double *)d_out);
+    check_hip("interp_dual_kernel");
+    dfrg_check(dfrg_memcpy_d2h(out_dev, d_out, sizeof(double) * 2 * n, s), "out");
+    dfrg_check(dfrg_stream_sync(s), "out");
+    dfrg_free(d_pts);
+    dfrg_free(d_out);
+    for (int i = 0; i < n; ++i) {
+      const double *x = pts + (size_t)i * DIM;
+      Dual1 r;
+      if constexpr (DIM == 1)
+        r = interp(x[0]);
+      else if constexpr (DIM == 2)
+        r = interp(x[0], x[1]);
+      else
+        r = interp(x[0], x[1], x[2]);
+      out_host[2 * i] = r.v;
+      out_host[2 * i + 1] = r.d;
+    }
+  }
+} // namespace
+
+extern "C" {
+
+// Dual-number data (value val[i], derivative der[i]) in the interpolators; out_dev / out_host get
+// (value, derivative) per point.  kind: 0 TexLinearInterpolator1D (log coordinates), 1
+// TexLinearInterpolator2D, 2 TexLinearInterpolator3D, 3 TexLinearInterpolator1DStack (bosonic),
+// 4 LinearInterpolator1D<Dual1> (log coordinates, FP64), 5 TexLinearInterpolator1D built from the
+// dual data and then updated with plain values 2 val[i] (the derivative table is kept).  cp as in
+// kat_interp for the same coordinates.
+int kat_interp_dual(int kind, const double *cp, const double *val, const double *der, int n_data, const double *pts,
+                    int n_pts, double *out_dev, double *out_host)
+{
+  return guard([&] {
+    using LF = LinearCoordinates1D<float>;
+    std::vector<Dual1> data(n_data);
+    for (int i = 0; i < n_data; ++i) data[i] = Dual1(val[i], der[i]);
+    auto need = [&](unsigned n) {
+      if ((unsigned)n_data != n) throw std::invalid_argument("data size does not match the coordinates");
+    };
+    switch (kind) {
+    case 0:
+    case 5: {
+      LogarithmicCoordinates1D<float> c((unsigned)cp[0], cp[1], cp[2], cp[3]);
+      need(c.size());
+      TexLinearInterpolator1D<Dual1, decltype(c)> I(data, c);
+      if (kind == 5) {
+        std::vector<double> twice(val, val + n_data);
+        for (double &v : twice) v *= 2.0;
+        I.update(twice.data());
+      }
+      interp_dual_eval<1>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 1: {
+      CoordinatePackND<LF, LF> c(LF(cp[0], cp[1], cp[2]), LF(cp[3], cp[4], cp[5]));
+      need(c.size());
+      TexLinearInterpolator2D<Dual1, decltype(c)> I(data.data(), c);
+      interp_dual_eval<2>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 2: {
+      CoordinatePackND<LF, LF, LF> c(LF(cp[0], cp[1], cp[2]), LF(cp[3], cp[4], cp[5]), LF(cp[6], cp[7], cp[8]));
+      need(c.size());
+      TexLinearInterpolator3D<Dual1, decltype(c)> I(c);
+      I.update(data.data());
+      interp_dual_eval<3>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 3: {
+      BosonicCoordinates1DFiniteT<int, float> c((int)cp[0], (int)cp[1], cp[2], (unsigned)cp[3], cp[4], cp[5], cp[6]);
+      need(c.size());
+      TexLinearInterpolator1DStack<Dual1, decltype(c)> I(data, c);
+      interp_dual_eval<2>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 4: {
+      LogarithmicCoordinates1D<double> c((unsigned)cp[0], cp[1], cp[2], cp[3]);
+      need(c.size());
+      LinearInterpolator1D<Dual1, decltype(c)> I(data, c);
+      interp_dual_eval<1>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    default: throw std::invalid_argument("unknown interpolator kind");
+    }
+  });
+}
+
+// operator[] of a Tex interpolator over dual data throws, as the reference's does
+int kat_interp_dual_index(void)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<float> c(8, 1e-3f, 10.f, 2.f);
+    TexLinearInterpolator1D<Dual1, decltype(c)> I(c);
+    (void)I[0];
+  });
+}
+
 // views: update() on a copy throws; stack coordinates reject start > stop
 int kat_interp_errors(int which)
 {
@@ -906,6 +1082,27 @@ int kat_x_extent(int which, double a, unsigned order, double tol, double *x_exte
     else
       f = [](double q2) { return 1.0 / powr<3>(1.0 + q2); };
     *x_extent = dfrg::tmpl::optimize_x_extent(provider(), f, order, tol);
+  });
+}
+
+// NT = dual number: the radial integrator over (f, g) polynomial pairs, get and a batched request;
+// out = (get value, get derivative, request value, request derivative)
+int kat_radial_dual(int nx, double x_extent, double k, const double *cf, const double *cg, int tbb, double *out)
+{
+  return guard([&] {
+    auto run = [&](auto &&integ) {
+      const Dual1 a = integ.get(k, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5], cg[0], cg[1], cg[2], cg[3], cg[4], cg[5]);
+      const Dual1 b =
+          integ.request(k, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5], cg[0], cg[1], cg[2], cg[3], cg[4], cg[5]).get();
+      out[0] = a.v;
+      out[1] = a.d;
+      out[2] = b.v;
+      out[3] = b.d;
+    };
+    if (tbb)
+      run(IntegratorTBB<4, Dual1, PolyRadialDual>(provider(), nx, x_extent));
+    else
+      run(IntegratorGPU<4, Dual1, PolyRadialDual>(provider(), nx, x_extent));
   });
 }
 

@@ -176,11 +176,79 @@ def test_tex_1D_stack(kat, kind, fermionic):
         assert abs(dev[0] - want) <= 2e-6 * want
 
 
+def run_dual(kat, kind, cp, val, der, pts):
+    cp, val, der, pts = (np.ascontiguousarray(v, dtype=np.float64) for v in (cp, val, der, pts))
+    n = len(pts)
+    dev, host = np.empty(2 * n), np.empty(2 * n)
+    print(f"[case] dual-number interpolator kind {kind}, {len(val)} table values, {n} points", flush=True)
+    rc = kat.kat_interp_dual(kind, P(cp), P(val), P(der), len(val), P(pts), n, P(dev), P(host))
+    assert rc == 0, kat.kat_last_error()
+    return dev.reshape(n, 2), host.reshape(n, 2)
+
+
+def test_dual_number_channels(kat):
+    # Dual-number data (the reference's autodiff::real, tex_linear_interpolation_1D.hh:58-72,
+    # 86-106): each channel is the FP32 table interpolator on that channel's data, so both must be
+    # bitwise the plain interpolator's results on the value data and on the derivative data.
+    rng = np.random.default_rng(31)
+    n1 = 40
+    log_cp = [n1, np.float32(1e-3), np.float32(30.0), np.float32(4.0)]
+    a2 = [lin_axis(rng, np.float32) for _ in range(2)]
+    a3 = [(int(rng.integers(4, 12)),) + lin_axis(rng, np.float32)[1:] for _ in range(3)]
+    T, n_s = 0.05, 24
+    st_cp = [-3, 9, T, n_s, 1e-3, 20.0, 3.0]
+    cases = [  # (dual kind, plain kind, cp, table size, points)
+        (0, 2, log_cp, n1, (1e-3 + rng.uniform(-0.05, 1.05, 150) * 30).astype(np.float32).astype(np.float64)),
+        (1, 3, [v for ax in a2 for v in ax], a2[0][0] * a2[1][0], points(rng, a2, 150, np.float32)),
+        (2, 4, [v for ax in a3 for v in ax], a3[0][0] * a3[1][0] * a3[2][0], points(rng, a3, 150, np.float32)),
+        (3, 5, st_cp, 12 * n_s, np.stack([(rng.integers(-5, 11, 150) * 2 * math.pi * T),
+                                          1e-3 + rng.uniform(-0.05, 1.05, 150) * 20], axis=1)
+         .astype(np.float32).astype(np.float64)),
+    ]
+    for kd, kp, cp, size, pts in cases:
+        val = rng.uniform(-2, 2, size)
+        der = rng.uniform(-5, 5, size)
+        dev, host = run_dual(kat, kd, cp, val, der, pts)
+        v_dev, v_host = run(kat, kp, cp, val, pts)
+        d_dev, d_host = run(kat, kp, cp, der, pts)
+        assert np.array_equal(dev[:, 0], v_dev) and np.array_equal(dev[:, 1], d_dev), kd
+        assert np.array_equal(host[:, 0], v_host) and np.array_equal(host[:, 1], d_host), kd
+    # update() with plain values replaces the value table and keeps the derivative table
+    kd, kp, cp, size, pts = cases[0]
+    val, der = rng.uniform(-2, 2, size), rng.uniform(-5, 5, size)
+    dev, host = run_dual(kat, 5, cp, val, der, pts)
+    assert np.array_equal(dev[:, 0], run(kat, kp, cp, 2 * val, pts)[0])
+    assert np.array_equal(dev[:, 1], run(kat, kp, cp, der, pts)[0])
+
+
+def test_dual_number_fp64_interpolator(kat):
+    # LinearInterpolator1D over dual data (reference linear_interpolation_1D.cu:76-91 instantiates
+    # it for autodiff::real): FP64 lerp of both channels.  Value data linear in the index is
+    # reproduced exactly; the derivative channel is checked against the lerp formula.
+    n, start, stop, bias = 50, 1e-3, 40.0, 5.0
+    rng = np.random.default_rng(32)
+    j = np.arange(n, dtype=np.float64)
+    val = 0.75 * j - 3.0
+    der = np.sin(j) + j ** 2 / n
+    pts = start + rng.uniform(-0.02, 1.05, 300) * (stop - start)
+    dev, host = run_dual(kat, 4, [n, start, stop, bias], val, der, pts)
+    b = (stop - start) / math.expm1(bias)
+    with np.errstate(invalid="ignore"):
+        idx = np.log1p((pts - start) / b) * (n - 1) / bias
+    idx = np.clip(np.nan_to_num(idx, nan=n - 1), 0, n - 1)
+    fl = np.floor(idx)
+    want_d = der[fl.astype(int)] * (1 - idx + fl) + der[np.ceil(idx).astype(int)] * (idx - fl)
+    assert np.max(np.abs(dev[:, 0] - (0.75 * idx - 3.0))) <= 1e-12 * n
+    assert np.max(np.abs(dev[:, 1] - want_d)) <= 1e-12 * n
+    assert np.max(np.abs(dev - host)) <= 1e-12 * n
+
+
 def test_interpolator_errors(kat):
     print("[case] interpolator errors", flush=True)
     assert kat.kat_interp_errors(0) == -2 and b"non-owner" in kat.kat_last_error()
     assert kat.kat_interp_errors(1) == -1 and b"start must be smaller than stop" in kat.kat_last_error()
     assert kat.kat_interp_errors(2) == -2 and b"at least 2 points" in kat.kat_last_error()
+    assert kat.kat_interp_dual_index() == -2 and b"dual-number data" in kat.kat_last_error()
 
 
 # ------------------------------------------------------------------ C-ABI tables (dfrg_table_*)

@@ -112,6 +112,23 @@ def test_radial_template_float(kat, qp):
     assert r.value == g.value
 
 
+def test_radial_template_dual(kat, qp):
+    # NT = a dual number (the reference's autodiff::real, types.hh:130-167): both channels are
+    # integrated; each must match the oracle's radial integral of its polynomial, on the GPU and
+    # TBB templates, for get() and a batched request()
+    rng = np.random.default_rng(5)
+    x, w = qp.get_points(64), qp.get_weights(64)
+    f, g = arr(rng.uniform(-1, 1, 6)), arr(rng.uniform(-1, 1, 6))
+    for tbb in (0, 1):
+        out = np.empty(4)
+        print(f"[case] radial dual tbb={tbb}", flush=True)
+        call(kat, "kat_radial_dual", 64, D(1.5), D(0.7), P(f), P(g), tbb, P(out))
+        want_f = f[0] + O.kat_radial(4, x, w, 1.5, 0.7, f, 0.0)
+        want_g = g[0] + O.kat_radial(4, x, w, 1.5, 0.7, g, 0.0)
+        assert out[0] == pytest.approx(want_f, rel=TOL) and out[1] == pytest.approx(want_g, rel=TOL)
+        assert out[2] == out[0] and out[3] == out[1]
+
+
 def test_two_angle_template(kat, qp):
     rng = np.random.default_rng(7)
     xq, wq = qp.get_points(64), qp.get_weights(64)
