@@ -117,6 +117,22 @@ namespace
       return complex<double>(c, 0.0);
     }
   };
+  // complex<float> variant: poly(q) (1 + 2i) on FP32 nodes
+  struct PolyRadialCF {
+    static __forceinline__ __host__ __device__ complex<float> kernel(const float q, const float, const float,
+                                                                     const float x0, const float x1, const float x2,
+                                                                     const float x3, const float x4, const float x5)
+    {
+      const float v = x0 + x1 * powr<1>(q) + x2 * powr<2>(q) + x3 * powr<3>(q) + x4 * powr<4>(q) + x5 * powr<5>(q);
+      return complex<float>(v, 2.0f * v);
+    }
+    static __forceinline__ __host__ __device__ complex<float> constant(const float, const float c, const float,
+                                                                       const float, const float, const float,
+                                                                       const float, const float)
+    {
+      return complex<float>(c, 0.0f);
+    }
+  };
   // FP32 variant of PolyRadial (NT = ctype = float)
   struct PolyRadialF {
     static __forceinline__ __host__ __device__ float kernel(const float q, const float, const float, const float x0,
@@ -358,6 +374,21 @@ int kat_radial_complex(int nx, double x_extent, double k, const double *c, doubl
     const auto v = integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5]);
     *out_re = v.re;
     *out_im = v.im;
+  });
+}
+
+// NT = complex<float>: get and a batched request; out = (get re, get im, request re, request im)
+int kat_radial_complex_float(int nx, float x_extent, float k, const double *c, float constant, double *out)
+{
+  return guard([&] {
+    IntegratorGPU<4, complex<float>, PolyRadialCF> integ(provider(), nx, x_extent);
+    const float cf[6] = {(float)c[0], (float)c[1], (float)c[2], (float)c[3], (float)c[4], (float)c[5]};
+    const auto a = integ.get(k, constant, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5]);
+    const auto b = integ.request(k, constant, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5]).get();
+    out[0] = a.re;
+    out[1] = a.im;
+    out[2] = b.re;
+    out[3] = b.im;
   });
 }
 

@@ -112,6 +112,18 @@ def test_radial_template_float(kat, qp):
     assert r.value == g.value
 
 
+def test_radial_template_complex_float(kat, qp):
+    # NT = complex<float> (reference NT list, types.hh:130-167): both components on FP32 nodes
+    c = arr([0.3, -0.2, 0.5, 0.1, -0.4, 0.2])
+    x, w = qp.get_points(64), qp.get_weights(64)
+    out = np.empty(4)
+    print("[case] radial complex<float>", flush=True)
+    call(kat, "kat_radial_complex_float", 64, C.c_float(1.5), C.c_float(0.7), P(c), C.c_float(0.25), P(out))
+    ref = O.kat_radial(4, x, w, 1.5, 0.7, c, 0.0)
+    assert out[0] == pytest.approx(0.25 + ref, rel=2e-6) and out[1] == pytest.approx(2 * ref, rel=2e-6)
+    assert out[2] == out[0] and out[3] == out[1]
+
+
 def test_radial_template_dual(kat, qp):
     # NT = a dual number (the reference's autodiff::real, types.hh:130-167): both channels are
     # integrated; each must match the oracle's radial integral of its polynomial, on the GPU and
