@@ -296,6 +296,35 @@ def test_mixed_forms_in_one_batch(gpu, qp):
             assert o1.to_host()[0] == mix[i]  # bitwise
 
 
+@pytest.mark.parametrize("block,grid", [(64, 0), (256, 0), (1024, 0), (256, 64)])
+def test_selfenergy_order_extremes(gpu, qp, block, grid):
+    # The batch's limits: radial orders 1..1024 (odd ones too, and more than one tile's 128 radial
+    # nodes), angular orders 2..64, tables of 2..256 points; p = 0 (|p - q| = q) and p beyond the
+    # table's last point
+    rng = np.random.default_rng(77)
+    cases = [(1, 2, 2, 0.0), (1024, 2, 256, 5.0), (3, 64, 2, 0.3), (129, 64, 256, 60.0), (7, 16, 37, 1e-3),
+             (300, 8, 128, 12.0)]
+    nq = np.array([c[0] for c in cases])
+    na = np.array([c[1] for c in cases])
+    p = np.array([c[3] for c in cases])
+    start, stop, bias = 1e-3, 40.0, 6.0
+    tables = [1.0 + 0.5 * np.tanh(rng.uniform(-1, 1) * np.sin(np.linspace(0, 3, c[2]))) for c in cases]
+    xq = [(qp.get_points(int(v)), qp.get_weights(int(v))) for v in nq]
+    xa = [(qp.get_points(int(v)), qp.get_weights(int(v))) for v in na]
+    coords = [O.logcoord(len(t), start, stop, bias) for t in tables]
+    ref = O.c1_selfenergy(p, tables, coords, xq, xa, 20.0, 1.0)
+    b = gpu.SEBatch(qp, p, nq, na, tables, start, stop, bias, x_extent=20.0, k=1.0)
+    b.set_shape(block, 8192, grid)
+    out = gpu.DeviceArray(len(p))
+    for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+        print(f"[case] order extremes block={block} grid={grid} mode={mode} tiles={b.shape()['n_tiles']}",
+              flush=True)
+        b.run(out, mode)
+        err = rel_err(out.to_host(), ref)
+        print(f"  max rel err {err.max():.3e}", flush=True)
+        assert err.max() <= tol
+
+
 def test_uneven_graph_replay(gpu, qp):
     c = uneven_subset(128)
     b = make_uneven(gpu, qp, c)
