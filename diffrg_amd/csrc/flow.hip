@@ -11,10 +11,10 @@
 //
 // Device layout: every resident flow owns 4 n doubles in one HBM pool (u, RK accumulator, two
 // stage buffers) and a slot in SoA descriptor arrays.  A point table lists (slot, i) of every
-// resident grid point; one wave evaluates one point at a time (one Gauss-Legendre node per lane,
-// wave64 DPP sums) over a run of consecutive points, so all resident flows advance together in
-// each launch.  One RK step is four launches,
-// one per stage; steps_per_launch steps are captured into one hipGraph.  A flow's step index is
+// resident grid point; one wave evaluates four points at a time (one Gauss-Legendre node per lane,
+// the eight node sums reduced together across lanes) over a run of consecutive points, so all
+// resident flows advance together in each launch.  One RK step is four launches, one per stage;
+// steps_per_launch steps are captured into one hipGraph.  A flow's step index is
 // the launch's base step (written by the host before each graph launch) plus the step within the
 // graph minus the base step at its admission, so flows admitted at different times share the
 // graph and a finished flow simply stops updating.  The graph's last node advances the device copy
@@ -127,23 +127,40 @@ namespace
           m2sig = in[i] + 2.0 * rho * up;
         }
       }
+      // Points go four at a time: their eight node sums (two threshold integrals each) are
+      // reduced together by dev::wave_sum8; integral kind of point pp is vector 2 pp + kind.
       double f_lane = 0.0;
-      for (int p = 0; p < cnt; ++p) {
-        if (!__builtin_amdgcn_readlane(active, p)) continue; // wave-uniform
-        const double pT = dev::lane_value(T, p), pk = dev::lane_value(k, p);
-        const double pm2pi = dev::lane_value(m2pi, p), pm2sig = dev::lane_value(m2sig, p);
-        // two threshold integrals, lane j evaluates node j (d = 3 radial element)
-        const double k2 = pk * pk, inv_T = 1.0 / pT, inv_8T = 0.125 * inv_T;
-        double ipi = 0.0, isg = 0.0;
-        for (int j = lane; j < F.nx; j += 64) {
-          const double a = F.A[j], dd = F.D[j];
-          ipi = fma(dd, bose_S2(fma(k2, a, pm2pi), pT, inv_8T, inv_T, s_exp2), ipi);
-          isg = fma(dd, bose_S2(fma(k2, a, pm2sig), pT, inv_8T, inv_T, s_exp2), isg);
+      for (int g0 = 0; g0 < cnt; g0 += 4) {
+        double v[8];
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) {
+          const int p = g0 + pp;
+          double ipi = 0.0, isg = 0.0;
+          if (p < cnt && __builtin_amdgcn_readlane(active, p)) { // wave-uniform
+            const double pT = dev::lane_value(T, p), pk = dev::lane_value(k, p);
+            const double pm2pi = dev::lane_value(m2pi, p), pm2sig = dev::lane_value(m2sig, p);
+            // two threshold integrals, lane j evaluates node j (d = 3 radial element)
+            const double k2 = pk * pk, inv_T = 1.0 / pT, inv_8T = 0.125 * inv_T;
+            for (int j = lane; j < F.nx; j += 64) {
+              const double a = F.A[j], dd = F.D[j];
+              ipi = fma(dd, bose_S2(fma(k2, a, pm2pi), pT, inv_8T, inv_T, s_exp2), ipi);
+              isg = fma(dd, bose_S2(fma(k2, a, pm2sig), pT, inv_8T, inv_T, s_exp2), isg);
+            }
+          }
+          v[2 * pp] = ipi;
+          v[2 * pp + 1] = isg;
         }
-        const double k5 = k2 * k2 * pk;
-        ipi = k5 * dev::wave_sum(ipi);
-        isg = k5 * dev::wave_sum(isg);
-        if (lane == p) f_lane = 0.5 * ((Nn - 1.0) * up * ipi + (3.0 * up + 2.0 * rho * upp) * isg);
+        const double y = dev::wave_sum8(v);
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) {
+          const int p = g0 + pp;
+          if (p >= cnt || !__builtin_amdgcn_readlane(active, p)) continue; // wave-uniform
+          const double pk = dev::lane_value(k, p);
+          const double k5 = pk * pk * (pk * pk) * pk;
+          const double ipi = k5 * dev::lane_value(y, dev::sum8_lane(2 * pp));
+          const double isg = k5 * dev::lane_value(y, dev::sum8_lane(2 * pp + 1));
+          if (lane == p) f_lane = 0.5 * ((Nn - 1.0) * up * ipi + (3.0 * up + 2.0 * rho * upp) * isg);
+        }
       }
       if (lane < cnt && active) {
         double *u = F.pool + F.off[s];

@@ -41,6 +41,49 @@ namespace dfrg
       return v;
     }
 
+    // Sums of eight wave-wide vectors in one pass, by recursive halving: pairs of vectors swap
+    // register halves across lanes (v_permlane32_swap, then v_permlane16_swap) so that one add
+    // combines two vectors, an 8-lane rotation merges the last pair, and DPP finishes inside
+    // 8-lane groups.  34 VALU instructions for the eight sums instead of eight wave_sum calls.
+    // Lane l returns the total of vector 4 b3 + 2 b4 + b5 (b = the bits of l); sum8_lane(v) is
+    // the first of the eight lanes that hold vector v's total.  Every vector is added with the same
+    // tree (lane l with l + 32, then + 16, then l ^ 8, then within eight lanes, each add
+    // commutative), so its total does not depend on its position or on the other seven vectors.
+    constexpr int sum8_lane(int v) { return 8 * (v >> 2) + 16 * ((v >> 1) & 1) + 32 * (v & 1); }
+
+    // Lanes 0-31 return a_l + a_{l+32}, lanes 32-63 b_{l-32} + b_l.
+    __device__ __forceinline__ double pair_sum32(double a, double b)
+    {
+      const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+      const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+      const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+      const double a2 = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+      const double b2 = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+      return a2 + b2;
+    }
+    // Within each 32-lane half: lanes 0-15 return a_l + a_{l+16}, lanes 16-31 b_{l-16} + b_l.
+    __device__ __forceinline__ double pair_sum16(double a, double b)
+    {
+      const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+      const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+      const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+      const double a2 = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+      const double b2 = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+      return a2 + b2;
+    }
+    __device__ __forceinline__ double wave_sum8(const double (&v)[8])
+    {
+      const double w0 = pair_sum32(v[0], v[1]), w1 = pair_sum32(v[2], v[3]);
+      const double w2 = pair_sum32(v[4], v[5]), w3 = pair_sum32(v[6], v[7]);
+      const double x0 = pair_sum16(w0, w1), x1 = pair_sum16(w2, w3);
+      const bool up = (threadIdx.x & 8) != 0;
+      double y = (up ? x1 : x0) + dpp_f64<0x128>(up ? x0 : x1); // row_ror:8 pairs lane l with l ^ 8
+      y += dpp_f64<0xB1>(y);                                     // lane ^ 1
+      y += dpp_f64<0x4E>(y);                                     // lane ^ 2
+      y += dpp_f64<0x141>(y);                                    // the two quads of each 8 lanes
+      return y;
+    }
+
     // Block sum in a fixed order: wave sums to LDS, then wave 0 adds them in index order.
     // `smem` needs blockDim.x / 64 doubles.  Returns the total to every thread of the block.
     __device__ __forceinline__ double block_sum(double v, double *smem)

@@ -377,6 +377,41 @@ int kat_radial_complex(int nx, double x_extent, double k, const double *c, doubl
   });
 }
 
+} // extern "C"
+
+namespace
+{
+  __global__ void wave_sum8_kernel(const double *in, double *out)
+  {
+    const int l = threadIdx.x; // one wave
+    double v[8];
+    for (int i = 0; i < 8; ++i) v[i] = in[i * 64 + l];
+    out[l] = dfrg::dev::wave_sum8(v);
+  }
+} // namespace
+
+extern "C" {
+
+// dfrg::dev::wave_sum8 on one wave: in holds 8 vectors of 64 lanes (vector-major), out the 64
+// lane results
+int kat_wave_sum8(const double *in, double *out)
+{
+  return guard([&] {
+    void *d_in = nullptr, *d_out = nullptr;
+    dfrg_check(dfrg_malloc(&d_in, sizeof(double) * 512), "in");
+    dfrg_check(dfrg_malloc(&d_out, sizeof(double) * 64), "out");
+    void *s = nullptr;
+    dfrg_check(dfrg_thread_stream(&s), "thread stream");
+    dfrg_check(dfrg_memcpy_h2d(d_in, in, sizeof(double) * 512, s), "in");
+    hipLaunchKernelGGL(wave_sum8_kernel, dim3(1), dim3(64), 0, (hipStream_t)s, (const double *)d_in, (double *)d_out);
+    check_hip("wave_sum8_kernel");
+    dfrg_check(dfrg_memcpy_d2h(out, d_out, sizeof(double) * 64, s), "out");
+    dfrg_check(dfrg_stream_sync(s), "out");
+    dfrg_free(d_in);
+    dfrg_free(d_out);
+  });
+}
+
 // NT = complex<float>: get and a batched request; out = (get re, get im, request re, request im)
 int kat_radial_complex_float(int nx, float x_extent, float k, const double *c, float constant, double *out)
 {
