@@ -45,11 +45,18 @@ namespace
   constexpr int kFlowGrid = 1024; // waves = kFlowGrid * kFlowBlock / 64, grid-stride over chunks
   constexpr double kPi = 3.14159265358979323846;
 
+  // One resident grid point: with its flow's slot, state offset, size and admission step, a
+  // lane's first load gives the addresses of everything else it reads, so the rest of its data
+  // (state values, per-flow constants, k) comes in one second round of loads.
+  struct FlowPoint {
+    int slot, i, off, n, admit, pad;
+  };
+
   struct FlowDev {
-    int *n, *off, *admit, *status, *bad; // per slot; status 0 idle, 1 running (host-owned)
-    double *h, *T, *N;                   // grid spacing, temperature, O(N)
-    int2 *points;                        // (slot, i) of every resident grid point
-    int *n_points;                       // device copy of the point count
+    int *status, *bad;    // per slot; status 0 idle, 1 running (host-owned)
+    double *h, *T, *N;    // per slot: grid spacing, temperature, O(N)
+    FlowPoint *points;    // every resident grid point
+    int *n_points;        // device copy of the point count
     int *base;                           // global step index of the first step of this launch
     double *pool;                        // state: u | acc | y0 | y1 per flow
     const double *A, *D;                 // node factors (header comment)
@@ -89,29 +96,35 @@ namespace
     const int base = *F.base + local;
     // A wave takes a chunk of consecutive points, sized so that all waves get work: one point each
     // while points are fewer than waves, up to 64 when they are many.  Lane l gathers the data of
-    // point w0 + l (slot, neighbours in u, couplings) in one round of loads; the wave then
-    // evaluates the points' right-hand sides one after the other with the point's values
-    // broadcast from its lane (lanes over Gauss-Legendre nodes, wave64 DPP sums), and lane l
-    // applies the RK stage update of its point.
+    // point w0 + l (its record, then neighbours in the stage input, its own state and
+    // accumulator, the flow's constants and k) in two rounds of loads; the wave then evaluates
+    // the points' right-hand sides with the point's values broadcast from its lane (lanes over
+    // Gauss-Legendre nodes), and lane l applies the RK stage update of its point.
     const int chunk = min(64, max(1, (P + n_waves - 1) / n_waves));
     for (int w0 = wave_id * chunk; w0 < P; w0 += n_waves * chunk) {
       const int cnt = min(chunk, P - w0);
-      int s = 0, i = 0, n = 0, active = 0;
+      int s = 0, i = 0, n = 0, off = 0, active = 0;
       double up = 0.0, upp = 0.0, rho = 0.0, m2pi = 0.0, m2sig = 0.0, T = 1.0, Nn = 0.0, k = 0.0;
+      double u_i = 0.0, acc_i = 0.0; // this point's state and RK accumulator
       if (lane < cnt) {
-        const int2 pt = F.points[w0 + lane];
-        s = pt.x;
-        i = pt.y;
-        const int step = base - F.admit[s];
-        active = F.status[s] == 1 && step >= 0 && step < F.n_steps;
-        if (active) {
-          n = F.n[s];
-          const double *u = F.pool + F.off[s];
+        const FlowPoint pt = F.points[w0 + lane];
+        s = pt.slot;
+        i = pt.i;
+        n = pt.n;
+        off = pt.off;
+        const int step = base - pt.admit;
+        // a listed point's state range is allocated while its flow is resident, so the loads
+        // below stay in it; a flow past its last step is left alone
+        if (step >= 0 && step < F.n_steps) {
+          const double *u = F.pool + off;
           const double *in = stage == 1 ? u : (stage == 3 ? u + 3 * n : u + 2 * n);
+          active = F.status[s] == 1;
           const double h = F.h[s];
           T = F.T[s];
           Nn = F.N[s];
           k = F.kt[3 * step + (stage == 1 ? 0 : (stage == 4 ? 2 : 1))];
+          u_i = u[i];
+          if (stage > 1) acc_i = u[n + i];
           if (i == 0) {
             up = (-3.0 * in[0] + 4.0 * in[1] - in[2]) / (2.0 * h);
             upp = (in[0] - 2.0 * in[1] + in[2]) / (h * h);
@@ -163,20 +176,22 @@ namespace
         }
       }
       if (lane < cnt && active) {
-        double *u = F.pool + F.off[s];
+        // only this lane reads or writes u[i] and acc[i] in this launch; the neighbours other
+        // lanes read are in the stage's input buffer, which this stage does not write
+        double *u = F.pool + off;
         double *acc = u + n, *y0 = u + 2 * n, *y1 = u + 3 * n;
         const double f = f_lane, dt = F.dt;
         if (stage == 1) {
           acc[i] = f;
-          y0[i] = u[i] + 0.5 * dt * f;
+          y0[i] = u_i + 0.5 * dt * f;
         } else if (stage == 2) {
-          acc[i] += 2.0 * f;
-          y1[i] = u[i] + 0.5 * dt * f;
+          acc[i] = acc_i + 2.0 * f;
+          y1[i] = u_i + 0.5 * dt * f;
         } else if (stage == 3) {
-          acc[i] += 2.0 * f;
-          y0[i] = u[i] + dt * f;
+          acc[i] = acc_i + 2.0 * f;
+          y0[i] = u_i + dt * f;
         } else {
-          const double un = u[i] + dt / 6.0 * (acc[i] + f);
+          const double un = u_i + dt / 6.0 * (acc_i + f);
           u[i] = un;
           if (!isfinite(un)) atomicOr(&F.bad[s], 1);
         }
@@ -330,9 +345,6 @@ namespace
       for (int i = 0; i < n; ++i) init[i] = r.desc.m2_0 + r.desc.lambda / 2.0 * (i * h);
       DFRG_HIP(hipMemcpy(fs->dev.pool + off, init.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
       const int one = 1, zero = 0;
-      DFRG_HIP(hipMemcpy(fs->dev.n + slot, &n, sizeof(int), hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.off + slot, &off, sizeof(int), hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.admit + slot, &fs->base, sizeof(int), hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.bad + slot, &zero, sizeof(int), hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.h + slot, &h, sizeof(double), hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.T + slot, &r.desc.T, sizeof(double), hipMemcpyHostToDevice));
@@ -346,14 +358,15 @@ namespace
       fs->points_dirty = true;
     }
     if (fs->points_dirty) {
-      std::vector<int2> pts;
+      std::vector<FlowPoint> pts;
       for (int s = 0; s < fs->max_resident; ++s) {
         if (fs->slot_flow[s] < 0) continue;
-        const int n = fs->flows[fs->slot_flow[s]].desc.n_grid;
-        for (int i = 0; i < n; ++i) pts.push_back(make_int2(s, i));
+        const FlowRecord &r = fs->flows[fs->slot_flow[s]];
+        const int n = r.desc.n_grid;
+        for (int i = 0; i < n; ++i) pts.push_back(FlowPoint{s, i, r.off, n, r.admit, 0});
       }
       const int P = (int)pts.size();
-      if (P) DFRG_HIP(hipMemcpy(fs->dev.points, pts.data(), sizeof(int2) * P, hipMemcpyHostToDevice));
+      if (P) DFRG_HIP(hipMemcpy(fs->dev.points, pts.data(), sizeof(FlowPoint) * P, hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.n_points, &P, sizeof(int), hipMemcpyHostToDevice));
       fs->points_dirty = false;
     }
@@ -419,7 +432,7 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
     nodes[2 * nx + 3 * st + 1] = std::exp(-(t + 0.5 * d.dt));
     nodes[2 * nx + 3 * st + 2] = std::exp(-(t + d.dt));
   }
-  const size_t pts_bytes = sizeof(int2) * std::min(max_points, fs->pool_doubles / 4 + 1);
+  const size_t pts_bytes = sizeof(FlowPoint) * std::min(max_points, fs->pool_doubles / 4 + 1);
 #define FS_ALLOC(ptr, bytes)                                                                                   \
   do {                                                                                                         \
     if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess || hipMemset((ptr), 0, (bytes)) != hipSuccess) {     \
@@ -428,9 +441,6 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
       return DFRG_ERR_HIP;                                                                                     \
     }                                                                                                          \
   } while (0)
-  FS_ALLOC(d.n, sizeof(int) * max_resident);
-  FS_ALLOC(d.off, sizeof(int) * max_resident);
-  FS_ALLOC(d.admit, sizeof(int) * max_resident);
   FS_ALLOC(d.status, sizeof(int) * max_resident);
   FS_ALLOC(d.bad, sizeof(int) * max_resident);
   FS_ALLOC(d.h, sizeof(double) * max_resident);
@@ -468,7 +478,7 @@ int dfrg_fs_destroy(dfrg_fs *fs)
   if (fs->graph) hipGraphDestroy(fs->graph);
   if (fs->stream) hipStreamDestroy(fs->stream);
   FlowDev &d = fs->dev;
-  for (void *p : {(void *)d.n, (void *)d.off, (void *)d.admit, (void *)d.status, (void *)d.bad, (void *)d.h,
+  for (void *p : {(void *)d.status, (void *)d.bad, (void *)d.h,
                   (void *)d.T, (void *)d.N, (void *)d.points, (void *)d.n_points, (void *)d.base, (void *)d.pool,
                   (void *)fs->d_nodes})
     if (p) hipFree(p);
