@@ -507,7 +507,8 @@ def main():
     achieved = fl1 * evals / (mx["ms"] * 1e-3) / 1e12
     peak = PEAK_FP64_VALU / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                       "frac": achieved / peak, "traffic": measured_traffic(), "kernel": f"se_kernel<{v1}>",
+                       "frac": achieved / peak, "traffic": measured_traffic(), "kernel": f"se_kernel<MIXED, {mx['shape']['block']}>",
+                       "integrand_form": {1: "general", 2: "narrow"}[v1],
                        "flop_per_eval": fl1, "valu_per_eval": va1, "loop": se_loop(na, mx["shape"]["block"]),
                        "evals_per_launch": evals,
                        "valu_issue_frac": valu_issue_frac(va1, evals, mx["ms"] * 1e-3),
