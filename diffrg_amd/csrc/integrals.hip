@@ -1006,7 +1006,7 @@ int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cac
   DFRG_HIP(hipEventCreate(&e1));
   const int cus = context().cu_count > 0 ? context().cu_count : 256;
   const int blocks[] = {128, 256, 512, 1024};
-  const int tiles[] = {1024, 2048, 4096, 8192, 16384, 32768};
+  const int tiles[] = {1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072};
   float best = 1e30f;
   int bb = b->block, bt = b->tile_evals, bg = b->grid;
   for (int bl : blocks) {
