@@ -183,6 +183,40 @@ namespace dfrg
     // partition, so a batched result is bitwise the single call's.
     // ------------------------------------------------------------------------------------------
     // IntegratorGPU<d> (integrator_gpu.hh:34-60)
+    // The digits of a flat node index (slowest first) in mixed radix r, walked with a fixed
+    // stride S: a thread's nodes i, i + S, i + 2S, ... of the flat loop, with the stride's digits
+    // added with carries instead of a division per node.  The slowest digit is clamped to its
+    // radix at the start, so a thread whose first node is past the end does not iterate.
+    template <int N> struct NodeWalk {
+      int dig[N], step[N], rad[N];
+      __device__ __forceinline__ NodeWalk(long long i, long long S, const int (&r)[N])
+      {
+#pragma unroll
+        for (int j = N - 1; j >= 1; --j) {
+          rad[j] = r[j];
+          dig[j] = (int)(i % r[j]);
+          i /= r[j];
+          step[j] = (int)(S % r[j]);
+          S /= r[j];
+        }
+        rad[0] = r[0];
+        dig[0] = (int)(i < r[0] ? i : r[0]);
+        step[0] = (int)S;
+      }
+      __device__ __forceinline__ bool valid() const { return dig[0] < rad[0]; }
+      __device__ __forceinline__ void next()
+      {
+        int carry = 0;
+#pragma unroll
+        for (int j = N - 1; j >= 1; --j) {
+          const int v = dig[j] + step[j] + carry; // < 2 rad[j]
+          carry = v >= rad[j];
+          dig[j] = carry ? v - rad[j] : v;
+        }
+        dig[0] += step[0] + carry;
+      }
+    };
+
     template <int d, class ctype, class NT, class KERNEL, class... T>
     __device__ __forceinline__ NT radial_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq, int nx,
                                              ctype x_extent, int b, int nb, ctype k, const T &...t)
@@ -207,12 +241,9 @@ namespace dfrg
                                              int na, ctype x_extent, int b, int nb, ctype k, const T &...t)
     {
       NT acc = NT(0);
-      const long long W = (long long)nx * na * na;
-      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
-        const int i2 = (int)(i % na);
-        const long long r = i / na;
-        const int i1 = (int)(r % na);
-        const int ix = (int)(r / na);
+      for (NodeWalk<3> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na, na}); w.valid();
+           w.next()) {
+        const int ix = w.dig[0], i1 = w.dig[1], i2 = w.dig[2];
         const ctype q = k * sqrt(xq[ix] * x_extent);
         const ctype cos1 = 2 * (xa[i1] - (ctype)0.5);
         const ctype int_element = (DiFfRG::powr<2>(q) * (ctype)0.5 * DiFfRG::powr<2>(k)) // x = p^2 / k^2 integral
@@ -233,14 +264,9 @@ namespace dfrg
                                              int na, ctype x_extent, int b, int nb, ctype k, const T &...t)
     {
       NT acc = NT(0);
-      const long long W = (long long)nx * na * na * na;
-      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
-        const int i2 = (int)(i % na); // cos2
-        long long r = i / na;
-        const int i3 = (int)(r % na); // phi
-        r /= na;
-        const int i1 = (int)(r % na); // cos1
-        const int ix = (int)(r / na);
+      for (NodeWalk<4> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na, na, na}); w.valid();
+           w.next()) {
+        const int ix = w.dig[0], i1 = w.dig[1], i3 = w.dig[2], i2 = w.dig[3]; // q, cos1, phi, cos2
         const ctype q = k * sqrt(xq[ix] * x_extent);
         const ctype cos1 = 2 * (xa[i1] - (ctype)0.5);
         const ctype phi = 2 * (ctype)M_PI * xa[i3];
@@ -262,10 +288,8 @@ namespace dfrg
     {
       NT acc = NT(0);
       constexpr ctype S_dm1 = DiFfRG::S_d_prec<ctype>(d - 1);
-      const long long W = (long long)nx * nm;
-      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
-        const int iy = (int)(i % nm);
-        const int ix = (int)(i / nm);
+      for (NodeWalk<2> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, nm}); w.valid(); w.next()) {
+        const int ix = w.dig[0], iy = w.dig[1];
         const ctype q = k * sqrt(xq[ix] * x_extent);
         const ctype x_weight = wq[ix] * x_extent;
         const ctype int_element = S_dm1 * (DiFfRG::powr<d - 3>(q) / (ctype)2 * DiFfRG::powr<2>(k)) /
@@ -287,10 +311,8 @@ namespace dfrg
     {
       NT acc = NT(0);
       constexpr ctype S_d = DiFfRG::S_d_prec<ctype>(d);
-      const long long W = (long long)nx * na;
-      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
-        const int iy = (int)(i % na);
-        const int ix = (int)(i / na);
+      for (NodeWalk<2> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na}); w.valid(); w.next()) {
+        const int ix = w.dig[0], iy = w.dig[1];
         const ctype q = k * sqrt(xq[ix] * x_extent);
         const ctype cos = 2 * (xa[iy] - (ctype)0.5);
         const ctype int_element = S_d                                                // solid angle
@@ -312,14 +334,9 @@ namespace dfrg
                                                      ctype k, const T &...t)
     {
       NT acc = NT(0);
-      const long long W = (long long)nx * na * na * nm;
-      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
-        const int in = (int)(i % nm);
-        long long r = i / nm;
-        const int iz = (int)(r % na); // phi
-        r /= na;
-        const int iy = (int)(r % na); // cos
-        const int ix = (int)(r / na);
+      for (NodeWalk<4> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na, na, nm}); w.valid();
+           w.next()) {
+        const int ix = w.dig[0], iy = w.dig[1], iz = w.dig[2], in = w.dig[3]; // q, cos, phi, n
         const ctype q = k * sqrt(xq[ix] * x_extent);
         const ctype int_element = (DiFfRG::powr<1>(q) / (ctype)2 * DiFfRG::powr<2>(k)) // x = q^2/k^2, 3 dims
                                   / DiFfRG::powr<3>(2 * (ctype)M_PI);                   // Fourier factor
@@ -346,12 +363,9 @@ namespace dfrg
     {
       NT acc = NT(0);
       constexpr ctype S_dm1 = DiFfRG::S_d_prec<ctype>(d - 1);
-      const long long W = (long long)nx * na * nm;
-      for (long long i = (long long)b * kBlock + threadIdx.x; i < W; i += (long long)nb * kBlock) {
-        const int in = (int)(i % nm);
-        const long long r = i / nm;
-        const int iy = (int)(r % na);
-        const int ix = (int)(r / na);
+      for (NodeWalk<3> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na, nm}); w.valid();
+           w.next()) {
+        const int ix = w.dig[0], iy = w.dig[1], in = w.dig[2];
         const ctype q = k * sqrt(xq[ix] * x_extent);
         const ctype int_element = S_dm1                                                // solid angle
                                   * (DiFfRG::powr<d - 3>(q) / (ctype)2 * DiFfRG::powr<2>(k)) // x = q^2/k^2
