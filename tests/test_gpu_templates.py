@@ -175,6 +175,27 @@ def test_three_angle_template(kat, qp):
         assert out.value == pytest.approx(ref, rel=TOL, abs=TOL * abs(const))
 
 
+@pytest.mark.parametrize("nx,na", [(37, 13), (5, 3), (129, 7), (300, 11)])
+def test_angle_templates_odd_sizes(kat, qp, nx, na):
+    # Orders whose node counts do not divide the thread stride, so every digit of a thread's
+    # node walk carries (the stride's lower digits are nonzero); a few sizes give more nodes than
+    # one pass of the grid, so threads take several strides.
+    rng = np.random.default_rng(nx * 100 + na)
+    xq, wq = qp.get_points(nx), qp.get_weights(nx)
+    xa, wa = qp.get_points(na), qp.get_weights(na)
+    x_ext, k = rng.uniform(1, 2), rng.uniform(0.1, 1)
+    c, a, b, p = (arr(rng.uniform(-1, 1, n)) for n in (6, 4, 4, 4))
+    const = rng.uniform(-1, 1)
+    print(f"[case] 2- and 3-angle templates nx={nx} na={na}", flush=True)
+    out = D()
+    call(kat, "kat_2ang", 1, nx, na, na, D(x_ext), D(k), P(c), P(a), P(b), D(const), C.byref(out))
+    ref = O.kat_4d2ang(xq, wq, xa, wa, x_ext, k, c, a, b, const)
+    assert out.value == pytest.approx(ref, rel=TOL, abs=TOL * abs(const))
+    call(kat, "kat_3ang", 1, nx, na, D(x_ext), D(k), P(c), P(a), P(b), P(p), D(const), C.byref(out))
+    ref = O.kat_4d(xq, wq, xa, wa, x_ext, k, c, a, b, p, const)
+    assert out.value == pytest.approx(ref, rel=TOL, abs=TOL * abs(const))
+
+
 @pytest.mark.parametrize("d", [2, 3, 4])
 def test_finiteT_template(kat, qp, d):
     rng = np.random.default_rng(60 + d)
