@@ -36,27 +36,30 @@ PEAK_VALU_WAVE_INSTR = 256 * 2.4e9
 MIXED_TOL = 1e-8
 GRAPH_SETS = 10  # call sets per hipGraph launch
 
-# (FP operations with FMA = 2, VALU instructions) per integrand evaluation in the two innermost
-# loops of se_kernel<MODE, BLOCK>, counted from the gfx950 assembly of csrc/integrals.hip by
-# `python -m diffrg_amd.isa_count`: "pairs" when a lane owns two angular pairs
-# (na^2 >= 2 BLOCK), "single" when it owns one pair and two radial nodes per iteration.
-# tests/test_isa_counts.py checks these figures against the current source.
+# (FP operations with FMA = 2, VALU instructions) per integrand evaluation in the innermost
+# loops of se_kernel<MODE, BLOCK, UNROLL>, counted from the gfx950 assembly of
+# csrc/integrals.hip by `python -m diffrg_amd.isa_count`: "pairs" when a lane owns two angular
+# pairs (na^2 >= 2 BLOCK), "pairs2" the same loop taking two radial nodes per step (the mixed
+# path's unroll 2, SEBatch.unroll), "single" when a lane owns one pair and two radial nodes per
+# iteration.  tests/test_isa_counts.py checks these figures against the current source.
 # Keyed by the integrand form (SEBatch.variants): 0 FP64, 1 mixed, 2 mixed on an integral that
 # admits the narrow exponent rounding.
 SE_OPS = {0: {"pairs": (223.5, 203.0), "single": (223.5, 205.0)},
-          1: {"pairs": (46.0, 44.0), "single": (46.0, 46.0)},
-          2: {"pairs": (46.0, 42.0), "single": (46.0, 44.0)}}
+          1: {"pairs": (46.0, 44.0), "pairs2": (46.0, 43.5), "single": (46.0, 46.0)},
+          2: {"pairs": (46.0, 42.0), "pairs2": (46.0, 41.5), "single": (46.0, 44.0)}}
 SE_OPS_BLOCK = {}  # (mode, block, loop) whose schedule differs from the others
 
 
-def se_loop(na, block):
-    return "pairs" if 2 * min(na * na, block) <= na * na else "single"
+def se_loop(na, block, unroll=1):
+    if 2 * min(na * na, block) > na * na:
+        return "single"
+    return "pairs2" if unroll == 2 else "pairs"
 
 
-def se_ops(variant, na, block):
-    """(FLOP, VALU) per evaluation of the loop se_kernel<., block> runs with integrand form
-    `variant` for na angular nodes."""
-    loop = se_loop(na, block)
+def se_ops(variant, na, block, unroll=1):
+    """(FLOP, VALU) per evaluation of the loop se_kernel<., block, unroll> runs with integrand
+    form `variant` for na angular nodes."""
+    loop = se_loop(na, block, unroll)
     return SE_OPS_BLOCK.get((variant, block, loop), SE_OPS[variant][loop])
 
 
@@ -237,7 +240,7 @@ def selfenergy(D, qp, args):
                 err = max(err, rel_err(o.to_host(), ref))
         ms = min(ms_eager, ms_graph)
         res[name] = {"ms": ms, "ms_eager": ms_eager, "ms_graph": ms_graph, "ms_streams": ms_streams, "shape": shape,
-                     "err": err, "launch": "hipGraph" if ms_graph < ms_eager else "eager"}
+                     "err": err, "launch": "hipGraph" if ms_graph < ms_eager else "eager", "unroll": b.unroll(mode)}
         log(f"selfenergy {name}: eager {ms_eager * 1e3:.1f} us, graph {ms_graph * 1e3:.1f} us, "
             f"{N_STREAMS} streams {ms_streams * 1e3:.1f} us per call set, "
             f"err {err:.2e}, shape {shape}")
@@ -254,7 +257,7 @@ def selfenergy(D, qp, args):
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
         shape = bb.autotune(mode, reps=3)
         ms, _ = time_steps(D, lambda: bb.run(outb, mode), max(1, args.steps // BATCH), max(1, args.warmup // BATCH))
-        fl, va = se_ops(se_variant(bb, mode), c["na"], shape["block"])
+        fl, va = se_ops(se_variant(bb, mode), c["na"], shape["block"], bb.unroll(mode))
         res[name][f"batched_x{BATCH}"] = {
             "integrals_per_s": nb / (ms * 1e-3), "us_per_launch": ms * 1e3, "shape": shape,
             "max_rel_err": rel_err(outb.to_host(), refb),
@@ -387,7 +390,8 @@ def uneven(D, qp, args):
         ms /= per_graph
         r[name] = {"integrals_per_s": n / (ms * 1e-3), "evals_per_s": b.evals / (ms * 1e-3), "ms_per_batch": ms,
                    "max_rel_err": rel_err(out.to_host(), ref), "shape": shape,
-                   "valu_issue_frac": sum(int(q) * int(a) ** 2 * se_ops(int(v), int(a), shape["block"])[1]
+                   "valu_issue_frac": sum(int(q) * int(a) ** 2 * se_ops(int(v), int(a), shape["block"],
+                                                                          b.unroll(mode))[1]
                                           for q, a, v in zip(c["nq"], c["na"], b.variants(mode)))
                                       / 64 / (ms * 1e-3)
                                       / PEAK_VALU_WAVE_INSTR}
@@ -502,18 +506,18 @@ def main():
     }
     na = se["na"]
     v1 = se["variant"][1]
-    fl1, va1 = se_ops(v1, na, mx["shape"]["block"])
-    fl0, va0 = se_ops(se["variant"][0], na, fp["shape"]["block"])
+    fl1, va1 = se_ops(v1, na, mx["shape"]["block"], mx["unroll"])
+    fl0, va0 = se_ops(se["variant"][0], na, fp["shape"]["block"], fp["unroll"])
     achieved = fl1 * evals / (mx["ms"] * 1e-3) / 1e12
     peak = PEAK_FP64_VALU / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                        "frac": achieved / peak, "traffic": measured_traffic(), "kernel": f"se_kernel<MIXED, {mx['shape']['block']}>",
                        "integrand_form": {1: "general", 2: "narrow"}[v1],
-                       "flop_per_eval": fl1, "valu_per_eval": va1, "loop": se_loop(na, mx["shape"]["block"]),
+                       "flop_per_eval": fl1, "valu_per_eval": va1, "loop": se_loop(na, mx["shape"]["block"], mx["unroll"]),
                        "evals_per_launch": evals,
                        "valu_issue_frac": valu_issue_frac(va1, evals, mx["ms"] * 1e-3),
                        "plain_fp64": {"flop_per_eval": fl0, "valu_per_eval": va0,
-                                      "loop": se_loop(na, fp["shape"]["block"]),
+                                      "loop": se_loop(na, fp["shape"]["block"], fp["unroll"]),
                                       "achieved": fl0 * evals / (fp["ms"] * 1e-3) / 1e12,
                                       "frac": fl0 * evals / (fp["ms"] * 1e-3) / PEAK_FP64_VALU,
                                       "valu_issue_frac": valu_issue_frac(va0, evals, fp["ms"] * 1e-3)}}

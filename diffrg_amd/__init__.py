@@ -88,6 +88,7 @@ SIGNATURES = {
     "dfrg_se_batch_graph_launch": [_vp, C.c_int, _vp, C.c_int, _vp],
     "dfrg_se_batch_evals": [_vp, C.POINTER(C.c_ulonglong)],
     "dfrg_se_batch_variants": [_vp, C.c_int, _ip],
+    "dfrg_se_batch_unroll": [_vp, _ip],
     "dfrg_ft_set_block": [C.c_int],
     "dfrg_ft_batch": [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
                       C.c_double, C.c_double, C.c_int, _vp, _vp],
@@ -406,6 +407,14 @@ class SEBatch:
         v = np.empty(self.n, dtype=np.int32)
         check(lib().dfrg_se_batch_variants(self.ptr, mode, v.ctypes.data_as(C.POINTER(C.c_int))))
         return v
+
+    def unroll(self, mode: int) -> int:
+        """Radial unroll of the paired evaluation loop a run in `mode` uses at the current shape."""
+        if mode == FP64:
+            return 1
+        u = C.c_int()
+        check(lib().dfrg_se_batch_unroll(self.ptr, C.byref(u)))
+        return u.value
 
     def set_shape(self, block=256, tile_evals=8192, grid=0):
         check(lib().dfrg_se_batch_set_shape(self.ptr, block, tile_evals, grid))

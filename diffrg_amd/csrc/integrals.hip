@@ -324,8 +324,10 @@ namespace
   // Sum over the block's share of (cos1, cos2) pairs and radial nodes of one tile.  Lanes own
   // pairs; with two or more pairs per lane, each radial step evaluates two pairs that share the
   // LDS reads of the q terms; with one pair per lane, lanes split the radial nodes and each step
-  // evaluates two of them.  Two independent chains per step hide the LDS latency.
-  template <int BLOCK, typename G>
+  // evaluates two of them.  Two independent chains per step hide the LDS latency.  UNROLL = 2
+  // takes two radial nodes per step of the paired loop (four chains), for launches with few waves
+  // per SIMD; the accumulation order, and so the result, is the same.
+  template <int BLOCK, int UNROLL, typename G>
   __device__ __forceinline__ double se_tile_sum(const G &g, int na, int l2, int nqt, const double2 *s_ang,
                                                 const double *s_w2, const double *s_qs, const double *s_qc,
                                                 const double *s_qt)
@@ -341,7 +343,19 @@ namespace
         const int prb = pr + pstride;
         const double2 a = s_ang[pr >> l2], b = s_ang[prb >> l2];
         double sa = 0.0, sb = 0.0;
-        for (int j = 0; j < nqt; ++j) {
+        int j = 0;
+        if constexpr (UNROLL == 2)
+          for (; j + 1 < nqt; j += 2) {
+            const double qs0 = s_qs[j], qc0 = s_qc[j], qt0 = s_qt[j];
+            const double qs1 = s_qs[j + 1], qc1 = s_qc[j + 1], qt1 = s_qt[j + 1];
+            const double ga0 = g(fma(qc0, a.x, qs0)), gb0 = g(fma(qc0, b.x, qs0));
+            const double ga1 = g(fma(qc1, a.x, qs1)), gb1 = g(fma(qc1, b.x, qs1));
+            sa = fma(qt0, ga0, sa);
+            sb = fma(qt0, gb0, sb);
+            sa = fma(qt1, ga1, sa);
+            sb = fma(qt1, gb1, sb);
+          }
+        for (; j < nqt; ++j) {
           const double qs = s_qs[j], qc = s_qc[j], qt = s_qt[j];
           const double ga = g(fma(qc, a.x, qs));
           const double gb = g(fma(qc, b.x, qs));
@@ -368,7 +382,7 @@ namespace
     return acc;
   }
 
-  template <int MODE, int BLOCK> __global__ __launch_bounds__(BLOCK) void se_kernel(SEArgs A)
+  template <int MODE, int BLOCK, int UNROLL> __global__ __launch_bounds__(BLOCK) void se_kernel(SEArgs A)
   {
     constexpr bool F64 = MODE == DFRG_FP64;
     __shared__ double s_tab64[F64 ? kMaxTab : 1];
@@ -465,7 +479,7 @@ namespace
       double acc;
       if (F64) {
         const auto g = [&](double pq2) { return se_G_fp64(s_tab64, T.ntab, T.c, T.inv_b, T.scale, k2, pq2); };
-        acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
+        acc = se_tile_sum<BLOCK, UNROLL>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
       } else {
         SEMixed m;
         m.tabA = s_tabA;
@@ -479,10 +493,10 @@ namespace
         m.imaxf = (float)(T.ntab - 2);
         if (T.narrow) { // tile-uniform
           const auto g = [&](double pq2) { return se_G_mixed<true>(m, pq2); };
-          acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
+          acc = se_tile_sum<BLOCK, UNROLL>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
         } else {
           const auto g = [&](double pq2) { return se_G_mixed<false>(m, pq2); };
-          acc = se_tile_sum<BLOCK>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
+          acc = se_tile_sum<BLOCK, UNROLL>(g, T.na, T.l2na, T.nqt, s_ang, s_w2, s_qs, s_qc, s_qt);
         }
       }
       const double part = dev::block_sum(acc, s_red);
@@ -503,14 +517,14 @@ namespace
     }
   }
 
-  template <int MODE> hipError_t launch_se(int block, int grid, hipStream_t s, const SEArgs &a)
+  template <int MODE, int UNROLL> hipError_t launch_se(int block, int grid, hipStream_t s, const SEArgs &a)
   {
     switch (block) {
-    case 64: hipLaunchKernelGGL((se_kernel<MODE, 64>), dim3(grid), dim3(64), 0, s, a); break;
-    case 128: hipLaunchKernelGGL((se_kernel<MODE, 128>), dim3(grid), dim3(128), 0, s, a); break;
-    case 256: hipLaunchKernelGGL((se_kernel<MODE, 256>), dim3(grid), dim3(256), 0, s, a); break;
-    case 512: hipLaunchKernelGGL((se_kernel<MODE, 512>), dim3(grid), dim3(512), 0, s, a); break;
-    case 1024: hipLaunchKernelGGL((se_kernel<MODE, 1024>), dim3(grid), dim3(1024), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((se_kernel<MODE, 64, UNROLL>), dim3(grid), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((se_kernel<MODE, 128, UNROLL>), dim3(grid), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((se_kernel<MODE, 256, UNROLL>), dim3(grid), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((se_kernel<MODE, 512, UNROLL>), dim3(grid), dim3(512), 0, s, a); break;
+    case 1024: hipLaunchKernelGGL((se_kernel<MODE, 1024, UNROLL>), dim3(grid), dim3(1024), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -634,12 +648,23 @@ namespace
     return a;
   }
 
+  // Radial unroll of the mixed paired loop: 2 when the launch puts at most 16 waves on a CU (4 per
+  // SIMD), where four independent evaluation chains per lane hide more latency than two; 1 for
+  // fuller launches, which hide it with waves.  Results do not depend on it.
+  int se_unroll(const dfrg_se_batch *b)
+  {
+    const long long cus = context().cu_count > 0 ? context().cu_count : 256;
+    const long long blocks = b->grid > 0 ? std::min(b->grid, b->n_tiles) : b->n_tiles;
+    return blocks * (b->block / 64) <= 16 * cus ? 2 : 1;
+  }
+
   int se_launch(dfrg_se_batch *b, int mode, double *d_out, hipStream_t s)
   {
     const SEArgs a = se_args(b, d_out);
     const int grid = b->grid > 0 ? std::min(b->grid, b->n_tiles) : b->n_tiles;
-    hipError_t e = mode == DFRG_FP64 ? launch_se<DFRG_FP64>(b->block, grid, s, a)
-                                     : launch_se<DFRG_MIXED>(b->block, grid, s, a);
+    hipError_t e = mode == DFRG_FP64    ? launch_se<DFRG_FP64, 1>(b->block, grid, s, a)
+                   : se_unroll(b) == 2 ? launch_se<DFRG_MIXED, 2>(b->block, grid, s, a)
+                                       : launch_se<DFRG_MIXED, 1>(b->block, grid, s, a);
     if (e != hipSuccess) {
       set_error("launch of se_kernel failed: %s", hipGetErrorString(e));
       return DFRG_ERR_HIP;
@@ -922,6 +947,13 @@ int dfrg_se_batch_variants(dfrg_se_batch *b, int mode, int *variant)
   DFRG_CHECK_ARG(mode == DFRG_FP64 || mode == DFRG_MIXED, "mode must be DFRG_FP64 or DFRG_MIXED");
   for (int i = 0; i < b->n_int; ++i)
     variant[i] = mode == DFRG_FP64 ? 0 : b->narrow[i] ? 2 : 1;
+  return DFRG_OK;
+}
+
+int dfrg_se_batch_unroll(dfrg_se_batch *b, int *unroll)
+{
+  DFRG_CHECK_ARG(b && unroll, "NULL argument");
+  *unroll = se_unroll(b);
   return DFRG_OK;
 }
 

@@ -176,6 +176,10 @@ int dfrg_se_batch_evals(dfrg_se_batch *b, unsigned long long *evals);
  * stays below 2^30, which rounds the exponent with one FMA and drops a guard (within the mixed
  * tolerance of form 1).  The choice depends on the integral alone, not on the batch. */
 int dfrg_se_batch_variants(dfrg_se_batch *b, int mode, int *variant);
+/* Radial unroll of the mixed path's paired evaluation loop at the current shape: 2 (two radial
+ * nodes per step) when the launch puts at most 16 waves on a CU, else 1.  The plain path always
+ * uses 1.  Results do not depend on it. */
+int dfrg_se_batch_unroll(dfrg_se_batch *b, int *unroll);
 
 /* Finite-T two-propagator batch (IntegratorFiniteTq0GPU form, integrator_finiteTq0_gpu.hh:20-44):
  *   I_i = sum_{j,n} el(q_j) wq_j [ wm_n (f(q_j,wn) + f(q_j,-wn)) ] + T_i sum_j el(q_j) wq_j f(q_j,0),

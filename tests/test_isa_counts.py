@@ -16,20 +16,34 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_se_kernel_counts_match_bench():
     with tempfile.TemporaryDirectory() as d:
         loops = I.inner_loops(I.compile_asm(os.path.join(ROOT, "diffrg_amd", "csrc", "integrals.hip"), d))
-    for mode in (0, 1):
-        for block in (64, 128, 256, 512, 1024):
-            name = f"se_kernelILi{mode}ELi{block}E"
-            # the integrand loops: the ones with transcendental instructions and more than one
-            # table lookup; the paired loop reads the radial terms once for two evaluations.  The
-            # mixed kernel holds both integrand forms; the narrow one has fewer FP64 instructions.
-            body = sorted((c for n, c in loops if name in n and c["trans"] >= 8), key=lambda c: (c["f64"], c["lds"]))
-            variants = (0,) if mode == 0 else (2, 1)
-            assert len(body) == 2 * len(variants), (name, body)
-            for j, c in enumerate(body):
-                variant, loop = variants[j // 2], ("pairs", "single")[j % 2]
-                na = 64 if loop == "pairs" else 8
-                assert bench.se_loop(na, block) == loop
-                assert bench.se_ops(variant, na, block) == (c["flop"] / 2, c["valu"] / 2), (name, variant, loop, c)
+    kernels = [(0, b, 1) for b in (64, 128, 256, 512, 1024)] + \
+              [(1, b, u) for b in (64, 128, 256, 512, 1024) for u in (1, 2)]
+    for mode, block, unroll in kernels:
+        name = f"se_kernelILi{mode}ELi{block}ELi{unroll}E"
+        # the integrand loops: the ones with transcendental instructions (4 per FP64 evaluation, 5
+        # per mixed one); the mixed kernel holds both integrand forms, told apart by their FP64
+        # instructions per evaluation (narrow 20, general 21).  Per form: the paired loop reads
+        # three radial terms per radial node (fewer LDS reads than the single loop per two
+        # evaluations), and with unroll 2 a paired loop of four evaluations per step comes first.
+        body = [c for n, c in loops if name in n and c["trans"] >= 8]
+        per_trans = 4 if mode == 0 else 5
+        forms = {}
+        for c in body:
+            evals = c["trans"] // per_trans
+            form = 0 if mode == 0 else (2 if c["f64"] // evals == 20 else 1)
+            forms.setdefault(form, []).append((evals, c["lds"], c))
+        assert sorted(forms) == ([0] if mode == 0 else [1, 2]), (name, body)
+        for form, ls in forms.items():
+            ls.sort(key=lambda t: (-t[0], t[1]))
+            want = ["pairs2", "pairs", "single"] if unroll == 2 else ["pairs", "single"]
+            assert [t[0] for t in ls] == ([4, 2, 2] if unroll == 2 else [2, 2]), (name, ls)
+            for kind, (evals, _, c) in zip(want, ls):
+                per_eval = (c["flop"] / evals, c["valu"] / evals)
+                if kind == "pairs" and unroll == 2:
+                    continue  # the remainder step of an odd radial count, not priced
+                na = 8 if kind == "single" else 64
+                assert bench.se_loop(na, block, unroll) == kind
+                assert bench.se_ops(form, na, block, unroll) == per_eval, (name, form, kind, c)
 
 
 def test_se_loop_choice():
