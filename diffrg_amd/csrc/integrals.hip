@@ -344,16 +344,21 @@ namespace
         const double2 a = s_ang[pr >> l2], b = s_ang[prb >> l2];
         double sa = 0.0, sb = 0.0;
         int j = 0;
-        if constexpr (UNROLL == 2)
-          for (; j + 1 < nqt; j += 2) {
-            const double qs0 = s_qs[j], qc0 = s_qc[j], qt0 = s_qt[j];
-            const double qs1 = s_qs[j + 1], qc1 = s_qc[j + 1], qt1 = s_qt[j + 1];
-            const double ga0 = g(fma(qc0, a.x, qs0)), gb0 = g(fma(qc0, b.x, qs0));
-            const double ga1 = g(fma(qc1, a.x, qs1)), gb1 = g(fma(qc1, b.x, qs1));
-            sa = fma(qt0, ga0, sa);
-            sb = fma(qt0, gb0, sb);
-            sa = fma(qt1, ga1, sa);
-            sb = fma(qt1, gb1, sb);
+        if constexpr (UNROLL > 1)
+          for (; j + UNROLL - 1 < nqt; j += UNROLL) {
+            double ga[UNROLL], gb[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+              const double qs = s_qs[j + u], qc = s_qc[j + u];
+              ga[u] = g(fma(qc, a.x, qs));
+              gb[u] = g(fma(qc, b.x, qs));
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+              const double qt = s_qt[j + u];
+              sa = fma(qt, ga[u], sa);
+              sb = fma(qt, gb[u], sb);
+            }
           }
         for (; j < nqt; ++j) {
           const double qs = s_qs[j], qc = s_qc[j], qt = s_qt[j];
