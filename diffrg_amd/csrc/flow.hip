@@ -9,15 +9,15 @@
 // with the closed-form bosonic Matsubara sum, cf. Examples/ONfiniteT/flows/V/V.kernel) per grid
 // point and RK stage.  u', u'' are second-order finite differences (one-sided at the ends).
 //
-// Device layout: every resident flow owns 4 n doubles in one HBM pool (u, RK accumulator, two
-// stage buffers) and a slot in SoA descriptor arrays.  A point table lists (slot, i) of every
-// resident grid point; one wave evaluates four points at a time (one Gauss-Legendre node per lane,
-// the eight node sums reduced together across lanes) over a run of consecutive points, so all
-// resident flows advance together in each launch.  One RK step is four launches, one per stage;
-// steps_per_launch steps are captured into one hipGraph.  A flow's step index is
-// the launch's base step (written by the host before each graph launch) plus the step within the
-// graph minus the base step at its admission, so flows admitted at different times share the
-// graph and a finished flow simply stops updating.  The graph's last node advances the device copy
+// Device layout: every resident flow owns 2 n doubles in one HBM pool (u after even and after odd
+// steps) and a slot in SoA descriptor arrays.  The host cuts the resident flows into runs of
+// consecutive grid points; one launch advances every run by a whole RK4 step, one workgroup per
+// run, evaluating the stages outside the run as far as the next stages' stencils reach
+// (flow_step_kernel), each wave four grid points at a time (one Gauss-Legendre node per lane, the
+// eight node sums reduced together across lanes).  steps_per_launch steps are captured into one hipGraph.  A flow's step
+// index is the launch's base step plus the step within the graph minus the base step at its
+// admission, so flows admitted at different times share the graph and a finished flow simply
+// stops updating.  The graph's last node advances the device copy
 // of the base step, which the host mirrors.  Between graph launches the host retires
 // finished and non-finite flows and admits queued ones while their HBM footprint fits the budget.
 //
@@ -41,30 +41,33 @@ using namespace dfrg;
 
 namespace
 {
-  constexpr int kFlowBlock = 256;
-  constexpr int kFlowGrid = 1024; // waves = kFlowGrid * kFlowBlock / 64, grid-stride over chunks
+  constexpr int kFlowGrid = 1024; // workgroups per step launch, grid-stride over runs
   constexpr double kPi = 3.14159265358979323846;
 
-  // One resident grid point: with its flow's slot, state offset, size and admission step, a
-  // lane's first load gives the addresses of everything else it reads, so the rest of its data
-  // (state values, per-flow constants, k) comes in one second round of loads.
-  struct FlowPoint {
-    int slot, i, off, n, admit, pad;
+  // A run of consecutive grid points [a, b) of one resident flow, which one workgroup advances by
+  // a whole RK4 step.  Runs are at least 2 points long (the one-sided stencils at the grid ends
+  // then find their points in the previous stage's range) and at most kMaxRun.
+  struct FlowRun {
+    int slot, a, b, n, off, admit;
   };
+
+  constexpr int kRunBlock = 512;                  // threads per workgroup: 8 waves share a run
+  constexpr int kHalo = 3;                        // stages 1-3 are also evaluated this far outside the run
+  constexpr int kMaxRun = 256;                    // longest run
+  constexpr int kSpan = kMaxRun + 2 * (kHalo + 1); // the run and the stencil reach of its stage 1
 
   struct FlowDev {
     int *status, *bad;    // per slot; status 0 idle, 1 running (host-owned)
     double *h, *T, *N;    // per slot: grid spacing, temperature, O(N)
-    FlowPoint *points;    // every resident grid point
-    int *n_points;        // device copy of the point count
-    int *base;                           // global step index of the first step of this launch
-    double *pool;                        // state: u | acc | y0 | y1 per flow
-    const double *A, *D;                 // node factors (header comment)
-    const double *kt;                    // k at (step, stage point): 3 per step, t, t + dt/2, t + dt
+    FlowRun *runs;        // every resident flow cut into runs
+    int *n_runs;          // device copy of the run count
+    int *base;            // global step index of the first step of this launch
+    double *pool;         // state: u at even steps | u at odd steps, per flow
+    const double *A, *D;  // node factors (header comment)
+    const double *kt;     // k at (step, stage point): 3 per step, t, t + dt/2, t + dt
     int nx, n_steps;
     double dt;
   };
-
   // S2(E) = coth(E/2T)/(4E^3) + csch^2(E/2T)/(8 T E^2) with e = exp(-E/T), coth = (1+e)/(1-e),
   // csch^2 = 4e/(1-e)^2, put over one denominator:
   //   S2 = (2 T (1+e)(1-e) + 4 e E) / (8 T E^3 (1-e)^2),   E = sqrt(E2).
@@ -84,117 +87,139 @@ namespace
     return num * inv_8T * (y * y * y) * (r * r);
   }
 
-  __global__ __launch_bounds__(kFlowBlock) void flow_stage_kernel(FlowDev F, int stage, int local)
+  // One RK4 step of every resident flow.  A workgroup takes a run [a, b) of one flow and keeps the
+  // flow's values at the points [a - kHalo - 1, b + kHalo] in LDS.  Stage s (halo w = 4 - s)
+  // evaluates the right-hand side at the points of [a - w, b + w) inside the grid, from the
+  // previous stage's values at their neighbours, which the previous stage computed on a range one
+  // point wider: points outside the run are evaluated again by the workgroup that owns them, with
+  // the same inputs and arithmetic, so every point's result is what a stage-by-stage sweep gives
+  // and does not depend on how flows are cut into runs.  Within a stage the waves share the points
+  // four at a time (lanes over Gauss-Legendre nodes).  u is read from the buffer of this step's
+  // parity and written to the other, so neighbouring runs read the old u.
+  __global__ __launch_bounds__(kRunBlock) void flow_step_kernel(FlowDev F, int local)
   {
     __shared__ double s_exp2[64]; // 2^(j/64)
-    for (int j = threadIdx.x; j < 64; j += kFlowBlock) s_exp2[j] = exp2(j * (1.0 / 64.0));
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int wave_id = (blockIdx.x * kFlowBlock + threadIdx.x) >> 6;
-    const int n_waves = (gridDim.x * kFlowBlock) >> 6;
-    const int P = *F.n_points;
+    __shared__ double s_u[kSpan], s_y[kSpan], s_acc[kSpan], s_f[kSpan];
+    __shared__ double s_m2pi[kSpan], s_m2sig[kSpan], s_up[kSpan], s_upp[kSpan];
+    for (int j = threadIdx.x; j < 64; j += kRunBlock) s_exp2[j] = exp2(j * (1.0 / 64.0));
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int n_wv = kRunBlock / 64;
+    const int R = *F.n_runs;
     const int base = *F.base + local;
-    // A wave takes a chunk of consecutive points, sized so that all waves get work: one point each
-    // while points are fewer than waves, up to 64 when they are many.  Lane l gathers the data of
-    // point w0 + l (its record, then neighbours in the stage input, its own state and
-    // accumulator, the flow's constants and k) in two rounds of loads; the wave then evaluates
-    // the points' right-hand sides with the point's values broadcast from its lane (lanes over
-    // Gauss-Legendre nodes), and lane l applies the RK stage update of its point.
-    const int chunk = min(64, max(1, (P + n_waves - 1) / n_waves));
-    for (int w0 = wave_id * chunk; w0 < P; w0 += n_waves * chunk) {
-      const int cnt = min(chunk, P - w0);
-      int s = 0, i = 0, n = 0, off = 0, active = 0;
-      double up = 0.0, upp = 0.0, rho = 0.0, m2pi = 0.0, m2sig = 0.0, T = 1.0, Nn = 0.0, k = 0.0;
-      double u_i = 0.0, acc_i = 0.0; // this point's state and RK accumulator
-      if (lane < cnt) {
-        const FlowPoint pt = F.points[w0 + lane];
-        s = pt.slot;
-        i = pt.i;
-        n = pt.n;
-        off = pt.off;
-        const int step = base - pt.admit;
-        // a listed point's state range is allocated while its flow is resident, so the loads
-        // below stay in it; a flow past its last step is left alone
-        if (step >= 0 && step < F.n_steps) {
-          const double *u = F.pool + off;
-          const double *in = stage == 1 ? u : (stage == 3 ? u + 3 * n : u + 2 * n);
-          active = F.status[s] == 1;
-          const double h = F.h[s];
-          T = F.T[s];
-          Nn = F.N[s];
-          k = F.kt[3 * step + (stage == 1 ? 0 : (stage == 4 ? 2 : 1))];
-          u_i = u[i];
-          if (stage > 1) acc_i = u[n + i];
-          if (i == 0) {
-            up = (-3.0 * in[0] + 4.0 * in[1] - in[2]) / (2.0 * h);
-            upp = (in[0] - 2.0 * in[1] + in[2]) / (h * h);
-          } else if (i == n - 1) {
-            up = (3.0 * in[n - 1] - 4.0 * in[n - 2] + in[n - 3]) / (2.0 * h);
-            upp = (in[n - 1] - 2.0 * in[n - 2] + in[n - 3]) / (h * h);
-          } else {
-            up = (in[i + 1] - in[i - 1]) / (2.0 * h);
-            upp = (in[i + 1] - 2.0 * in[i] + in[i - 1]) / (h * h);
-          }
-          rho = i * h;
-          m2pi = in[i];
-          m2sig = in[i] + 2.0 * rho * up;
-        }
+    // node factors, one node per lane when there are at most 64 (the rest loop over nodes)
+    const bool lane_nodes = F.nx <= 64;
+    double A_l = 0.0, D_l = 0.0;
+    if (lane_nodes && lane < F.nx) {
+      A_l = F.A[lane];
+      D_l = F.D[lane];
+    }
+    __syncthreads();
+    for (int r = blockIdx.x; r < R; r += gridDim.x) {
+      const FlowRun run = F.runs[r]; // workgroup-uniform
+      const int step = base - run.admit;
+      // a flow outside its steps, or idle, is left alone (workgroup-uniform)
+      if (step < 0 || step >= F.n_steps || F.status[run.slot] != 1) continue;
+      const int n = run.n, a = run.a, b = run.b;
+      const double h = F.h[run.slot], T = F.T[run.slot], Nn = F.N[run.slot], dt = F.dt;
+      const double k_t = F.kt[3 * step], k_half = F.kt[3 * step + 1], k_dt = F.kt[3 * step + 2];
+      const double inv_T = 1.0 / T, inv_8T = 0.125 * inv_T;
+      const double *u_old = F.pool + run.off + (step & 1) * n;
+      double *u_new = F.pool + run.off + ((step + 1) & 1) * n;
+      const int x0 = a - kHalo - 1; // grid point of LDS entry 0
+      for (int i = tid; i < b - a + 2 * (kHalo + 1); i += kRunBlock) {
+        const int x = x0 + i;
+        const double v = x >= 0 && x < n ? u_old[x] : 0.0;
+        s_u[i] = v;
+        s_y[i] = v;
       }
-      // Points go four at a time: their eight node sums (two threshold integrals each) are
-      // reduced together by dev::wave_sum8; integral kind of point pp is vector 2 pp + kind.
-      double f_lane = 0.0;
-      for (int g0 = 0; g0 < cnt; g0 += 4) {
-        double v[8];
+      __syncthreads();
+      for (int stage = 1; stage <= 4; ++stage) {
+        const int w = 4 - stage;
+        const int p0 = max(a - w, 0) - x0, p1 = min(b + w, n) - x0; // evaluated entries [p0, p1)
+        const double k = stage == 1 ? k_t : (stage == 4 ? k_dt : k_half);
+        // stencil terms of the evaluated points from this stage's input
+        for (int p = p0 + tid; p < p1; p += kRunBlock) {
+          const int x = x0 + p;
+          const double yv = s_y[p];
+          double up, upp;
+          if (x == 0) {
+            up = (-3.0 * yv + 4.0 * s_y[p + 1] - s_y[p + 2]) / (2.0 * h);
+            upp = (yv - 2.0 * s_y[p + 1] + s_y[p + 2]) / (h * h);
+          } else if (x == n - 1) {
+            up = (3.0 * yv - 4.0 * s_y[p - 1] + s_y[p - 2]) / (2.0 * h);
+            upp = (yv - 2.0 * s_y[p - 1] + s_y[p - 2]) / (h * h);
+          } else {
+            up = (s_y[p + 1] - s_y[p - 1]) / (2.0 * h);
+            upp = (s_y[p + 1] - 2.0 * yv + s_y[p - 1]) / (h * h);
+          }
+          const double rho = x * h;
+          s_up[p] = up;
+          s_upp[p] = upp;
+          s_m2pi[p] = yv;
+          s_m2sig[p] = yv + 2.0 * rho * up;
+        }
+        __syncthreads();
+        // right-hand sides, four points per wave at a time: their eight node sums (two threshold
+        // integrals each) are reduced together by dev::wave_sum8 (kind of point pp: vector 2 pp + kind)
+        const double k2 = k * k, k5 = k * k * (k * k) * k;
+        for (int g0 = p0 + 4 * wave; g0 < p1; g0 += 4 * n_wv) {
+          double v[8];
 #pragma unroll
-        for (int pp = 0; pp < 4; ++pp) {
-          const int p = g0 + pp;
-          double ipi = 0.0, isg = 0.0;
-          if (p < cnt && __builtin_amdgcn_readlane(active, p)) { // wave-uniform
-            const double pT = dev::lane_value(T, p), pk = dev::lane_value(k, p);
-            const double pm2pi = dev::lane_value(m2pi, p), pm2sig = dev::lane_value(m2sig, p);
-            // two threshold integrals, lane j evaluates node j (d = 3 radial element)
-            const double k2 = pk * pk, inv_T = 1.0 / pT, inv_8T = 0.125 * inv_T;
-            for (int j = lane; j < F.nx; j += 64) {
-              const double a = F.A[j], dd = F.D[j];
-              ipi = fma(dd, bose_S2(fma(k2, a, pm2pi), pT, inv_8T, inv_T, s_exp2), ipi);
-              isg = fma(dd, bose_S2(fma(k2, a, pm2sig), pT, inv_8T, inv_T, s_exp2), isg);
+          for (int pp = 0; pp < 4; ++pp) {
+            const int p = g0 + pp;
+            double ipi = 0.0, isg = 0.0;
+            if (p < p1) { // wave-uniform
+              const double pm2pi = s_m2pi[p], pm2sig = s_m2sig[p];
+              // two threshold integrals, lane j evaluates node j (d = 3 radial element)
+              if (lane_nodes) {
+                if (lane < F.nx) {
+                  ipi = fma(D_l, bose_S2(fma(k2, A_l, pm2pi), T, inv_8T, inv_T, s_exp2), 0.0);
+                  isg = fma(D_l, bose_S2(fma(k2, A_l, pm2sig), T, inv_8T, inv_T, s_exp2), 0.0);
+                }
+              } else {
+                for (int j = lane; j < F.nx; j += 64) {
+                  const double an = F.A[j], dd = F.D[j];
+                  ipi = fma(dd, bose_S2(fma(k2, an, pm2pi), T, inv_8T, inv_T, s_exp2), ipi);
+                  isg = fma(dd, bose_S2(fma(k2, an, pm2sig), T, inv_8T, inv_T, s_exp2), isg);
+                }
+              }
+            }
+            v[2 * pp] = ipi;
+            v[2 * pp + 1] = isg;
+          }
+          const double sum = dev::wave_sum8(v);
+#pragma unroll
+          for (int pp = 0; pp < 4; ++pp) {
+            const int p = g0 + pp;
+            if (p >= p1) continue; // wave-uniform
+            const double ipi = k5 * dev::lane_value(sum, dev::sum8_lane(2 * pp));
+            const double isg = k5 * dev::lane_value(sum, dev::sum8_lane(2 * pp + 1));
+            if (lane == 0) {
+              const double up = s_up[p], upp = s_upp[p], rho = (x0 + p) * h;
+              s_f[p] = 0.5 * ((Nn - 1.0) * up * ipi + (3.0 * up + 2.0 * rho * upp) * isg);
             }
           }
-          v[2 * pp] = ipi;
-          v[2 * pp + 1] = isg;
         }
-        const double y = dev::wave_sum8(v);
-#pragma unroll
-        for (int pp = 0; pp < 4; ++pp) {
-          const int p = g0 + pp;
-          if (p >= cnt || !__builtin_amdgcn_readlane(active, p)) continue; // wave-uniform
-          const double pk = dev::lane_value(k, p);
-          const double k5 = pk * pk * (pk * pk) * pk;
-          const double ipi = k5 * dev::lane_value(y, dev::sum8_lane(2 * pp));
-          const double isg = k5 * dev::lane_value(y, dev::sum8_lane(2 * pp + 1));
-          if (lane == p) f_lane = 0.5 * ((Nn - 1.0) * up * ipi + (3.0 * up + 2.0 * rho * upp) * isg);
+        __syncthreads();
+        // the RK4 stage update of the evaluated points
+        for (int p = p0 + tid; p < p1; p += kRunBlock) {
+          const double f = s_f[p], u = s_u[p];
+          if (stage == 1) {
+            s_acc[p] = f;
+            s_y[p] = u + 0.5 * dt * f;
+          } else if (stage == 2) {
+            s_acc[p] = s_acc[p] + 2.0 * f;
+            s_y[p] = u + 0.5 * dt * f;
+          } else if (stage == 3) {
+            s_acc[p] = s_acc[p] + 2.0 * f;
+            s_y[p] = u + dt * f;
+          } else {
+            const double un = u + dt / 6.0 * (s_acc[p] + f);
+            u_new[x0 + p] = un;
+            if (!isfinite(un)) atomicOr(&F.bad[run.slot], 1);
+          }
         }
-      }
-      if (lane < cnt && active) {
-        // only this lane reads or writes u[i] and acc[i] in this launch; the neighbours other
-        // lanes read are in the stage's input buffer, which this stage does not write
-        double *u = F.pool + off;
-        double *acc = u + n, *y0 = u + 2 * n, *y1 = u + 3 * n;
-        const double f = f_lane, dt = F.dt;
-        if (stage == 1) {
-          acc[i] = f;
-          y0[i] = u_i + 0.5 * dt * f;
-        } else if (stage == 2) {
-          acc[i] = acc_i + 2.0 * f;
-          y1[i] = u_i + 0.5 * dt * f;
-        } else if (stage == 3) {
-          acc[i] = acc_i + 2.0 * f;
-          y0[i] = u_i + dt * f;
-        } else {
-          const double un = u_i + dt / 6.0 * (acc_i + f);
-          u[i] = un;
-          if (!isfinite(un)) atomicOr(&F.bad[s], 1);
-        }
+        __syncthreads();
       }
     }
   }
@@ -230,7 +255,8 @@ struct dfrg_fs {
   std::map<long long, FlowRecord> flows;
   std::deque<long long> queue;
   long long next_id = 1;
-  bool points_dirty = true;
+  bool runs_dirty = true;
+  size_t runs_cap = 0; // FlowRun entries allocated on the device
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -279,8 +305,7 @@ namespace
     DFRG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     DFRG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     for (int st = 0; st < fs->steps_per_launch; ++st)
-      for (int stage = 1; stage <= 4; ++stage)
-        hipLaunchKernelGGL(flow_stage_kernel, dim3(kFlowGrid), dim3(kFlowBlock), 0, cs, fs->dev, stage, st);
+      hipLaunchKernelGGL(flow_step_kernel, dim3(kFlowGrid), dim3(kRunBlock), 0, cs, fs->dev, st);
     hipLaunchKernelGGL(flow_base_kernel, dim3(1), dim3(64), 0, cs, fs->dev.base, fs->steps_per_launch);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cs, &g);
@@ -307,19 +332,22 @@ namespace
       if (!done && !bad[s]) continue;
       const int n = r.desc.n_grid;
       r.u.resize(n);
-      DFRG_HIP(hipMemcpy(r.u.data(), fs->dev.pool + r.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      // u after the last step taken sits in the buffer of that step count's parity
+      const int taken = std::min(fs->base - r.admit, fs->n_steps);
+      DFRG_HIP(hipMemcpy(r.u.data(), fs->dev.pool + r.off + (taken & 1) * n, sizeof(double) * n,
+                         hipMemcpyDeviceToHost));
       r.state = bad[s] ? 3 : 2;
       r.t_done = t;
       if (r.state == 2) {
         fs->latencies.push_back(r.t_done - r.t_submit);
         ++fs->finished;
       }
-      fs_free_range(fs, r.off, 4 * n);
+      fs_free_range(fs, r.off, 2 * n);
       const int zero = 0;
       DFRG_HIP(hipMemcpy(fs->dev.status + s, &zero, sizeof(int), hipMemcpyHostToDevice));
       fs->slot_flow[s] = -1;
       r.slot = -1;
-      fs->points_dirty = true;
+      fs->runs_dirty = true;
     }
     return DFRG_OK;
   }
@@ -337,13 +365,13 @@ namespace
           break;
         }
       if (slot < 0) break;
-      const int off = fs_alloc_range(fs, 4 * n);
+      const int off = fs_alloc_range(fs, 2 * n);
       if (off < 0) break; // does not fit the HBM budget now: wait for retirements (FIFO)
       fs->queue.pop_front();
-      std::vector<double> init(4 * n, 0.0);
+      std::vector<double> init(2 * n, 0.0); // u at step 0 (even buffer); odd buffer written by step 0
       const double h = r.desc.rho_max / (n - 1);
       for (int i = 0; i < n; ++i) init[i] = r.desc.m2_0 + r.desc.lambda / 2.0 * (i * h);
-      DFRG_HIP(hipMemcpy(fs->dev.pool + off, init.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.pool + off, init.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
       const int one = 1, zero = 0;
       DFRG_HIP(hipMemcpy(fs->dev.bad + slot, &zero, sizeof(int), hipMemcpyHostToDevice));
       DFRG_HIP(hipMemcpy(fs->dev.h + slot, &h, sizeof(double), hipMemcpyHostToDevice));
@@ -355,20 +383,46 @@ namespace
       r.off = off;
       r.admit = fs->base;
       r.state = 1;
-      fs->points_dirty = true;
+      fs->runs_dirty = true;
     }
-    if (fs->points_dirty) {
-      std::vector<FlowPoint> pts;
+    if (fs->runs_dirty) {
+      // cut every resident flow into runs of at most L points, with the smallest L (>= 8) whose
+      // runs fit the workgroups resident at once (two per CU), so that a step takes one round of
+      // workgroups (a flow of n >= 4 points then has runs of >= 2 points)
+      std::vector<int> sizes;
+      long long P = 0;
+      for (int s = 0; s < fs->max_resident; ++s)
+        if (fs->slot_flow[s] >= 0) {
+          sizes.push_back(fs->flows[fs->slot_flow[s]].desc.n_grid);
+          P += sizes.back();
+        }
+      const long long slots = 2LL * (context().cu_count > 0 ? context().cu_count : 256);
+      int L = (int)std::min<long long>(kMaxRun, std::max<long long>(8, (P + slots - 1) / slots));
+      auto count = [&](int len) {
+        long long c = 0;
+        for (int n : sizes) c += (n + len - 1) / len;
+        return c;
+      };
+      while (L < kMaxRun && count(L) > slots) ++L;
+      std::vector<FlowRun> runs;
       for (int s = 0; s < fs->max_resident; ++s) {
         if (fs->slot_flow[s] < 0) continue;
         const FlowRecord &r = fs->flows[fs->slot_flow[s]];
-        const int n = r.desc.n_grid;
-        for (int i = 0; i < n; ++i) pts.push_back(FlowPoint{s, i, r.off, n, r.admit, 0});
+        const int n = r.desc.n_grid, cnt = (n + L - 1) / L, len = n / cnt, extra = n % cnt;
+        for (int j = 0, a = 0; j < cnt; ++j) {
+          const int b = a + len + (j < extra ? 1 : 0);
+          runs.push_back(FlowRun{s, a, b, n, r.off, r.admit});
+          a = b;
+        }
       }
-      const int P = (int)pts.size();
-      if (P) DFRG_HIP(hipMemcpy(fs->dev.points, pts.data(), sizeof(FlowPoint) * P, hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.n_points, &P, sizeof(int), hipMemcpyHostToDevice));
-      fs->points_dirty = false;
+      const int R = (int)runs.size();
+      if ((size_t)R > fs->runs_cap) {
+        set_error("internal: %d flow runs exceed the %zu allocated", R, fs->runs_cap);
+        return DFRG_ERR_RUNTIME;
+      }
+      if (R) DFRG_HIP(hipMemcpy(fs->dev.runs, runs.data(), sizeof(FlowRun) * R, hipMemcpyHostToDevice));
+      DFRG_HIP(hipMemcpy(fs->dev.n_runs, &R, sizeof(int), hipMemcpyHostToDevice));
+      fs->runs_dirty = false;
     }
     return DFRG_OK;
   }
@@ -383,7 +437,7 @@ namespace
 
 extern "C" {
 
-size_t dfrg_fs_flow_bytes(int n_grid) { return (size_t)4 * n_grid * sizeof(double); }
+size_t dfrg_fs_flow_bytes(int n_grid) { return (size_t)2 * n_grid * sizeof(double); }
 
 int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_resident, int nx, int n_steps,
                    double k_end, int steps_per_launch)
@@ -432,7 +486,9 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
     nodes[2 * nx + 3 * st + 1] = std::exp(-(t + 0.5 * d.dt));
     nodes[2 * nx + 3 * st + 2] = std::exp(-(t + d.dt));
   }
-  const size_t pts_bytes = sizeof(FlowPoint) * std::min(max_points, fs->pool_doubles / 4 + 1);
+  // runs: at most one per 4 points of each resident flow (n >= 4, L >= 8), plus one per flow
+  fs->runs_cap = std::min(max_points, fs->pool_doubles / 4 + (size_t)max_resident + 1);
+  const size_t runs_bytes = sizeof(FlowRun) * fs->runs_cap;
 #define FS_ALLOC(ptr, bytes)                                                                                   \
   do {                                                                                                         \
     if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess || hipMemset((ptr), 0, (bytes)) != hipSuccess) {     \
@@ -446,8 +502,8 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
   FS_ALLOC(d.h, sizeof(double) * max_resident);
   FS_ALLOC(d.T, sizeof(double) * max_resident);
   FS_ALLOC(d.N, sizeof(double) * max_resident);
-  FS_ALLOC(d.points, pts_bytes);
-  FS_ALLOC(d.n_points, sizeof(int));
+  FS_ALLOC(d.runs, runs_bytes);
+  FS_ALLOC(d.n_runs, sizeof(int));
   FS_ALLOC(d.base, sizeof(int));
   FS_ALLOC(fs->d_nodes, sizeof(double) * nodes.size());
   FS_ALLOC(d.pool, sizeof(double) * fs->pool_doubles);
@@ -479,7 +535,7 @@ int dfrg_fs_destroy(dfrg_fs *fs)
   if (fs->stream) hipStreamDestroy(fs->stream);
   FlowDev &d = fs->dev;
   for (void *p : {(void *)d.status, (void *)d.bad, (void *)d.h,
-                  (void *)d.T, (void *)d.N, (void *)d.points, (void *)d.n_points, (void *)d.base, (void *)d.pool,
+                  (void *)d.T, (void *)d.N, (void *)d.runs, (void *)d.n_runs, (void *)d.base, (void *)d.pool,
                   (void *)fs->d_nodes})
     if (p) hipFree(p);
   delete fs;
