@@ -24,8 +24,8 @@ def run_all(fs, ids, max_iter=100000):
     return out
 
 
-def oracle_flow(qp, n, T, n_steps, k_end, m2_0=0.2, lam=1.0):
-    x, w = qp.get_points(64), qp.get_weights(64)
+def oracle_flow(qp, n, T, n_steps, k_end, m2_0=0.2, lam=1.0, nx=64):
+    x, w = qp.get_points(nx), qp.get_weights(nx)
     fp = O.flow_params(n, 4.0, T, 0.0, 4.0, m2_0, lam)
     return O.flow_lpa(fp, x, w, 20.0, 1.0, k_end, n_steps)
 
@@ -41,6 +41,20 @@ def test_single_flow_vs_oracle(gpu, qp):
     print(f"  max |u - u_oracle| = {err:.3e}, latency {lat:.3f} s", flush=True)
     assert err <= TOL_FLOW
     assert u[0] > 0.2  # fluctuations raise the mass in the symmetric phase
+
+
+@pytest.mark.parametrize("nx", [24, 96])
+def test_flow_node_counts(gpu, qp, nx):
+    # fewer nodes than lanes, and more (the kernel's loop over nodes)
+    fs = gpu.FlowServer(qp, hbm_budget=1 << 20, max_resident=2, nx=nx, n_steps=300, k_end=0.2,
+                        steps_per_launch=20)
+    print(f"[case] flow n=40 T=0.15 with {nx} nodes, 300 RK4 steps", flush=True)
+    fid = fs.submit(40, 0.15)
+    u, _ = run_all(fs, [fid])[fid]
+    ref = oracle_flow(qp, 40, 0.15, 300, 0.2, nx=nx)
+    err = np.max(np.abs(u - ref))
+    print(f"  max |u - u_oracle| = {err:.3e}", flush=True)
+    assert err <= TOL_FLOW
 
 
 def test_mix_equals_alone_and_admission(gpu, qp):
