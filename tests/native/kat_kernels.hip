@@ -4,6 +4,8 @@
 // integrator_finiteTq0_cpu.cc) and the three built-in integrands written as ordinary KERNELs
 // with regulators and LinearInterpolator1D.  Each function returns 0, or -1 (invalid argument) /
 // -2 (runtime error) with the exception message in kat_last_error().
+#include <thread>
+
 #include <DiFfRG/common/math.hh>
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
 #include <DiFfRG/discretization/grid/coordinates.hh>
@@ -424,6 +426,49 @@ int kat_radial_complex_float(int nx, float x_extent, float k, const double *c, f
     out[1] = a.im;
     out[2] = b.re;
     out[3] = b.im;
+  });
+}
+
+// Concurrent callers, as the reference's TBB get_data runs get() (physics/utils.hh:205-220): n_thr
+// host threads share one radial and one two-angle integrator; thread t evaluates ks[i] for
+// i = t, t + n_thr, ..., alternating get() and request().get().  out_conc[i] (radial) and
+// out_conc[n + i] (two-angle) against the same calls made one after another in out_seq.
+int kat_concurrent(int n_thr, const double *ks, int n, const double *c, const double *a, const double *bc,
+                   double *out_conc, double *out_seq)
+{
+  return guard([&] {
+    IntegratorGPU<4, double, PolyRadial> rad(provider(), 64, 1.5);
+    Integrator4D2AngGPU<double, Poly2Ang> ang(provider(), {{48, 12, 12}}, 1.5);
+    auto eval_rad = [&](double k, bool req) {
+      return req ? rad.request(k, 0.25, c[0], c[1], c[2], c[3], c[4], c[5]).get()
+                 : rad.get(k, 0.25, c[0], c[1], c[2], c[3], c[4], c[5]);
+    };
+    auto eval_ang = [&](double k, bool req) {
+      return req ? ang.request(k, 0.5, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], bc[0], bc[1],
+                               bc[2], bc[3]).get()
+                 : ang.get(k, 0.5, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], bc[0], bc[1], bc[2],
+                           bc[3]);
+    };
+    for (int i = 0; i < n; ++i) {
+      out_seq[i] = eval_rad(ks[i], false);
+      out_seq[n + i] = eval_ang(ks[i], false);
+    }
+    std::vector<std::thread> th;
+    std::vector<std::string> errs(n_thr);
+    for (int t = 0; t < n_thr; ++t)
+      th.emplace_back([&, t] {
+        try {
+          for (int i = t; i < n; i += n_thr) {
+            out_conc[i] = eval_rad(ks[i], (i / n_thr) % 2 == 1);
+            out_conc[n + i] = eval_ang(ks[i], (i / n_thr) % 2 == 0);
+          }
+        } catch (const std::exception &e) {
+          errs[t] = e.what();
+        }
+      });
+    for (auto &x : th) x.join();
+    for (auto &e : errs)
+      if (!e.empty()) throw std::runtime_error(e);
   });
 }
 

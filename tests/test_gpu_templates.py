@@ -141,6 +141,20 @@ def test_radial_template_dual(kat, qp):
         assert out[2] == out[0] and out[3] == out[1]
 
 
+def test_concurrent_callers(kat):
+    # 8 host threads share one radial and one two-angle integrator, mixing get() and request()
+    # (the reference's get_data calls get() from TBB threads): every result bitwise equals the same
+    # call made alone
+    rng = np.random.default_rng(12)
+    n = 64
+    ks = arr(rng.uniform(0.2, 1.0, n))
+    c, a, b = arr(rng.uniform(-1, 1, 6)), arr(rng.uniform(-1, 1, 4)), arr(rng.uniform(-1, 1, 4))
+    conc, seq = np.empty(2 * n), np.empty(2 * n)
+    print("[case] 8 threads x 64 calls on shared integrators", flush=True)
+    call(kat, "kat_concurrent", 8, P(ks), n, P(c), P(a), P(b), P(conc), P(seq))
+    assert np.array_equal(conc, seq)
+
+
 def test_two_angle_template(kat, qp):
     rng = np.random.default_rng(7)
     xq, wq = qp.get_points(64), qp.get_weights(64)
