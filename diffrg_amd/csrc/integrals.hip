@@ -988,8 +988,9 @@ int dfrg_se_batch_graph_launch(dfrg_se_batch *b, int mode, double *d_out, int la
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cs, &g);
     hipStreamDestroy(cs);
-    if (rc != DFRG_OK) return rc;
-    if (e != hipSuccess) {
+    if (rc != DFRG_OK || e != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      if (rc != DFRG_OK) return rc;
       set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
       return DFRG_ERR_HIP;
     }
