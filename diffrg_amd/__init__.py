@@ -189,6 +189,16 @@ class DeviceArray:
             pass
 
 
+def _need(a: "DeviceArray", n: int, dtype, what: str) -> None:
+    """A device argument must hold at least n values of dtype: the kernels index it without bounds."""
+    if not isinstance(a, DeviceArray) or a.ptr is None:
+        raise ValueError(f"{what}: a live DeviceArray is required")
+    if a.dtype != np.dtype(dtype):
+        raise ValueError(f"{what}: dtype {a.dtype} where {np.dtype(dtype)} is required")
+    if a.n < n:
+        raise ValueError(f"{what}: holds {a.n} values, {n} are required")
+
+
 def stream_sync(stream=None) -> None:
     check(lib().dfrg_stream_sync(stream))
 
@@ -282,6 +292,8 @@ class Table:
         return d.value
 
     def eval(self, d_y: "DeviceArray", d_out: "DeviceArray", stream=None) -> None:
+        _need(d_y, 0, np.float64, "y")
+        _need(d_out, d_y.n, np.float64, "out")
         check(lib().dfrg_table_eval(self.ptr, d_y.ptr, d_y.n, d_out.ptr, stream))
 
     def close(self) -> None:
@@ -356,12 +368,17 @@ class QuadratureProvider:
 
 def threshold_batch(d_m2: DeviceArray, d_x: DeviceArray, d_w: DeviceArray, d_out: DeviceArray, x_extent=20.0,
                     k=1.0, mode=FP64, d=4, stream=None) -> None:
+    # only the output is checked here (one check per call keeps this launch-sized call cheap)
+    _need(d_out, d_m2.n, np.float64, "out")
     check(lib().dfrg_threshold_batch(d, d_m2.ptr, d_m2.n, d_x.ptr, d_w.ptr, d_x.n, x_extent, k, mode, d_out.ptr,
                                      stream))
 
 
 def ft_batch(dev: dict, d_out: DeviceArray, extent, k=1.0, mode=FP64, d=4, radial_mode=1, stream=None) -> None:
     """dev holds DeviceArrays T, p0, pabs, m1, m2, mx, mw, m_off (int32), m_n (int32), x, w."""
+    # only the output is checked here (one check per call keeps this launch-sized call cheap); the
+    # inputs are the caller's: per-external arrays of n values, m_off/m_n int32 into mx/mw
+    _need(d_out, dev["T"].n, np.float64, "out")
     check(lib().dfrg_ft_batch(d, radial_mode, dev["T"].n, dev["T"].ptr, dev["p0"].ptr, dev["pabs"].ptr,
                               dev["m1"].ptr, dev["m2"].ptr, dev["mx"].ptr, dev["mw"].ptr, dev["m_off"].ptr,
                               dev["m_n"].ptr, dev["x"].ptr, dev["w"].ptr, dev["x"].n, extent, k, mode, d_out.ptr,
@@ -425,9 +442,11 @@ class SEBatch:
         return {"block": b.value, "tile_evals": t.value, "grid": g.value, "n_tiles": n.value}
 
     def run(self, d_out: DeviceArray, mode=FP64, stream=None):
+        _need(d_out, self.n, np.float64, "out")
         check(lib().dfrg_se_batch_run(self.ptr, mode, d_out.ptr, stream))
 
     def graph_launch(self, d_out: DeviceArray, mode=FP64, launches=1, stream=None):
+        _need(d_out, self.n, np.float64, "out")
         check(lib().dfrg_se_batch_graph_launch(self.ptr, mode, d_out.ptr, launches, stream))
 
     def autotune(self, mode=FP64, reps=3, cache_path=None) -> dict:
