@@ -395,11 +395,13 @@ class SEBatch:
 
     def __init__(self, qp: QuadratureProvider, p, nq, na, tables: Sequence[np.ndarray], start, stop, bias,
                  x_extent=20.0, k=1.0):
+        self.ptr = None  # until the library object exists
         n = len(p)
         tab_n = np.array([len(t) for t in tables], np.int32)
         tab_off = np.zeros(n, np.int32)
         tab_off[1:] = np.cumsum(tab_n)[:-1]
-        flat = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float64) for t in tables]))
+        # an empty call set reaches the library, which rejects it with its own message
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float64) for t in tables] or [np.zeros(0)]))
         arr = lambda v, dt: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dt), (n,)))
         self._keep = [arr(p, np.float64), arr(nq, np.int32), arr(na, np.int32), tab_off, tab_n,
                       arr(start, np.float64), arr(stop, np.float64), arr(bias, np.float64), flat]

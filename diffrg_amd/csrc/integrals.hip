@@ -731,7 +731,10 @@ int dfrg_ft_batch(int d, int radial_mode, int n_int, const double *d_T, const do
   DFRG_CHECK_ARG(radial_mode == 0 || radial_mode == 1, "radial_mode must be 0 or 1");
   DFRG_CHECK_ARG(nx >= 1 && nx <= 1024, "radial order %d out of range [1,1024]", nx);
   DFRG_CHECK_ARG(mode == DFRG_FP64 || mode == DFRG_MIXED, "invalid mode %d", mode);
+  DFRG_CHECK_ARG(n_int >= 0, "invalid size n_int=%d", n_int);
   if (n_int == 0) return DFRG_OK;
+  DFRG_CHECK_ARG(d_T && d_p0 && d_pabs && d_m1 && d_m2 && d_mx && d_mw && d_m_off && d_m_n && d_x && d_w && d_out,
+                 "NULL device pointer");
   FTArgs a{d, radial_mode, n_int, nx, d_T, d_p0, d_pabs, d_m1, d_m2, d_mx, d_mw, d_m_off, d_m_n, d_x, d_w,
            extent, k, 0.0, d_out};
   a.pref = S_d_host(d - 1) / std::pow(2.0 * kPi, d - 1);

@@ -367,3 +367,21 @@ def test_invalid_arguments(gpu, qp):
     b = selfenergy_batch(gpu, qp, c)
     with pytest.raises(gpu.DfrgInvalidArgument):
         b.set_shape(100, 1024, 0)
+
+
+def test_empty_and_negative_batch_sizes(gpu, qp):
+    """An empty call set is a no-op (no launch, no pointer touched); a negative size or an empty
+    self-energy batch is an argument error, not a launch."""
+    L = gpu.lib()
+    print("[case] empty threshold / finite-T batches", flush=True)
+    assert L.dfrg_threshold_batch(4, None, 0, None, None, 1, 20.0, 1.0, gpu.FP64, None, None) == 0
+    assert L.dfrg_ft_batch(4, 1, 0, *([None] * 11), 16, 20.0, 1.0, gpu.MIXED, None, None) == 0
+    print("[case] negative sizes", flush=True)
+    with pytest.raises(gpu.DfrgInvalidArgument, match="n_int"):
+        gpu.check(L.dfrg_threshold_batch(4, None, -1, None, None, 1, 20.0, 1.0, gpu.FP64, None, None))
+    with pytest.raises(gpu.DfrgInvalidArgument, match="n_int"):
+        gpu.check(L.dfrg_ft_batch(4, 1, -3, *([None] * 11), 16, 20.0, 1.0, gpu.FP64, None, None))
+    with pytest.raises(gpu.DfrgInvalidArgument, match="NULL"):
+        gpu.check(L.dfrg_ft_batch(4, 1, 2, *([None] * 11), 16, 20.0, 1.0, gpu.FP64, None, None))
+    with pytest.raises(gpu.DfrgInvalidArgument, match="n_int"):
+        gpu.SEBatch(qp, [], [], [], [], [], [], [])
