@@ -77,6 +77,7 @@ SIGNATURES = {
     "dfrg_qp_gl": [_vp, C.c_int, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)],
     "dfrg_qp_matsubara": [_vp, C.c_double, C.c_double, C.c_int, _ip, _dp, C.POINTER(_vp), C.POINTER(_vp),
                           C.POINTER(_vp), C.POINTER(_vp)],
+    "dfrg_valu_peak": [C.c_int, C.POINTER(C.c_double)],
     "dfrg_threshold_batch": [C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_int, _vp, _vp],
     "dfrg_se_batch_create": [C.POINTER(_vp), _vp, C.c_int, _dp, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int,
                              C.c_double, C.c_double],
@@ -146,6 +147,13 @@ def device_props() -> dict:
     cu, clk, hbm = C.c_int(), C.c_int(), C.c_ulonglong()
     check(lib().dfrg_device_props(name, 256, C.byref(cu), C.byref(clk), C.byref(hbm)))
     return {"name": name.value.decode(), "cu_count": cu.value, "clock_khz": clk.value, "hbm_bytes": hbm.value}
+
+
+def valu_peak(fp64: bool = True) -> float:
+    """Measured vector FMA rate of the device in FLOP/s (independent FMA chains, all SIMDs busy)."""
+    r = C.c_double()
+    check(lib().dfrg_valu_peak(1 if fp64 else 0, C.byref(r)))
+    return r.value
 
 
 # ------------------------------------------------------------------------------- device memory

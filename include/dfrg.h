@@ -54,6 +54,11 @@ const char *dfrg_last_error(void);
 int dfrg_device_count(int *count);
 /* name: buffer of len bytes; any output pointer may be NULL. */
 int dfrg_device_props(char *name, int len, int *cu_count, int *clock_khz, unsigned long long *hbm_bytes);
+/* Measured vector FMA rate of the device in FLOP/s (fp64 = 1: FP64 FMAs, 0: plain FP32 FMAs), from
+ * independent FMA chains at eight waves per SIMD; the best of three ~1 ms launches after a warm-up.
+ * No counterpart in the reference: it confirms on the device the published vector peak that the
+ * integrands' VALU roofline is quoted against.  Synchronous; allocates and frees its own buffer. */
+int dfrg_valu_peak(int fp64, double *flop_per_s);
 
 int dfrg_malloc(void **d_ptr, size_t bytes);
 int dfrg_free(void *d_ptr);
