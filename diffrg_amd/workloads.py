@@ -2,7 +2,7 @@
 
 All randomness comes from SplitMix64 (Steele, Lea & Flood 2014) with seed 0xD1FF0000 + c for
 workload c; u = (z >> 11) * 2^-53.  The arrays built here are the inputs of both the GPU path
-and the CPU checker, so both see bitwise-identical data.
+and the CPU oracle, so both see bitwise-identical data.
 
   0 threshold    256 LPA threshold integrals, m2 ~ U[-0.8, 4], 64-node GL in q^2 over [0, 20]
   1 selfenergy   64 external p on LogarithmicCoordinates1D(64, 1e-3, 40, 6), two-angle form

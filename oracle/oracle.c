@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU restatement of DiFfRG v1.0.1's momentum-loop integrators.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline); see oracle.h.
+ * TEST INFRASTRUCTURE ONLY (reference values of the tests and the CPU baseline); see oracle.h.
  * Integrator loops follow the reference's TBB forms (element, weight and kernel
  * per node, plain FP64 sums).  Summation is made deterministic: per outer node
  * partial sums are added in index order, independent of the OpenMP thread count.

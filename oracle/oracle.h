@@ -2,8 +2,8 @@
  * oracle.h -- CPU restatement of DiFfRG v1.0.1's momentum-loop integrator path.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in diffrg_amd/ or include/ links, loads or
- * calls this code.  It is used by tests/ (as the parity checker), by the smoke
- * check (as the checker) and by bench.py's cpu_baseline leg (as the timed CPU
+ * calls this code.  It is used by tests/ (as the parity reference), by the smoke
+ * check (as its reference values) and by bench.py's cpu_baseline leg (as the timed CPU
  * baseline).  Every function cites the reference file:line it restates; paths
  * are relative to the DiFfRG source tree.
  *

@@ -1,7 +1,7 @@
 """ctypes view of liboracle.so, the CPU restatement of DiFfRG's integrator path.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` as
-the checker, and by the cpu_baseline leg of ``bench.py``.  The product
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` for
+its reference values, and by the cpu_baseline leg of ``bench.py``.  The product
 (``diffrg_amd``) never imports this module.
 """
 from __future__ import annotations
