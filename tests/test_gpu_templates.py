@@ -100,6 +100,20 @@ def test_radial_complex_and_requests(kat, qp):
         assert out[i] == pytest.approx(O.kat_radial(4, x, w, 1.5, k, c, 0.0), rel=TOL)
 
 
+def test_many_requests_in_one_batch(kat, qp):
+    # 20,000 futures read only after all are requested: one batch, its tuple and result buffers
+    # grown well past any earlier batch of the integrator
+    n = 20000
+    ks = arr(np.linspace(0.05, 2.0, n))
+    c = arr([0.3, -0.2, 0.5, 0.1, -0.4, 0.2])
+    out = np.empty(n)
+    print(f"[case] {n} radial requests in one batch", flush=True)
+    call(kat, "kat_radial_requests", 64, D(1.5), P(ks), n, P(c), P(out))
+    x, w = qp.get_points(64), qp.get_weights(64)
+    ref = np.array([O.kat_radial(4, x, w, 1.5, k, c, 0.0) for k in ks])
+    assert np.max(np.abs(out - ref)) <= TOL * np.max(np.abs(ref))  # ref may cross zero in k
+
+
 def test_radial_template_float(kat, qp):
     # NT = ctype = float: FP32 nodes and arithmetic, against the FP64 oracle on the same order
     x, w = qp.get_points(64), qp.get_weights(64)
