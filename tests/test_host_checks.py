@@ -41,3 +41,17 @@ def test_wrappers_check_outputs_first():
         D.threshold_batch(_Fake(8), _Fake(64), _Fake(64), small)
     with pytest.raises(ValueError, match="holds 3"):
         D.ft_batch({"T": _Fake(8)}, small, 6.0)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N>1 stops before touching a GPU, with a JSON line that says why."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert msg["n_gpus"] == 2 and "one GPU" in msg["error"]
