@@ -235,6 +235,27 @@ def test_uneven_batch_parity(gpu, qp):
             assert err.max() <= tol
 
 
+def test_uneven_full_size(gpu, qp):
+    """All 4096 integrals of the uneven batch (347,385,856 evaluations), FP64 and mixed, against the
+    oracle; a persistent launch and a one-tile-per-block launch of the same tiles agree bitwise."""
+    c = uneven_subset(len(W.uneven()["p"]))
+    print(f"[case] uneven full size ({len(c['p'])} integrals): oracle", flush=True)
+    ref = uneven_oracle(qp, c)
+    b = make_uneven(gpu, qp, c)
+    out = gpu.DeviceArray(len(c["p"]))
+    for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+        res = []
+        for grid in (0, 1024):
+            b.set_shape(256, 65536, grid)
+            print(f"[case] uneven full size mode={mode} grid={grid} tiles={b.shape()['n_tiles']}", flush=True)
+            b.run(out, mode)
+            res.append(out.to_host())
+            err = rel_err(res[-1], ref)
+            print(f"  max rel err {err.max():.3e}", flush=True)
+            assert err.max() <= tol
+        assert np.array_equal(res[0], res[1])
+
+
 def test_uneven_alone_equals_in_mix(gpu, qp):
     c = uneven_subset(256)
     b = make_uneven(gpu, qp, c)
