@@ -1049,19 +1049,25 @@ int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cac
   const int blocks[] = {128, 256, 512, 1024};
   const int tiles[] = {1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072};
   float best = 1e30f;
+  bool all_ok = true;
   int bb = b->block, bt = b->tile_evals, bg = b->grid;
   for (int bl : blocks) {
     for (int te : tiles) {
       for (int gmul : {0, 2, 4, 8}) {
         const int gr = gmul == 0 ? 0 : gmul * cus * 512 / bl;
         if (dfrg_se_batch_set_shape(b, bl, te, gr) != DFRG_OK) continue;
-        se_launch(b, mode, d_out, s); // warm-up
-        hipEventRecord(e0, s);
-        for (int r = 0; r < reps; ++r) se_launch(b, mode, d_out, s);
-        hipEventRecord(e1, s);
-        hipEventSynchronize(e1);
+        // a shape counts only when every launch and event call of its measurement succeeded
+        bool ok = se_launch(b, mode, d_out, s) == DFRG_OK; // warm-up
+        ok = ok && hipEventRecord(e0, s) == hipSuccess;
+        for (int r = 0; r < reps && ok; ++r) ok = se_launch(b, mode, d_out, s) == DFRG_OK;
+        ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
         float t = 0.f;
-        hipEventElapsedTime(&t, e0, e1);
+        ok = ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess && t > 0.f;
+        if (!ok) {
+          all_ok = false;
+          hipGetLastError(); // clear a launch error so it is not reported by a later call
+          continue;
+        }
         t /= reps;
         if (t < best) {
           best = t;
@@ -1076,8 +1082,12 @@ int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cac
   hipEventDestroy(e1);
   hipStreamDestroy(s);
   hipFree(d_out);
-  DFRG_TRY(dfrg_se_batch_set_shape(b, bb, bt, bg));
-  if (cache_path) {
+  DFRG_TRY(dfrg_se_batch_set_shape(b, bb, bt, bg)); // the previous shape when no candidate succeeded
+  if (best >= 1e30f) {
+    set_error("autotune: no launch shape could be measured");
+    return DFRG_ERR_RUNTIME;
+  }
+  if (cache_path && all_ok) {
     std::ofstream o(cache_path, std::ios::app);
     o << key << " " << bb << " " << bt << " " << bg << " " << best << "\n";
   }
