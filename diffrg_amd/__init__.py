@@ -90,6 +90,7 @@ SIGNATURES = {
     "dfrg_se_batch_evals": [_vp, C.POINTER(C.c_ulonglong)],
     "dfrg_se_batch_variants": [_vp, C.c_int, _ip],
     "dfrg_se_batch_unroll": [_vp, _ip],
+    "dfrg_se_batch_run_noalloc": [_vp, C.c_int, _vp, _vp],
     "dfrg_ft_set_block": [C.c_int],
     "dfrg_ft_batch": [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
                       C.c_double, C.c_double, C.c_int, _vp, _vp],
@@ -454,6 +455,13 @@ class SEBatch:
     def run(self, d_out: DeviceArray, mode=FP64, stream=None):
         _need(d_out, self.n, np.float64, "out")
         check(lib().dfrg_se_batch_run(self.ptr, mode, d_out.ptr, stream))
+
+    def run_noalloc(self, d_out: DeviceArray, mode, stream):
+        """run() with the library's static scratch: allocates nothing; `stream` (a Stream or a
+        raw handle) is required.  Calls that overlap on one GPU race."""
+        _need(d_out, self.n, np.float64, "out")
+        h = stream.ptr if isinstance(stream, Stream) else stream
+        check(lib().dfrg_se_batch_run_noalloc(self.ptr, mode, d_out.ptr, h))
 
     def graph_launch(self, d_out: DeviceArray, mode=FP64, launches=1, stream=None):
         _need(d_out, self.n, np.float64, "out")

@@ -544,6 +544,13 @@ namespace
   bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
   std::atomic<int> g_ft_block{0}; // threads per integral of dfrg_ft_batch, 0 = automatic
+
+  // Zeroes the first word of n_lines consecutive 128-byte lines (ticket counters and queue words).
+  __global__ __launch_bounds__(256) void clear_lines_kernel(unsigned *lines, int n_lines)
+  {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_lines) lines[(long long)i * dev::kCounterStride] = 0u;
+  }
 } // namespace
 
 // ----------------------------------------------------------------------------------------------
@@ -663,9 +670,20 @@ namespace
     return blocks * (b->block / 64) <= 16 * cus ? 2 : 1;
   }
 
-  int se_launch(dfrg_se_batch *b, int mode, double *d_out, hipStream_t s)
+  // Cross-block scratch of one launch: partial slots, ticket counters and the work queue.
+  struct SEScratch {
+    double *partials;
+    unsigned *counters, *queue;
+  };
+
+  int se_launch(dfrg_se_batch *b, int mode, double *d_out, hipStream_t s, const SEScratch *scr = nullptr)
   {
-    const SEArgs a = se_args(b, d_out);
+    SEArgs a = se_args(b, d_out);
+    if (scr) {
+      a.partials = scr->partials;
+      a.counters = scr->counters;
+      a.queue = scr->queue;
+    }
     const int grid = b->grid > 0 ? std::min(b->grid, b->n_tiles) : b->n_tiles;
     hipError_t e = mode == DFRG_FP64    ? launch_se<DFRG_FP64, 1>(b->block, grid, s, a)
                    : se_unroll(b) == 2 ? launch_se<DFRG_MIXED, 2>(b->block, grid, s, a)
@@ -971,6 +989,38 @@ int dfrg_se_batch_run(dfrg_se_batch *b, int mode, double *d_out, void *stream)
   DFRG_CHECK_ARG(b && d_out, "NULL argument");
   DFRG_CHECK_ARG(mode == DFRG_FP64 || mode == DFRG_MIXED, "invalid mode %d", mode);
   return se_launch(b, mode, d_out, resolve_stream(stream));
+}
+
+int dfrg_se_batch_run_noalloc(dfrg_se_batch *b, int mode, double *d_out, void *stream)
+{
+  DFRG_TRY(require_init());
+  DFRG_CHECK_ARG(b && d_out, "NULL argument");
+  DFRG_CHECK_ARG(stream != nullptr, "dfrg_se_batch_run_noalloc needs a stream (NULL would create one)");
+  DFRG_CHECK_ARG(mode == DFRG_FP64 || mode == DFRG_MIXED, "invalid mode %d", mode);
+  DFRG_CHECK_ARG(b->n_int <= DFRG_NOALLOC_MAX_INTEGRALS, "%d integrals exceed the static scratch limit of %d",
+                 b->n_int, DFRG_NOALLOC_MAX_INTEGRALS);
+  DFRG_CHECK_ARG(b->n_tiles <= DFRG_NOALLOC_MAX_TILES,
+                 "%d tiles exceed the static scratch limit of %d (a larger tile_evals gives fewer tiles)", b->n_tiles,
+                 DFRG_NOALLOC_MAX_TILES);
+  char *base = (char *)noalloc_scratch();
+  if (base == nullptr) {
+    set_error("static scratch of this device was not resolved");
+    return DFRG_ERR_RUNTIME;
+  }
+  // layout: queue (two lines), counters (one line per integral), partials (one line per tile)
+  static_assert(2 * dev::kCounterStride * sizeof(unsigned) == 256, "queue takes the first 256 bytes");
+  static_assert(dev::kCounterStride * sizeof(unsigned) == 128 && dev::kSlotStride * sizeof(double) == 128,
+                "one 128-byte line per counter and per partial slot");
+  SEScratch scr;
+  scr.queue = (unsigned *)base;
+  scr.counters = (unsigned *)(base + 256);
+  scr.partials = (double *)(base + 256 + 128 * (size_t)DFRG_NOALLOC_MAX_INTEGRALS);
+  hipStream_t s = (hipStream_t)stream;
+  // the queue's two lines and the batch's counter lines are consecutive: clear them all
+  const int lines = 2 + b->n_int;
+  hipLaunchKernelGGL(clear_lines_kernel, dim3((lines + 255) / 256), dim3(256), 0, s, scr.queue, lines);
+  DFRG_LAUNCHED("clear_lines_kernel");
+  return se_launch(b, mode, d_out, s, &scr);
 }
 
 int dfrg_se_batch_graph_launch(dfrg_se_batch *b, int mode, double *d_out, int launches, void *stream)

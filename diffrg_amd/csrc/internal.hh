@@ -35,6 +35,10 @@ namespace dfrg
 
   // Stream of the calling thread when `stream` is NULL.
   hipStream_t resolve_stream(void *stream);
+
+  // Device address of the current device's static scratch (DFRG_NOALLOC_SCRATCH_BYTES), resolved
+  // in dfrg_init; nullptr before it.
+  void *noalloc_scratch();
 } // namespace dfrg
 
 #define DFRG_HIP(call)                                                                                         \

@@ -21,6 +21,15 @@ namespace dfrg
     };
     Pool g_dev_pool, g_host_pool;
 
+    // Scratch of the entry point that allocates nothing (dfrg_se_batch_run_noalloc): partial
+    // slots, ticket counters and the work queue of se_kernel, DFRG_NOALLOC_SCRATCH_BYTES per
+    // device, held until the process ends.  In host code `scratch` is only a handle: its device
+    // address on each GPU is resolved once per device in dfrg_init (hipGetSymbolAddress) and kept in
+    // g_scratch_ptr, which is what every host-side use takes.
+    static __device__ __align__(256) unsigned char scratch[DFRG_NOALLOC_SCRATCH_BYTES];
+    constexpr int kMaxDevices = 64;
+    void *g_scratch_ptr[kMaxDevices] = {};
+
     size_t size_class(size_t bytes)
     {
       size_t c = 256;
@@ -72,6 +81,12 @@ namespace dfrg
     }
     return g_thread_stream;
   }
+
+  void *noalloc_scratch()
+  {
+    const int d = context().device;
+    return d >= 0 && d < kMaxDevices ? g_scratch_ptr[d] : nullptr;
+  }
 } // namespace dfrg
 
 using namespace dfrg;
@@ -109,6 +124,17 @@ int dfrg_init(int device)
     return DFRG_OK;
   }
   DFRG_HIP(hipSetDevice(device));
+  // this device's copy of the static scratch: resolve its address once, with the device
+  // current, and zero it (its counters must start at zero)
+  DFRG_CHECK_ARG(device < kMaxDevices, "device %d beyond the %d devices the library supports", device,
+                 kMaxDevices);
+  if (g_scratch_ptr[device] == nullptr) {
+    void *p = nullptr;
+    DFRG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DFRG_HIP(hipMemset(p, 0, DFRG_NOALLOC_SCRATCH_BYTES));
+    DFRG_HIP(hipDeviceSynchronize());
+    g_scratch_ptr[device] = p;
+  }
   hipDeviceProp_t prop;
   DFRG_HIP(hipGetDeviceProperties(&prop, device));
   c.device = device;

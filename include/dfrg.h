@@ -186,6 +186,22 @@ int dfrg_se_batch_variants(dfrg_se_batch *b, int mode, int *variant);
  * uses 1.  Results do not depend on it. */
 int dfrg_se_batch_unroll(dfrg_se_batch *b, int *unroll);
 
+/* dfrg_se_batch_run without any allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The batch (inputs) must exist already and `stream` must not be NULL (the
+ * NULL stream would create the calling thread's stream on first use).  Partial slots, ticket
+ * counters and the work queue live in a static device array of DFRG_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends), so a batch may have at most DFRG_NOALLOC_MAX_INTEGRALS integrals
+ * and DFRG_NOALLOC_MAX_TILES tiles at its current shape (dfrg_se_batch_get_shape); larger ones give
+ * DFRG_ERR_INVALID and launch nothing.  Each call clears the counters and queue it uses with a fill
+ * kernel before its launch.  Calls on one GPU share that GPU's array: calls that overlap on one
+ * GPU (different streams, or different host threads) race and must be serialised by the caller.
+ * Results are bitwise those of dfrg_se_batch_run. */
+#define DFRG_NOALLOC_MAX_INTEGRALS 4096
+#define DFRG_NOALLOC_MAX_TILES 16384
+#define DFRG_NOALLOC_SCRATCH_BYTES (256 + 128 * DFRG_NOALLOC_MAX_INTEGRALS + 128 * DFRG_NOALLOC_MAX_TILES)
+int dfrg_se_batch_run_noalloc(dfrg_se_batch *b, int mode, double *d_out, void *stream);
+
 /* Finite-T two-propagator batch (IntegratorFiniteTq0GPU form, integrator_finiteTq0_gpu.hh:20-44):
  *   I_i = sum_{j,n} el(q_j) wq_j [ wm_n (f(q_j,wn) + f(q_j,-wn)) ] + T_i sum_j el(q_j) wq_j f(q_j,0),
  *   f(q,q0) = RBdot(q^2) G(q0,q^2,m1_i)^2 G(q0+p0_i, q^2+pabs_i^2, m2_i),  G(w,s,m) = 1/(w^2+s+RB(k^2,s)+m).
