@@ -18,6 +18,9 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
+# stand-ins for the reference headers the drop-in does not replace (json, math, types, regulators,
+# coordinates ...), for builds without the reference tree; never ahead of the reference's own
+STANDALONE = os.path.join(INCLUDE, "standalone")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libdfrg.so")
 
@@ -64,7 +67,7 @@ KAT_LIB = os.path.join(ROOT, "tests", "native", "libdfrg_kat.so")
 def build_kat(verbose: bool = True) -> str:
     """Compile the test library that instantiates the C++ integrator templates (tests/native)."""
     tmp = KAT_LIB + ".tmp"
-    cmd = [HIPCC, *COMMON_FLAGS, "-fopenmp", "-shared", KAT_SRC, "-o", tmp, f"-L{PKG}", "-ldfrg",
+    cmd = [HIPCC, *COMMON_FLAGS, f"-I{STANDALONE}", "-fopenmp", "-shared", KAT_SRC, "-o", tmp, f"-L{PKG}", "-ldfrg",
            "-Wl,-rpath,$ORIGIN/../../diffrg_amd"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)

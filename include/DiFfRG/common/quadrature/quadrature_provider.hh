@@ -11,8 +11,8 @@
 #include <type_traits>
 #include <vector>
 
-#include <DiFfRG/common/hip_prefix.hh>
-#include <DiFfRG/common/json.hh>
+#include "dfrg/check.hh"
+#include "dfrg/math.hh"
 
 namespace DiFfRG
 {
@@ -24,7 +24,8 @@ namespace DiFfRG
     // QuadratureProvider() (quadrature_provider.cu:199-204): vacuum 48, min Matsubara 32, add 0
     QuadratureProvider() : QuadratureProvider(48, 32, 0) {}
     // QuadratureProvider(json) (quadrature_provider.cu:206-228)
-    QuadratureProvider(const JSONValue &json)
+    template <dfrg::JsonLike JSON>
+    QuadratureProvider(const JSON &json)
         : QuadratureProvider((int)json.get_uint("/integration/vacuum_quad_size", 48),
                              (int)json.get_uint("/integration/min_matsubara_size", 16),
                              (int)json.get_uint("/integration/add_matsubara_size", 0))
@@ -32,8 +33,8 @@ namespace DiFfRG
     }
     QuadratureProvider(int vacuum_quad_size, int min_matsubara_size, int add_matsubara_size)
     {
-      ensure_device();
-      dfrg_check(dfrg_qp_create(&qp, vacuum_quad_size, min_matsubara_size, add_matsubara_size), "QuadratureProvider");
+      dfrg::ensure_device();
+      dfrg::check(dfrg_qp_create(&qp, vacuum_quad_size, min_matsubara_size, add_matsubara_size), "QuadratureProvider");
     }
     QuadratureProvider(const QuadratureProvider &) = delete;
     QuadratureProvider &operator=(const QuadratureProvider &) = delete;
@@ -102,7 +103,7 @@ namespace DiFfRG
       auto it = cache.find(order);
       if (it != cache.end()) return it->second;
       const void *hx, *hw;
-      dfrg_check(dfrg_qp_gl(qp, (int)order, fp32_flag<NT>(), nullptr, nullptr, &hx, &hw), "get_points");
+      dfrg::check(dfrg_qp_gl(qp, (int)order, fp32_flag<NT>(), nullptr, nullptr, &hx, &hw), "get_points");
       const NT *x = (const NT *)hx, *w = (const NT *)hw;
       return cache.emplace(order, std::make_pair(std::vector<NT>(x, x + order), std::vector<NT>(w, w + order)))
           .first->second;
@@ -112,7 +113,7 @@ namespace DiFfRG
       (void)device; // one device per process
       check_type(type);
       const void *dx, *dw;
-      dfrg_check(dfrg_qp_gl(qp, (int)order, fp32_flag<NT>(), &dx, &dw, nullptr, nullptr), "get_device_points");
+      dfrg::check(dfrg_qp_gl(qp, (int)order, fp32_flag<NT>(), &dx, &dw, nullptr, nullptr), "get_device_points");
       return (const NT *)(which == 0 ? dx : dw);
     }
     template <typename NT> const std::pair<std::vector<NT>, std::vector<NT>> &host_mats(NT T, NT E)
@@ -121,7 +122,7 @@ namespace DiFfRG
       int n = 0;
       double Tr = 0;
       const void *hx, *hw;
-      dfrg_check(dfrg_qp_matsubara(qp, (double)T, (double)E, fp32_flag<NT>(), &n, &Tr, nullptr, nullptr, &hx, &hw),
+      dfrg::check(dfrg_qp_matsubara(qp, (double)T, (double)E, fp32_flag<NT>(), &n, &Tr, nullptr, nullptr, &hx, &hw),
                  "get_matsubara_points");
       auto &cache = mats_cache<NT>();
       auto it = cache.find(hx);
@@ -135,7 +136,7 @@ namespace DiFfRG
       int n = 0;
       double Tr = 0;
       const void *dx, *dw;
-      dfrg_check(dfrg_qp_matsubara(qp, (double)T, (double)E, fp32_flag<NT>(), &n, &Tr, &dx, &dw, nullptr, nullptr),
+      dfrg::check(dfrg_qp_matsubara(qp, (double)T, (double)E, fp32_flag<NT>(), &n, &Tr, &dx, &dw, nullptr, nullptr),
                  "get_device_matsubara_points");
       return (const NT *)(which == 0 ? dx : dw);
     }

@@ -6,24 +6,23 @@
 #include <future>
 #include <vector>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 namespace DiFfRG
 {
   template <typename NT, typename KERNEL> class Integrator4D2AngTBB
   {
   public:
-    using ctype = typename get_type::ctype<NT>;
-    Integrator4D2AngTBB(QuadratureProvider &qp, const std::array<uint, 3> grid_sizes, const ctype x_extent,
-                        const JSONValue &)
+    using ctype = typename dfrg::ctype_t<NT>;
+    template <dfrg::JsonLike JSON>
+    Integrator4D2AngTBB(QuadratureProvider &qp, const std::array<unsigned, 3> grid_sizes, const ctype x_extent,
+                        const JSON &)
         : Integrator4D2AngTBB(qp, grid_sizes, x_extent)
     {
     }
-    Integrator4D2AngTBB(QuadratureProvider &qp, std::array<uint, 3> grid_sizes, const ctype x_extent,
-                        const uint max_block_size = 0)
+    Integrator4D2AngTBB(QuadratureProvider &qp, std::array<unsigned, 3> grid_sizes, const ctype x_extent,
+                        const unsigned max_block_size = 0)
         : grid_sizes(grid_sizes), x_extent(x_extent), x_quadrature_p(qp.get_points<ctype>(grid_sizes[0])),
           x_quadrature_w(qp.get_weights<ctype>(grid_sizes[0])), ang_quadrature_p1(qp.get_points<ctype>(grid_sizes[1])),
           ang_quadrature_w1(qp.get_weights<ctype>(grid_sizes[1])), ang_quadrature_p2(qp.get_points<ctype>(grid_sizes[2])),
@@ -39,11 +38,11 @@ namespace DiFfRG
       for (int idx_x = 0; idx_x < (int)grid_sizes[0]; ++idx_x) {
         const ctype q = k * std::sqrt(x_quadrature_p[idx_x] * x_extent);
         NT value = NT(0);
-        for (uint idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
+        for (unsigned idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
           const ctype cos1 = 2 * (ang_quadrature_p1[idx_y] - (ctype)0.5);
-          const ctype int_element = 2 * (ctype)M_PI * (powr<2>(q) * (ctype)0.5 * powr<2>(k)) *
-                                    std::sqrt(1. - powr<2>(cos1)) / powr<4>(2 * (ctype)M_PI);
-          for (uint idx_z = 0; idx_z < grid_sizes[2]; ++idx_z) {
+          const ctype int_element = 2 * (ctype)M_PI * (dfrg::powr<2>(q) * (ctype)0.5 * dfrg::powr<2>(k)) *
+                                    std::sqrt(1. - dfrg::powr<2>(cos1)) / dfrg::powr<4>(2 * (ctype)M_PI);
+          for (unsigned idx_z = 0; idx_z < grid_sizes[2]; ++idx_z) {
             const ctype cos2 = 2 * (ang_quadrature_p2[idx_z] - (ctype)0.5);
             const ctype weight =
                 2 * ang_quadrature_w2[idx_z] * 2 * ang_quadrature_w1[idx_y] * x_quadrature_w[idx_x] * x_extent;
@@ -63,7 +62,7 @@ namespace DiFfRG
     }
 
   private:
-    const std::array<uint, 3> grid_sizes;
+    const std::array<unsigned, 3> grid_sizes;
     const ctype x_extent;
     const std::vector<ctype> &x_quadrature_p, &x_quadrature_w, &ang_quadrature_p1, &ang_quadrature_w1,
         &ang_quadrature_p2, &ang_quadrature_w2;

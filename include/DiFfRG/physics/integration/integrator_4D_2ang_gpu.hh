@@ -1,84 +1,66 @@
 // DiFfRG/physics/integration/integrator_4D_2ang_gpu.hh -- Integrator4D2AngGPU<NT, KERNEL>
-// (reference integrator_4D_2ang_gpu.hh:58-341):
+// (reference integrator_4D_2ang_gpu.hh:58-341) on the tile engine of dfrg/integrate.hh:
 //   2 pi int dq dcos1 dcos2 (2pi)^-4 q^2/2 k^2 sqrt(1 - cos1^2) KERNEL::kernel(q, cos1, cos2, k, t...) + constant
-// with q^2 in [0, x_extent k^2].  One launch per get() and per batch of request()s; angular orders
-// must be equal.
+// with q^2 in [0, x_extent k^2].  get() is one launch; request()s are batched (one launch per
+// batch, bitwise the values of get()).  The angular orders must be equal.
 #pragma once
 
 #include <array>
 #include <future>
 #include <memory>
-#include <tuple>
 #include <stdexcept>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 #include "dfrg/integrate.hh"
+#include "dfrg/math.hh"
 
 namespace DiFfRG
 {
   template <typename NT, typename KERNEL> class Integrator4D2AngGPU
   {
-  public:
-    using ctype = typename get_type::ctype<NT>;
+    using Shape = dfrg::tmpl::Angle2;
 
-    Integrator4D2AngGPU(QuadratureProvider &quadrature_provider, const std::array<uint, 3> grid_sizes,
-                        const ctype x_extent, const JSONValue &json)
+  public:
+    using ctype = dfrg::ctype_t<NT>;
+
+    template <dfrg::JsonLike JSON>
+    Integrator4D2AngGPU(QuadratureProvider &quadrature_provider, const std::array<unsigned, 3> grid_sizes,
+                        const ctype x_extent, const JSON &json)
         : Integrator4D2AngGPU(quadrature_provider, grid_sizes, x_extent,
                               json.get_uint("/integration/cudathreadsperblock", 256))
     {
     }
-    Integrator4D2AngGPU(QuadratureProvider &quadrature_provider, std::array<uint, 3> grid_sizes, const ctype x_extent,
-                        const uint max_block_size = 256)
-        : grid_sizes(grid_sizes), x_extent(x_extent)
+    Integrator4D2AngGPU(QuadratureProvider &quadrature_provider, std::array<unsigned, 3> grid_sizes,
+                        const ctype x_extent, const unsigned max_block_size = 256)
     {
       (void)max_block_size;
       if (grid_sizes[2] != grid_sizes[1])
         throw std::runtime_error("Grid sizes must be currently equal for all angular dimensions!");
-      ptr_x_quadrature_p = quadrature_provider.get_device_points<ctype>(grid_sizes[0]);
-      ptr_x_quadrature_w = quadrature_provider.get_device_weights<ctype>(grid_sizes[0]);
-      ptr_ang_quadrature_p = quadrature_provider.get_device_points<ctype>(grid_sizes[1]);
-      ptr_ang_quadrature_w = quadrature_provider.get_device_weights<ctype>(grid_sizes[1]);
+      nodes.xq = quadrature_provider.get_device_points<ctype>(grid_sizes[0]);
+      nodes.wq = quadrature_provider.get_device_weights<ctype>(grid_sizes[0]);
+      nodes.nx = (int)grid_sizes[0];
+      nodes.na = (int)grid_sizes[1];
+      nodes.x_extent = x_extent;
+      using dfrg::tmpl::AngKind;
+      ang = dfrg::tmpl::upload_angles(dfrg::tmpl::angle_tables(quadrature_provider.get_points<ctype>(grid_sizes[1]),
+                                                               quadrature_provider.get_weights<ctype>(grid_sizes[1]),
+                                                               {AngKind::cos_jacobian, AngKind::cos}));
+      nodes.ang = ang.get();
     }
 
-    template <typename... T> NT get(const ctype k, const T &...t) const { return launch(k, t...).wait(); }
-
-    // Deferred: requests are batched, and reading one future launches the whole batch
-    // (dfrg::tmpl::RequestBatcher); results are bitwise those of get().
+    template <typename... T> NT get(const ctype k, const T &...t) const
+    {
+      return dfrg::tmpl::get<Shape, NT, KERNEL>(nodes, k, t...);
+    }
     template <typename... T> std::future<NT> request(const ctype k, const T &...t) const
     {
-      using Args = std::tuple<ctype, T...>;
-      const long long nodes = (long long)grid_sizes[0] * grid_sizes[1] * grid_sizes[2];
-      return batcher->enqueue(0ULL, dfrg::tmpl::grid_for(nodes), KERNEL::constant(k, t...), Args(k, t...), [this]() {
-        return [xq = ptr_x_quadrature_p, wq = ptr_x_quadrature_w, xa = ptr_ang_quadrature_p,
-                wa = ptr_ang_quadrature_w, nx = (int)grid_sizes[0], na = (int)grid_sizes[1],
-                xe = x_extent](const Args *a, int r, int nb, NT *part, unsigned *cnt, NT *res, hipStream_t s) {
-          hipLaunchKernelGGL((dfrg::tmpl::angle2_batch_kernel<ctype, NT, KERNEL, Args>), dim3(r * nb),
-                             dim3(dfrg::tmpl::kBlock), 0, s, xq, wq, xa, wa, nx, na, xe, a, nb, part, cnt, res);
-        };
-      });
+      return dfrg::tmpl::request<Shape, NT, KERNEL>(*batcher, 0ULL, nodes, k, t...);
     }
 
   private:
-    template <typename... T> dfrg::tmpl::Pending<NT> launch(const ctype k, const T &...t) const
-    {
-      auto p = dfrg::tmpl::Pending<NT>::make();
-      p.constant = KERNEL::constant(k, t...);
-      const long long nodes = (long long)grid_sizes[0] * grid_sizes[1] * grid_sizes[2];
-      hipLaunchKernelGGL((dfrg::tmpl::angle2_kernel<ctype, NT, KERNEL, T...>), dim3(dfrg::tmpl::grid_for(nodes)),
-                         dim3(dfrg::tmpl::kBlock), 0, p.stream, ptr_x_quadrature_p, ptr_x_quadrature_w,
-                         ptr_ang_quadrature_p, ptr_ang_quadrature_w, (int)grid_sizes[0], (int)grid_sizes[1], x_extent,
-                         k, p.partials(), p.counter(), p.result(), t...);
-      p.enqueue_result();
-      return p;
-    }
-
-    std::array<uint, 3> grid_sizes;
-    ctype x_extent;
-    const ctype *ptr_x_quadrature_p, *ptr_x_quadrature_w, *ptr_ang_quadrature_p, *ptr_ang_quadrature_w;
+    dfrg::tmpl::Nodes<ctype> nodes;
+    std::shared_ptr<dfrg::tmpl::AngNode<ctype>> ang;
     std::shared_ptr<dfrg::tmpl::RequestBatcher<NT>> batcher = std::make_shared<dfrg::tmpl::RequestBatcher<NT>>();
   };
 } // namespace DiFfRG

@@ -8,24 +8,23 @@
 #include <future>
 #include <vector>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 namespace DiFfRG
 {
   template <typename NT, typename KERNEL> class Integrator4DTBB
   {
   public:
-    using ctype = typename get_type::ctype<NT>;
-    Integrator4DTBB(QuadratureProvider &qp, const std::array<uint, 4> grid_sizes, const ctype x_extent,
-                    const JSONValue &)
+    using ctype = typename dfrg::ctype_t<NT>;
+    template <dfrg::JsonLike JSON>
+    Integrator4DTBB(QuadratureProvider &qp, const std::array<unsigned, 4> grid_sizes, const ctype x_extent,
+                    const JSON &)
         : Integrator4DTBB(qp, grid_sizes, x_extent)
     {
     }
-    Integrator4DTBB(QuadratureProvider &qp, std::array<uint, 4> grid_sizes, const ctype x_extent,
-                    const uint max_block_size = 0)
+    Integrator4DTBB(QuadratureProvider &qp, std::array<unsigned, 4> grid_sizes, const ctype x_extent,
+                    const unsigned max_block_size = 0)
         : grid_sizes(grid_sizes), x_extent(x_extent), xp(qp.get_points<ctype>(grid_sizes[0])),
           xw(qp.get_weights<ctype>(grid_sizes[0])), p1(qp.get_points<ctype>(grid_sizes[1])),
           w1(qp.get_weights<ctype>(grid_sizes[1])), p2(qp.get_points<ctype>(grid_sizes[2])),
@@ -42,13 +41,13 @@ namespace DiFfRG
       for (int idx_x = 0; idx_x < (int)grid_sizes[0]; ++idx_x) {
         const ctype q = k * std::sqrt(xp[idx_x] * x_extent);
         NT value = NT(0);
-        for (uint i1 = 0; i1 < grid_sizes[1]; ++i1) {
+        for (unsigned i1 = 0; i1 < grid_sizes[1]; ++i1) {
           const ctype cos1 = 2 * (p1[i1] - (ctype)0.5);
           const ctype int_element =
-              (powr<2>(q) * (ctype)0.5 * powr<2>(k)) * std::sqrt(1. - powr<2>(cos1)) / powr<4>(2 * (ctype)M_PI);
-          for (uint i2 = 0; i2 < grid_sizes[2]; ++i2) {
+              (dfrg::powr<2>(q) * (ctype)0.5 * dfrg::powr<2>(k)) * std::sqrt(1. - dfrg::powr<2>(cos1)) / dfrg::powr<4>(2 * (ctype)M_PI);
+          for (unsigned i2 = 0; i2 < grid_sizes[2]; ++i2) {
             const ctype cos2 = 2 * (p2[i2] - (ctype)0.5);
-            for (uint i3 = 0; i3 < grid_sizes[3]; ++i3) {
+            for (unsigned i3 = 0; i3 < grid_sizes[3]; ++i3) {
               const ctype phi = 2 * (ctype)M_PI * p3[i3];
               const ctype weight = 2 * (ctype)M_PI * w3[i3] * 2 * w2[i2] * 2 * w1[i1] * xw[idx_x] * x_extent;
               value += int_element * weight * KERNEL::kernel(q, cos1, cos2, phi, k, t...);
@@ -68,7 +67,7 @@ namespace DiFfRG
     }
 
   private:
-    const std::array<uint, 4> grid_sizes;
+    const std::array<unsigned, 4> grid_sizes;
     const ctype x_extent;
     const std::vector<ctype> &xp, &xw, &p1, &w1, &p2, &w2, &p3, &w3;
   };

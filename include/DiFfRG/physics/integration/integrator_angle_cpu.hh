@@ -7,10 +7,8 @@
 #include <future>
 #include <vector>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 namespace DiFfRG
 {
@@ -19,15 +17,16 @@ namespace DiFfRG
     static_assert(d > 1, "IntegratorAngleTBB: d must be greater than 1, otherwise angles are not needed");
 
   public:
-    using ctype = typename get_type::ctype<NT>;
+    using ctype = typename dfrg::ctype_t<NT>;
 
-    IntegratorAngleTBB(QuadratureProvider &quadrature_provider, const std::array<uint, 2> grid_sizes,
-                       const ctype x_extent, const JSONValue &)
+    template <dfrg::JsonLike JSON>
+    IntegratorAngleTBB(QuadratureProvider &quadrature_provider, const std::array<unsigned, 2> grid_sizes,
+                       const ctype x_extent, const JSON &)
         : IntegratorAngleTBB(quadrature_provider, grid_sizes, x_extent)
     {
     }
-    IntegratorAngleTBB(QuadratureProvider &quadrature_provider, std::array<uint, 2> grid_sizes, const ctype x_extent,
-                       const uint max_block_size = 0)
+    IntegratorAngleTBB(QuadratureProvider &quadrature_provider, std::array<unsigned, 2> grid_sizes, const ctype x_extent,
+                       const unsigned max_block_size = 0)
         : grid_sizes(grid_sizes), x_extent(x_extent),
           x_quadrature_p(quadrature_provider.get_points<ctype>(grid_sizes[0])),
           x_quadrature_w(quadrature_provider.get_weights<ctype>(grid_sizes[0])),
@@ -39,15 +38,15 @@ namespace DiFfRG
 
     template <typename... T> NT get(const ctype k, const T &...t) const
     {
-      constexpr ctype S_d = S_d_prec<ctype>(d);
+      constexpr ctype S_d = dfrg::S_d_prec<ctype>(d);
       std::vector<NT> part(grid_sizes[0], NT(0));
 #pragma omp parallel for schedule(static)
       for (int idx_x = 0; idx_x < (int)grid_sizes[0]; ++idx_x) {
         const ctype q = k * std::sqrt(x_quadrature_p[idx_x] * x_extent);
-        const ctype int_element = S_d * (powr<d - 2>(q) / (ctype)2 * powr<2>(k)) * (1 / (ctype)2) /
-                                  powr<d>(2 * (ctype)M_PI);
+        const ctype int_element = S_d * (dfrg::powr<d - 2>(q) / (ctype)2 * dfrg::powr<2>(k)) * (1 / (ctype)2) /
+                                  dfrg::powr<d>(2 * (ctype)M_PI);
         NT value = NT(0);
-        for (uint idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
+        for (unsigned idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
           const ctype cos = 2 * (ang_quadrature_p[idx_y] - (ctype)0.5);
           const ctype weight = 2 * ang_quadrature_w[idx_y] * x_quadrature_w[idx_x] * x_extent;
           value += int_element * weight * KERNEL::kernel(q, cos, k, t...);
@@ -65,7 +64,7 @@ namespace DiFfRG
     }
 
   private:
-    std::array<uint, 2> grid_sizes;
+    std::array<unsigned, 2> grid_sizes;
     const ctype x_extent;
     const std::vector<ctype> &x_quadrature_p, &x_quadrature_w, &ang_quadrature_p, &ang_quadrature_w;
   };

@@ -8,27 +8,26 @@
 #include <future>
 #include <vector>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 namespace DiFfRG
 {
   template <int d, typename NT, typename KERNEL> class IntegratorTBB
   {
   public:
-    using ctype = typename get_type::ctype<NT>;
-    IntegratorTBB(QuadratureProvider &qp, const std::array<uint, 1> grid_size, const ctype x_extent,
-                  const uint max_block_size = 0)
+    using ctype = typename dfrg::ctype_t<NT>;
+    IntegratorTBB(QuadratureProvider &qp, const std::array<unsigned, 1> grid_size, const ctype x_extent,
+                  const unsigned max_block_size = 0)
         : IntegratorTBB(qp, grid_size[0], x_extent, max_block_size)
     {
     }
-    IntegratorTBB(QuadratureProvider &qp, const std::array<uint, 1> grid_size, const ctype x_extent, const JSONValue &)
+    template <dfrg::JsonLike JSON>
+    IntegratorTBB(QuadratureProvider &qp, const std::array<unsigned, 1> grid_size, const ctype x_extent, const JSON &)
         : IntegratorTBB(qp, grid_size[0], x_extent)
     {
     }
-    IntegratorTBB(QuadratureProvider &qp, const uint grid_size, const ctype x_extent, const uint max_block_size = 0)
+    IntegratorTBB(QuadratureProvider &qp, const unsigned grid_size, const ctype x_extent, const unsigned max_block_size = 0)
         : grid_size(grid_size), x_extent(x_extent), x_quadrature_p(qp.get_points<ctype>(grid_size)),
           x_quadrature_w(qp.get_weights<ctype>(grid_size))
     {
@@ -37,12 +36,12 @@ namespace DiFfRG
 
     template <typename... T> NT get(const ctype k, const T &...t) const
     {
-      constexpr ctype S_d = S_d_prec<ctype>(d);
+      constexpr ctype S_d = dfrg::S_d_prec<ctype>(d);
       std::vector<NT> part(grid_size, NT(0));
 #pragma omp parallel for schedule(static)
       for (int idx_x = 0; idx_x < (int)grid_size; ++idx_x) {
         const ctype q = k * std::sqrt(x_quadrature_p[idx_x] * x_extent);
-        const ctype int_element = S_d * (powr<d - 2>(q) / (ctype)2 * powr<2>(k)) / powr<d>(2 * (ctype)M_PI);
+        const ctype int_element = S_d * (dfrg::powr<d - 2>(q) / (ctype)2 * dfrg::powr<2>(k)) / dfrg::powr<d>(2 * (ctype)M_PI);
         const ctype weight = x_quadrature_w[idx_x] * x_extent;
         part[idx_x] = int_element * weight * KERNEL::kernel(q, k, t...);
       }
@@ -57,7 +56,7 @@ namespace DiFfRG
     }
 
   private:
-    const uint grid_size;
+    const unsigned grid_size;
     const ctype x_extent;
     const std::vector<ctype> &x_quadrature_p;
     const std::vector<ctype> &x_quadrature_w;

@@ -8,10 +8,8 @@
 #include <future>
 #include <vector>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 #include "dfrg/matsubara_state.hh"
 
@@ -20,18 +18,19 @@ namespace DiFfRG
   template <typename NT, typename KERNEL> class Integrator4DFiniteTq0TBB
   {
   public:
-    using ctype = typename get_type::ctype<NT>;
+    using ctype = typename dfrg::ctype_t<NT>;
 
-    Integrator4DFiniteTq0TBB(QuadratureProvider &quadrature_provider, const std::array<uint, 4> grid_sizes,
-                             const ctype x_extent, const ctype q0_extent, const uint q0_summands,
-                             const JSONValue &json)
+    template <dfrg::JsonLike JSON>
+    Integrator4DFiniteTq0TBB(QuadratureProvider &quadrature_provider, const std::array<unsigned, 4> grid_sizes,
+                             const ctype x_extent, const ctype q0_extent, const unsigned q0_summands,
+                             const JSON &json)
         : Integrator4DFiniteTq0TBB(quadrature_provider, grid_sizes, x_extent, json.get_double("/physical/T"))
     {
       (void)q0_extent;
       (void)q0_summands;
     }
-    Integrator4DFiniteTq0TBB(QuadratureProvider &quadrature_provider, std::array<uint, 4> grid_sizes,
-                             const ctype x_extent, const ctype T, const uint max_block_size = 0)
+    Integrator4DFiniteTq0TBB(QuadratureProvider &quadrature_provider, std::array<unsigned, 4> grid_sizes,
+                             const ctype x_extent, const ctype T, const unsigned max_block_size = 0)
         : grid_sizes(grid_sizes), x_extent(x_extent)
     {
       (void)max_block_size;
@@ -55,14 +54,14 @@ namespace DiFfRG
 #pragma omp parallel for schedule(static)
       for (int idx_x = 0; idx_x < (int)grid_sizes[0]; ++idx_x) {
         const ctype q = k * std::sqrt(xq[idx_x] * x_extent);
-        const ctype int_element = (powr<1>(q) / (ctype)2 * powr<2>(k)) / powr<3>(2 * (ctype)M_PI);
+        const ctype int_element = (dfrg::powr<1>(q) / (ctype)2 * dfrg::powr<2>(k)) / dfrg::powr<3>(2 * (ctype)M_PI);
         NT value = NT(0);
-        for (uint idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
+        for (unsigned idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
           const ctype cos = 2 * (xc[idx_y] - (ctype)0.5);
-          for (uint idx_z = 0; idx_z < grid_sizes[2]; ++idx_z) {
+          for (unsigned idx_z = 0; idx_z < grid_sizes[2]; ++idx_z) {
             const ctype phi = 2 * (ctype)M_PI * xp[idx_z];
             const ctype x_weight = 2 * (ctype)M_PI * wp[idx_z] * 2 * wc[idx_y] * wq[idx_x] * x_extent;
-            for (uint idx_0 = 0; idx_0 < mats.size; ++idx_0) {
+            for (unsigned idx_0 = 0; idx_0 < mats.size; ++idx_0) {
               const ctype q0 = mats.x[idx_0];
               const ctype weight = x_weight * mats.w[idx_0];
               value += int_element * weight *
@@ -85,7 +84,7 @@ namespace DiFfRG
     }
 
   private:
-    std::array<uint, 4> grid_sizes;
+    std::array<unsigned, 4> grid_sizes;
     const ctype x_extent;
     const ctype *xq, *wq, *xc, *wc, *xp, *wp;
     dfrg::tmpl::MatsubaraState<ctype, false> mats;

@@ -8,10 +8,8 @@
 #include <future>
 #include <vector>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 #include "dfrg/matsubara_state.hh"
 
@@ -22,15 +20,16 @@ namespace DiFfRG
     static_assert(d >= 3, "dimension must be at least 3");
 
   public:
-    using ctype = typename get_type::ctype<NT>;
+    using ctype = typename dfrg::ctype_t<NT>;
 
-    IntegratorAngleFiniteTq0TBB(QuadratureProvider &quadrature_provider, const std::array<uint, 2> grid_sizes,
-                                const ctype x_extent, const JSONValue &json)
+    template <dfrg::JsonLike JSON>
+    IntegratorAngleFiniteTq0TBB(QuadratureProvider &quadrature_provider, const std::array<unsigned, 2> grid_sizes,
+                                const ctype x_extent, const JSON &json)
         : IntegratorAngleFiniteTq0TBB(quadrature_provider, grid_sizes, x_extent, json.get_double("/physical/T"))
     {
     }
-    IntegratorAngleFiniteTq0TBB(QuadratureProvider &quadrature_provider, std::array<uint, 2> grid_sizes,
-                                const ctype x_extent, const ctype T, const uint max_block_size = 0)
+    IntegratorAngleFiniteTq0TBB(QuadratureProvider &quadrature_provider, std::array<unsigned, 2> grid_sizes,
+                                const ctype x_extent, const ctype T, const unsigned max_block_size = 0)
         : grid_sizes(grid_sizes), x_extent(x_extent)
     {
       (void)max_block_size;
@@ -48,18 +47,18 @@ namespace DiFfRG
     template <typename... T> NT get(const ctype k, const T &...t)
     {
       mats.track(k);
-      constexpr ctype S_dm1 = S_d_prec<ctype>(d - 1);
+      constexpr ctype S_dm1 = dfrg::S_d_prec<ctype>(d - 1);
       std::vector<NT> part(grid_sizes[0], NT(0));
 #pragma omp parallel for schedule(static)
       for (int idx_x = 0; idx_x < (int)grid_sizes[0]; ++idx_x) {
         const ctype q = k * std::sqrt(xq[idx_x] * x_extent);
-        const ctype int_element = S_dm1 * (powr<d - 3>(q) / (ctype)2 * powr<2>(k)) * (1 / (ctype)2) /
-                                  powr<d - 1>(2 * (ctype)M_PI);
+        const ctype int_element = S_dm1 * (dfrg::powr<d - 3>(q) / (ctype)2 * dfrg::powr<2>(k)) * (1 / (ctype)2) /
+                                  dfrg::powr<d - 1>(2 * (ctype)M_PI);
         NT value = NT(0);
-        for (uint idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
+        for (unsigned idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
           const ctype cos = 2 * (xc[idx_y] - (ctype)0.5);
           const ctype x_weight = 2 * wc[idx_y] * wq[idx_x] * x_extent;
-          for (uint idx_z = 0; idx_z < mats.size; ++idx_z) {
+          for (unsigned idx_z = 0; idx_z < mats.size; ++idx_z) {
             const ctype q0 = mats.x[idx_z];
             const ctype weight = x_weight * mats.w[idx_z];
             value += int_element * weight * (KERNEL::kernel(q, cos, q0, k, t...) + KERNEL::kernel(q, cos, -q0, k, t...));
@@ -79,7 +78,7 @@ namespace DiFfRG
     }
 
   private:
-    std::array<uint, 2> grid_sizes;
+    std::array<unsigned, 2> grid_sizes;
     const ctype x_extent;
     const ctype *xq, *wq, *xc, *wc;
     dfrg::tmpl::MatsubaraState<ctype, false> mats;

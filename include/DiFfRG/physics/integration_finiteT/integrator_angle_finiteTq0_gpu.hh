@@ -2,21 +2,19 @@
 // IntegratorAngleFiniteTq0GPU<d, NT, KERNEL> (reference integrator_angle_finiteTq0_gpu.hh:52-226):
 // (d-1) spatial dimensions with one angle and the Matsubara sum,
 //   S_{d-1}/(2pi)^{d-1} int dq q^{d-2} (1/2) int dcos [ sum_i w_i (f(q,cos,x_i) + f(q,cos,-x_i)) + T f(q,cos,0) ]
-// with KERNEL::kernel(q, cos, q0, k, t...).  The rule follows E (dfrg::tmpl::MatsubaraState).
-// One launch per get() and per batch of request()s.
+// with KERNEL::kernel(q, cos, q0, k, t...), on the tile engine of dfrg/integrate.hh.  The rule
+// follows E (dfrg::tmpl::MatsubaraState); a new rule starts a new batch of requests.
 #pragma once
 
 #include <array>
+#include <cstdint>
 #include <future>
 #include <memory>
-#include <tuple>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 #include "dfrg/integrate.hh"
+#include "dfrg/math.hh"
 #include "dfrg/matsubara_state.hh"
 
 namespace DiFfRG
@@ -24,71 +22,65 @@ namespace DiFfRG
   template <int d, typename NT, typename KERNEL> class IntegratorAngleFiniteTq0GPU
   {
     static_assert(d >= 3, "dimension must be at least 3");
+    using Shape = dfrg::tmpl::Angle1FiniteT<d>;
 
   public:
-    using ctype = typename get_type::ctype<NT>;
+    using ctype = dfrg::ctype_t<NT>;
 
-    IntegratorAngleFiniteTq0GPU(QuadratureProvider &quadrature_provider, const std::array<uint, 2> grid_sizes,
-                                const ctype x_extent, const JSONValue &json)
+    template <dfrg::JsonLike JSON>
+    IntegratorAngleFiniteTq0GPU(QuadratureProvider &quadrature_provider, const std::array<unsigned, 2> grid_sizes,
+                                const ctype x_extent, const JSON &json)
         : IntegratorAngleFiniteTq0GPU(quadrature_provider, grid_sizes, x_extent, json.get_double("/physical/T"),
                                       json.get_uint("/integration/cudathreadsperblock", 256))
     {
     }
-    IntegratorAngleFiniteTq0GPU(QuadratureProvider &quadrature_provider, const std::array<uint, 2> grid_sizes,
-                                const ctype x_extent, const ctype T, const uint max_block_size = 256)
-        : grid_sizes(grid_sizes), x_extent(x_extent)
+    IntegratorAngleFiniteTq0GPU(QuadratureProvider &quadrature_provider, const std::array<unsigned, 2> grid_sizes,
+                                const ctype x_extent, const ctype T, const unsigned max_block_size = 256)
     {
       (void)max_block_size;
-      ptr_x_quadrature_p = quadrature_provider.get_device_points<ctype>(grid_sizes[0]);
-      ptr_x_quadrature_w = quadrature_provider.get_device_weights<ctype>(grid_sizes[0]);
-      ptr_ang_quadrature_p = quadrature_provider.get_device_points<ctype>(grid_sizes[1]);
-      ptr_ang_quadrature_w = quadrature_provider.get_device_weights<ctype>(grid_sizes[1]);
+      base.xq = quadrature_provider.get_device_points<ctype>(grid_sizes[0]);
+      base.wq = quadrature_provider.get_device_weights<ctype>(grid_sizes[0]);
+      base.nx = (int)grid_sizes[0];
+      base.na = (int)grid_sizes[1];
+      base.x_extent = x_extent;
+      using dfrg::tmpl::AngKind;
+      ang = dfrg::tmpl::upload_angles(dfrg::tmpl::angle_tables(quadrature_provider.get_points<ctype>(grid_sizes[1]),
+                                                               quadrature_provider.get_weights<ctype>(grid_sizes[1]),
+                                                               {AngKind::cos}));
+      base.ang = ang.get();
       mats.provider = &quadrature_provider;
       mats.set_T(T);
     }
 
     void set_T(const ctype T, const ctype E = 0) { mats.set_T(T, E); }
     void set_E(const ctype E) { mats.set_E(E); }
-    uint matsubara_size() const { return mats.size; }
+    unsigned matsubara_size() const { return mats.size; }
 
     template <typename... T> NT get(const ctype k, const T &...t)
     {
       mats.track(k);
-      auto p = dfrg::tmpl::Pending<NT>::make();
-      p.constant = KERNEL::constant(k, t...);
-      hipLaunchKernelGGL((dfrg::tmpl::angle1_finiteT_kernel<d, ctype, NT, KERNEL, T...>),
-                         dim3(dfrg::tmpl::grid_for(nodes())), dim3(dfrg::tmpl::kBlock), 0, p.stream,
-                         ptr_x_quadrature_p, ptr_x_quadrature_w, ptr_ang_quadrature_p, ptr_ang_quadrature_w, mats.x,
-                         mats.w, (int)grid_sizes[0], (int)grid_sizes[1], (int)mats.size, x_extent, mats.T, k,
-                         p.partials(), p.counter(), p.result(), t...);
-      p.enqueue_result();
-      return p.wait();
+      return dfrg::tmpl::get<Shape, NT, KERNEL>(nodes(), k, t...);
     }
-
-    // Deferred and batched (dfrg::tmpl::RequestBatcher); a new Matsubara rule starts a new batch.
     template <typename... T> std::future<NT> request(const ctype k, const T &...t)
     {
       mats.track(k);
-      using Args = std::tuple<ctype, T...>;
-      const unsigned long long state = (unsigned long long)(uintptr_t)mats.x;
-      return batcher->enqueue(state, dfrg::tmpl::grid_for(nodes()), KERNEL::constant(k, t...), Args(k, t...), [this]() {
-        return [xq = ptr_x_quadrature_p, wq = ptr_x_quadrature_w, xa = ptr_ang_quadrature_p,
-                wa = ptr_ang_quadrature_w, mx = mats.x, mw = mats.w, nx = (int)grid_sizes[0], na = (int)grid_sizes[1],
-                nm = (int)mats.size, xe = x_extent,
-                temp = mats.T](const Args *a, int r, int nb, NT *part, unsigned *cnt, NT *res, hipStream_t s) {
-          hipLaunchKernelGGL((dfrg::tmpl::angle1_finiteT_batch_kernel<d, ctype, NT, KERNEL, Args>), dim3(r * nb),
-                             dim3(dfrg::tmpl::kBlock), 0, s, xq, wq, xa, wa, mx, mw, nx, na, nm, xe, temp, a, nb, part,
-                             cnt, res);
-        };
-      });
+      return dfrg::tmpl::request<Shape, NT, KERNEL>(*batcher, (unsigned long long)(uintptr_t)mats.x, nodes(), k,
+                                                    t...);
     }
 
   private:
-    long long nodes() const { return (long long)grid_sizes[0] * grid_sizes[1] * mats.size; }
+    dfrg::tmpl::Nodes<ctype> nodes() const
+    {
+      dfrg::tmpl::Nodes<ctype> n = base;
+      n.mx = mats.x;
+      n.mw = mats.w;
+      n.nm = (int)mats.size;
+      n.m_T = mats.T;
+      return n;
+    }
 
-    std::array<uint, 2> grid_sizes;
-    ctype x_extent;
-    const ctype *ptr_x_quadrature_p, *ptr_x_quadrature_w, *ptr_ang_quadrature_p, *ptr_ang_quadrature_w;
+    dfrg::tmpl::Nodes<ctype> base;
+    std::shared_ptr<dfrg::tmpl::AngNode<ctype>> ang;
     dfrg::tmpl::MatsubaraState<ctype, true> mats;
     std::shared_ptr<dfrg::tmpl::RequestBatcher<NT>> batcher = std::make_shared<dfrg::tmpl::RequestBatcher<NT>>();
   };

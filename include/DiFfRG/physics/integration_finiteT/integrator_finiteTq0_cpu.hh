@@ -10,10 +10,8 @@
 #include <limits>
 #include <vector>
 
-#include <DiFfRG/common/json.hh>
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
-#include <DiFfRG/common/types.hh>
 
 namespace DiFfRG
 {
@@ -22,15 +20,16 @@ namespace DiFfRG
     static_assert(d >= 2, "dimension must be at least 2");
 
   public:
-    using ctype = typename get_type::ctype<NT>;
+    using ctype = typename dfrg::ctype_t<NT>;
 
-    IntegratorFiniteTq0TBB(QuadratureProvider &qp, const std::array<uint, 1> grid_sizes, const ctype x_extent,
-                           const JSONValue &json)
+    template <dfrg::JsonLike JSON>
+    IntegratorFiniteTq0TBB(QuadratureProvider &qp, const std::array<unsigned, 1> grid_sizes, const ctype x_extent,
+                           const JSON &json)
         : IntegratorFiniteTq0TBB(qp, grid_sizes, x_extent, json.get_double("/physical/T"))
     {
     }
-    IntegratorFiniteTq0TBB(QuadratureProvider &qp, const std::array<uint, 1> _grid_sizes, const ctype x_extent,
-                           const ctype T, const uint max_block_size = 0)
+    IntegratorFiniteTq0TBB(QuadratureProvider &qp, const std::array<unsigned, 1> _grid_sizes, const ctype x_extent,
+                           const ctype T, const unsigned max_block_size = 0)
         : quadrature_provider(qp), grid_sizes{{_grid_sizes[0], 0}}, x_extent(x_extent),
           m_T(std::numeric_limits<ctype>::quiet_NaN()), m_E(0), manual_E(false)
     {
@@ -42,12 +41,12 @@ namespace DiFfRG
 
     void set_T(const ctype T, const ctype E = 0)
     {
-      if (is_close(T, m_T) && (std::abs(E - m_E) / std::max(E, m_E) < 2.5e-2)) return;
+      if (dfrg::is_close(T, m_T) && (std::abs(E - m_E) / std::max(E, m_E) < 2.5e-2)) return;
       m_T = T;
-      m_E = is_close(E, (ctype)0) ? 10 * m_T : E;
-      manual_E = !is_close(E, (ctype)0);
+      m_E = dfrg::is_close(E, (ctype)0) ? 10 * m_T : E;
+      manual_E = !dfrg::is_close(E, (ctype)0);
       const auto &mp = quadrature_provider.get_matsubara_points<ctype>(m_T, m_E);
-      grid_sizes[1] = (uint)mp.size();
+      grid_sizes[1] = (unsigned)mp.size();
       ptr_matsubara_quadrature_p = mp.data();
       ptr_matsubara_quadrature_w = quadrature_provider.get_matsubara_weights<ctype>(m_T, m_E).data();
     }
@@ -59,15 +58,15 @@ namespace DiFfRG
         set_T(m_T, k);
         manual_E = false;
       }
-      constexpr ctype S_dm1 = S_d_prec<ctype>(d - 1);
+      constexpr ctype S_dm1 = dfrg::S_d_prec<ctype>(d - 1);
       std::vector<NT> part(grid_sizes[0], NT(0));
 #pragma omp parallel for schedule(static)
       for (int idx_x = 0; idx_x < (int)grid_sizes[0]; ++idx_x) {
         const ctype q = k * std::sqrt(ptr_x_quadrature_p[idx_x] * x_extent);
         const ctype x_weight = ptr_x_quadrature_w[idx_x] * x_extent;
-        const ctype int_element = S_dm1 * (powr<d - 3>(q) / (ctype)2 * powr<2>(k)) / powr<d - 1>(2 * (ctype)M_PI);
+        const ctype int_element = S_dm1 * (dfrg::powr<d - 3>(q) / (ctype)2 * dfrg::powr<2>(k)) / dfrg::powr<d - 1>(2 * (ctype)M_PI);
         NT value = NT(0);
-        for (uint idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
+        for (unsigned idx_y = 0; idx_y < grid_sizes[1]; ++idx_y) {
           const ctype q0 = ptr_matsubara_quadrature_p[idx_y];
           const ctype weight = x_weight * ptr_matsubara_quadrature_w[idx_y];
           value += int_element * weight * (KERNEL::kernel(q, q0, k, t...) + KERNEL::kernel(q, -q0, k, t...));
@@ -87,7 +86,7 @@ namespace DiFfRG
 
   private:
     QuadratureProvider &quadrature_provider;
-    std::array<uint, 2> grid_sizes;
+    std::array<unsigned, 2> grid_sizes;
     const ctype x_extent;
     ctype m_T, m_E;
     bool manual_E;

@@ -13,8 +13,8 @@
 #include <stdexcept>
 #include <vector>
 
-#include <DiFfRG/common/hip_prefix.hh>
-#include <DiFfRG/common/math.hh>
+#include "dfrg/check.hh"
+#include "dfrg/math.hh"
 #include <DiFfRG/discretization/grid/coordinates.hh>
 
 namespace DiFfRG
@@ -36,11 +36,11 @@ namespace DiFfRG
     LinearInterpolator1D(const NT *in_data, const Coordinates &coordinates)
         : size(coordinates.size()), coordinates(coordinates), owner(true)
     {
-      ensure_device();
+      dfrg::ensure_device();
       m_data = std::shared_ptr<NT[]>(new NT[size]);
       for (unsigned i = 0; i < size; ++i) m_data[i] = NT(in_data[i]);
       void *p = nullptr;
-      dfrg_check(dfrg_malloc(&p, sizeof(NT) * size), "LinearInterpolator1D");
+      dfrg::check(dfrg_malloc(&p, sizeof(NT) * size), "LinearInterpolator1D");
       device_data = std::shared_ptr<void>(p, [](void *q) { dfrg_free(q); });
       device_data_ptr = (const NT *)p;
       update();
@@ -60,9 +60,9 @@ namespace DiFfRG
     void update()
     {
       if (!owner) throw std::runtime_error("Cannot update data of non-owner interpolator");
-      dfrg_check(dfrg_memcpy_h2d((void *)device_data_ptr, m_data.get(), sizeof(NT) * size, nullptr),
+      dfrg::check(dfrg_memcpy_h2d((void *)device_data_ptr, m_data.get(), sizeof(NT) * size, nullptr),
                  "LinearInterpolator1D::update");
-      dfrg_check(dfrg_stream_sync(nullptr), "LinearInterpolator1D::update");
+      dfrg::check(dfrg_stream_sync(nullptr), "LinearInterpolator1D::update");
     }
 
     NT *data() const { return m_data.get(); }

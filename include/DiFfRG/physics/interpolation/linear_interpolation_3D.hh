@@ -7,9 +7,9 @@
 #include <array>
 #include <vector>
 
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/discretization/grid/coordinates.hh>
-#include <DiFfRG/physics/interpolation/common.hh>
+#include "dfrg/interp.hh"
 
 namespace DiFfRG
 {
@@ -30,9 +30,9 @@ namespace DiFfRG
     }
     LinearInterpolator3D(const Coordinates &coordinates)
         : size(coordinates.size()), shape(coordinates.sizes()), coordinates(coordinates),
-          table(std::make_shared<internal::TableStorage<NT>>(coordinates.size())), owner(true)
+          table(std::make_shared<dfrg::interp::TableStorage<NT>>(coordinates.size())), owner(true)
     {
-      internal::require_axes(shape.data(), 3, "LinearInterpolator3D");
+      dfrg::interp::require_axes(shape.data(), 3, "LinearInterpolator3D");
       d_table = table->device_ptr();
       h_table = table->host_ptr();
       table->upload();
@@ -67,14 +67,14 @@ namespace DiFfRG
 #else
       const NT *d = h_table;
 #endif
-      return internal::lerp3(d, C(idx[0]), C(idx[1]), C(idx[2]), shape[0], shape[1], shape[2]);
+      return dfrg::interp::lerp3(d, C(idx[0]), C(idx[1]), C(idx[2]), shape[0], shape[1], shape[2]);
     }
 
   private:
     unsigned size;
     std::array<unsigned, 3> shape;
     Coordinates coordinates;
-    std::shared_ptr<internal::TableStorage<NT>> table;
+    std::shared_ptr<dfrg::interp::TableStorage<NT>> table;
     const NT *d_table = nullptr;
     NT *h_table = nullptr;
     bool owner;

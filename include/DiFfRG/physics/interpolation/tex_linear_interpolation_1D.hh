@@ -9,9 +9,9 @@
 
 #include <vector>
 
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/discretization/grid/coordinates.hh>
-#include <DiFfRG/physics/interpolation/common.hh>
+#include "dfrg/interp.hh"
 
 namespace DiFfRG
 {
@@ -20,7 +20,7 @@ namespace DiFfRG
     static_assert(Coordinates::dim == 1, "TexLinearInterpolator1D requires 1D coordinates");
 
   public:
-    using ReturnType = typename internal::__TLITypes<NT>::ReturnType;
+    using ReturnType = typename dfrg::interp::TexReturn<NT>::type;
 
     TexLinearInterpolator1D(const std::vector<NT> &data, const Coordinates &coordinates)
         : TexLinearInterpolator1D(data.data(), coordinates)
@@ -35,7 +35,7 @@ namespace DiFfRG
         : size(coordinates.size()), coordinates(coordinates), owner(true)
     {
       if (size < 2) throw std::runtime_error("TexLinearInterpolator1D: the table needs at least 2 points");
-      tables = std::make_shared<internal::TexTables<NT>>(size);
+      tables = std::make_shared<dfrg::interp::TexTables<NT>>(size);
       d_table = tables->device();
       d_table_AD = tables->device_AD();
       h_table = tables->host();
@@ -62,14 +62,14 @@ namespace DiFfRG
     float *data_AD() const { return h_table_AD; }
     ReturnType &operator[](const unsigned i)
     {
-      if constexpr (internal::DualTraits<NT>::is_dual)
+      if constexpr (dfrg::DualTraits<NT>::is_dual)
         throw std::runtime_error("Cannot access dual-number data directly");
       else
         return h_table[i];
     }
     const ReturnType &operator[](const unsigned i) const
     {
-      if constexpr (internal::DualTraits<NT>::is_dual)
+      if constexpr (dfrg::DualTraits<NT>::is_dual)
         throw std::runtime_error("Cannot access dual-number data directly");
       else
         return h_table[i];
@@ -80,18 +80,18 @@ namespace DiFfRG
     {
       const float idx = float(coordinates.backward(x));
       const unsigned n = size;
-      const auto lerp = [idx, n](const float *t) { return internal::lerp1(t, idx, n); };
+      const auto lerp = [idx, n](const float *t) { return dfrg::interp::lerp1(t, idx, n); };
 #ifdef __HIP_DEVICE_COMPILE__
-      return internal::tex_value<NT>(d_table, d_table_AD, lerp);
+      return dfrg::interp::tex_value<NT>(d_table, d_table_AD, lerp);
 #else
-      return internal::tex_value<NT>(h_table, h_table_AD, lerp);
+      return dfrg::interp::tex_value<NT>(h_table, h_table_AD, lerp);
 #endif
     }
 
   private:
     unsigned size;
     Coordinates coordinates;
-    std::shared_ptr<internal::TexTables<NT>> tables;
+    std::shared_ptr<dfrg::interp::TexTables<NT>> tables;
     const float *d_table = nullptr, *d_table_AD = nullptr;
     float *h_table = nullptr, *h_table_AD = nullptr;
     bool owner;

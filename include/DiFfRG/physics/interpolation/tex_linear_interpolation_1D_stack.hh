@@ -9,9 +9,9 @@
 
 #include <vector>
 
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/discretization/grid/combined_coordinates.hh>
-#include <DiFfRG/physics/interpolation/common.hh>
+#include "dfrg/interp.hh"
 
 namespace DiFfRG
 {
@@ -20,7 +20,7 @@ namespace DiFfRG
     static_assert(Coordinates::dim == 2, "TexLinearInterpolator1DStack requires 2D coordinates");
 
   public:
-    using ReturnType = typename internal::__TLITypes<NT>::ReturnType;
+    using ReturnType = typename dfrg::interp::TexReturn<NT>::type;
 
     TexLinearInterpolator1DStack(const std::vector<NT> &data, const Coordinates &coordinates)
         : TexLinearInterpolator1DStack(data.data(), coordinates)
@@ -38,7 +38,7 @@ namespace DiFfRG
       if (stack_size > max_stack_size)
         throw std::runtime_error("TexLinearInterpolator1DStack: stack size exceeds max_stack_size");
       if (p_size < 2) throw std::runtime_error("TexLinearInterpolator1DStack: every table needs at least 2 points");
-      tables = std::make_shared<internal::TexTables<NT>>(stack_size * p_size);
+      tables = std::make_shared<dfrg::interp::TexTables<NT>>(stack_size * p_size);
       d_table = tables->device();
       d_table_AD = tables->device_AD();
       h_table = tables->host();
@@ -71,14 +71,14 @@ namespace DiFfRG
     float *data_AD(const unsigned s) const { return h_table_AD ? h_table_AD + (size_t)s * p_size : nullptr; }
     ReturnType &operator[](const unsigned i)
     {
-      if constexpr (internal::DualTraits<NT>::is_dual)
+      if constexpr (dfrg::DualTraits<NT>::is_dual)
         throw std::runtime_error("Cannot access dual-number data directly");
       else
         return h_table[i];
     }
     const ReturnType &operator[](const unsigned i) const
     {
-      if constexpr (internal::DualTraits<NT>::is_dual)
+      if constexpr (dfrg::DualTraits<NT>::is_dual)
         throw std::runtime_error("Cannot access dual-number data directly");
       else
         return h_table[i];
@@ -93,18 +93,18 @@ namespace DiFfRG
       const float ip = float(std::get<1>(mp));
       const unsigned n = p_size;
       const size_t off = (size_t)s * p_size;
-      const auto lerp = [=](const float *t) { return internal::lerp1(t + off, ip, n); };
+      const auto lerp = [=](const float *t) { return dfrg::interp::lerp1(t + off, ip, n); };
 #ifdef __HIP_DEVICE_COMPILE__
-      return internal::tex_value<NT>(d_table, d_table_AD, lerp);
+      return dfrg::interp::tex_value<NT>(d_table, d_table_AD, lerp);
 #else
-      return internal::tex_value<NT>(h_table, h_table_AD, lerp);
+      return dfrg::interp::tex_value<NT>(h_table, h_table_AD, lerp);
 #endif
     }
 
   private:
     unsigned stack_size, p_size;
     Coordinates coordinates;
-    std::shared_ptr<internal::TexTables<NT>> tables;
+    std::shared_ptr<dfrg::interp::TexTables<NT>> tables;
     const float *d_table = nullptr, *d_table_AD = nullptr;
     float *h_table = nullptr, *h_table_AD = nullptr;
     bool owner;

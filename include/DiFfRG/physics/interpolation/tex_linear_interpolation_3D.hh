@@ -7,9 +7,9 @@
 #include <array>
 #include <vector>
 
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/discretization/grid/coordinates.hh>
-#include <DiFfRG/physics/interpolation/common.hh>
+#include "dfrg/interp.hh"
 
 namespace DiFfRG
 {
@@ -18,7 +18,7 @@ namespace DiFfRG
     static_assert(Coordinates::dim == 3, "TexLinearInterpolator3D requires 3D coordinates");
 
   public:
-    using ReturnType = typename internal::__TLITypes<NT>::ReturnType;
+    using ReturnType = typename dfrg::interp::TexReturn<NT>::type;
 
     TexLinearInterpolator3D(const std::vector<NT> &data, const Coordinates &coordinates)
         : TexLinearInterpolator3D(data.data(), coordinates)
@@ -32,8 +32,8 @@ namespace DiFfRG
     TexLinearInterpolator3D(const Coordinates &coordinates)
         : size(coordinates.size()), shape(coordinates.sizes()), coordinates(coordinates), owner(true)
     {
-      internal::require_axes(shape.data(), 3, "TexLinearInterpolator3D");
-      tables = std::make_shared<internal::TexTables<NT>>(size);
+      dfrg::interp::require_axes(shape.data(), 3, "TexLinearInterpolator3D");
+      tables = std::make_shared<dfrg::interp::TexTables<NT>>(size);
       d_table = tables->device();
       d_table_AD = tables->device_AD();
       h_table = tables->host();
@@ -60,14 +60,14 @@ namespace DiFfRG
     float *data_AD() const { return h_table_AD; }
     ReturnType &operator[](const unsigned i)
     {
-      if constexpr (internal::DualTraits<NT>::is_dual)
+      if constexpr (dfrg::DualTraits<NT>::is_dual)
         throw std::runtime_error("Cannot access dual-number data directly");
       else
         return h_table[i];
     }
     const ReturnType &operator[](const unsigned i) const
     {
-      if constexpr (internal::DualTraits<NT>::is_dual)
+      if constexpr (dfrg::DualTraits<NT>::is_dual)
         throw std::runtime_error("Cannot access dual-number data directly");
       else
         return h_table[i];
@@ -79,11 +79,11 @@ namespace DiFfRG
       const auto idx = coordinates.backward(x, y, z);
       const float ix = float(idx[0]), iy = float(idx[1]), iz = float(idx[2]);
       const unsigned nx = shape[0], ny = shape[1], nz = shape[2];
-      const auto lerp = [=](const float *t) { return internal::lerp3(t, ix, iy, iz, nx, ny, nz); };
+      const auto lerp = [=](const float *t) { return dfrg::interp::lerp3(t, ix, iy, iz, nx, ny, nz); };
 #ifdef __HIP_DEVICE_COMPILE__
-      return internal::tex_value<NT>(d_table, d_table_AD, lerp);
+      return dfrg::interp::tex_value<NT>(d_table, d_table_AD, lerp);
 #else
-      return internal::tex_value<NT>(h_table, h_table_AD, lerp);
+      return dfrg::interp::tex_value<NT>(h_table, h_table_AD, lerp);
 #endif
     }
 
@@ -91,7 +91,7 @@ namespace DiFfRG
     unsigned size;
     std::array<unsigned, 3> shape;
     Coordinates coordinates;
-    std::shared_ptr<internal::TexTables<NT>> tables;
+    std::shared_ptr<dfrg::interp::TexTables<NT>> tables;
     const float *d_table = nullptr, *d_table_AD = nullptr;
     float *h_table = nullptr, *h_table_AD = nullptr;
     bool owner;

@@ -2,8 +2,9 @@
 // of external points (reference physics/utils.hh:31-220).
 //
 // fun.request<NT>(k, point..., args...) is called for every grid point, as in the reference.  The
-// GPU integrators of this library collect those requests and run each batch as one launch
-// (dfrg::tmpl::RequestBatcher), so request_data over n points costs one launch and one
+// GPU integrators of this library collect those requests into one batch per integrator
+// (dfrg::tmpl::RequestBatcher), and request_data launches the batches of its loop as it returns,
+// asynchronously, so host work after it overlaps the GPU: n points cost one launch and one
 // synchronisation instead of n.  get_data, which the reference runs as a TBB parallel loop of
 // get() calls, is built on the same requests: one batched launch, then the results in order.
 #pragma once
@@ -16,6 +17,8 @@
 
 #include <DiFfRG/common/types.hh>
 #include <DiFfRG/discretization/grid/coordinates.hh>
+
+#include "dfrg/integrate.hh"
 
 namespace DiFfRG
 {
@@ -40,6 +43,7 @@ namespace DiFfRG
     for (unsigned i = 0; i < grid.size(); ++i)
       futures->emplace_back(
           internal::call_at(grid[i], [&](const auto &...p) { return fun.template request<NT>(k, p..., args...); }));
+    dfrg::tmpl::flush_thread(); // the loop's requests are one batch per integrator: launch them now
     return futures;
   }
 

@@ -1,7 +1,6 @@
-// DiFfRG/common/hip_prefix.hh -- the C ABI of libdfrg.so and its error translation for C++
-// callers.  check_hip() is the analogue of the reference's check_cuda()
-// (DiFfRG/src/DiFfRG/common/cuda_prefix.cu:9-14): it throws std::runtime_error on a pending
-// device error.
+// dfrg/check.hh -- the C ABI of libdfrg.so and its error translation for the C++ headers.
+// check_hip() is the analogue of the reference's check_cuda() (DiFfRG/src/DiFfRG/common/
+// cuda_prefix.cu:9-14): it throws std::runtime_error on a pending device error.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,10 +10,10 @@
 
 #include "dfrg.h"
 
-namespace DiFfRG
+namespace dfrg
 {
   // DFRG_ERR_INVALID -> std::invalid_argument, every other failure -> std::runtime_error
-  inline void dfrg_check(int rc, const char *what)
+  inline void check(int rc, const char *what)
   {
     if (rc == DFRG_OK) return;
     const std::string msg = std::string(what) + ": " + dfrg_last_error();
@@ -33,6 +32,6 @@ namespace DiFfRG
   {
     const int rc = dfrg_init(0);
     if (rc == DFRG_ERR_NO_DEVICE) throw std::runtime_error("No HIP devices found!");
-    dfrg_check(rc, "dfrg_init");
+    check(rc, "dfrg_init");
   }
-} // namespace DiFfRG
+} // namespace dfrg

@@ -1,192 +1,385 @@
-// dfrg/integrate.hh -- generic gfx950 kernels behind the DiFfRG-compatible integrator templates.
+// dfrg/integrate.hh -- the tile engine behind the drop-in integrator templates.
 //
 // A user KERNEL (static __host__ __device__ kernel()/constant()) is device code of the caller's
-// translation unit, so these kernels are templates compiled there by hipcc.  Each call of get()
-// or request() is one launch on the calling thread's stream: every lane evaluates a strided set
-// of quadrature nodes (element * weight * KERNEL::kernel), lanes are reduced with a wave64 DPP
-// tree, waves in LDS, and blocks through per-block partials whose last-arriving block sums them
-// in block order.  The result (one NT) is copied into pinned host memory on the same stream.
-// Per-call scratch comes from libdfrg.so's caching pools, so no call allocates after warm-up.
+// translation unit, so these kernels are templates compiled there by hipcc.  One integral is the
+// flat node space (radial node, inner nodes...) of its integrator; a tile is a contiguous range of
+// that space.  A block takes one tile: it stages the tile's radial nodes q_j and their factors
+// (integration element x weight) in LDS, its lanes walk the tile's nodes in mixed radix (no division
+// per node) and read the inner nodes' coordinates and weight factors from a per-integrator table,
+// and each node adds KERNEL::kernel(...) x (radial factor x inner factors) to an FP64 accumulator.
+// Blocks reduce with a wave64 DPP tree and LDS; the tiles of one integral meet in per-tile partial
+// slots whose last-arriving block adds them in slot order.  The tile plan, and so the summation
+// order, depends on the integral's node counts only: results are bitwise reproducible, and a
+// request in any batch gives bitwise the value of get().
 //
-// Summation order is fixed by the grid (a function of the node count only), so a result is
-// bitwise reproducible run to run.
+// Accumulation is in FP64 for every number type: FP32 number types (float, complex<float>) evaluate
+// the KERNEL in FP32 on FP32 nodes and accumulate in FP64, the mixed form of the template path.
+//
+// get() launches at once and waits.  request() adds to the integrator's open batch; a batch is
+// launched (asynchronously: tuples to HBM, one kernel, results back into pinned memory, an event)
+// when any future of this thread is read, when request_data() finishes its loop, when the batch is
+// full, or when a request needs another launch configuration.  Reading a future launches every open
+// batch of the calling thread first, so batches of different integrators run back to back before
+// the first wait.
+//
+// Scratch: every host thread owns an arena in HBM on its stream (ticket counters, partial slots,
+// argument tuples, results).  Counters are zeroed when a thread takes an arena (new, grown, or left
+// by a thread that ended) and every launch's finishing blocks reset the counters they used, so
+// consecutive launches on the stream share it without clearing.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
-#include <future>
-#include <stdexcept>
-#include <vector>
-#include <mutex>
-#include <memory>
+#include <exception>
 #include <functional>
+#include <future>
+#include <initializer_list>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
 #include <tuple>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
-#include <DiFfRG/common/complex_math.hh>
-#include <DiFfRG/common/dual_traits.hh>
-#include <DiFfRG/common/hip_prefix.hh>
-#include <DiFfRG/common/math.hh>
-
+#include "dfrg/check.hh"
 #include "dfrg/device.hh"
+#include "dfrg/dual.hh"
+#include "dfrg/math.hh"
 
 namespace dfrg
 {
   namespace tmpl
   {
-    constexpr int kBlock = 256;
-    constexpr int kMaxBlocks = 1024;
-    constexpr int kNodesPerLane = 4;
+    constexpr int kMaxQ = 256;   // radial nodes a tile may span (LDS staging)
+    constexpr long long kMinTile = 1024;
+    constexpr long long kMaxTile = 65536;
 
     // ------------------------------------------------------------------------------------------
-    // reductions over the number type NT: float, double, or a two-component type reduced component
-    // by component -- DiFfRG::complex<float|double> (re, im) or a dual number (value, derivative;
-    // DiFfRG/common/dual_traits.hh)
+    // number types: scalar (float, double) or two-component (complex: real/imag; dual numbers:
+    // value/derivative), accumulated component-wise in FP64
     // ------------------------------------------------------------------------------------------
+    template <class NT> constexpr bool is_scalar_nt = std::is_same_v<NT, double> || std::is_same_v<NT, float>;
+
     template <class NT> struct Parts {
-      static constexpr bool dual = DiFfRG::internal::DualTraits<NT>::is_dual;
+      static constexpr bool dual = DualTraits<NT>::is_dual;
       static __host__ __device__ __forceinline__ auto first(const NT &v)
       {
         if constexpr (dual)
-          return DiFfRG::internal::DualTraits<NT>::value(v);
+          return DualTraits<NT>::value(v);
         else
-          return v.re;
+          return v.real();
       }
       static __host__ __device__ __forceinline__ auto second(const NT &v)
       {
         if constexpr (dual)
-          return DiFfRG::internal::DualTraits<NT>::derivative(v);
+          return DualTraits<NT>::derivative(v);
         else
-          return v.im;
+          return v.imag();
       }
-      template <class P> static __host__ __device__ __forceinline__ NT make(P a, P b)
+      using P = std::remove_cv_t<std::remove_reference_t<decltype(first(std::declval<NT>()))>>;
+      static __host__ __device__ __forceinline__ NT make(double a, double b)
       {
         if constexpr (dual)
-          return DiFfRG::internal::DualTraits<NT>::make(a, b);
+          return DualTraits<NT>::make(P(a), P(b));
         else
-          return NT(a, b);
+          return NT(P(a), P(b));
       }
-      using P = decltype(first(std::declval<NT>()));
     };
-    template <class NT> constexpr bool is_scalar_nt = std::is_same_v<NT, double> || std::is_same_v<NT, float>;
 
-    __device__ __forceinline__ float wave_sum_f32(float v)
-    {
-      v += dev::dpp_f32<0xB1>(v);
-      v += dev::dpp_f32<0x4E>(v);
-      v += dev::dpp_f32<0x141>(v);
-      v += dev::dpp_f32<0x140>(v);
-      v += __shfl_xor(v, 16, 64);
-      v += __shfl_xor(v, 32, 64);
-      return v;
-    }
-    template <class NT> __device__ __forceinline__ NT wave_reduce(NT v)
-    {
-      if constexpr (std::is_same_v<NT, double>)
-        return dev::wave_sum(v);
-      else if constexpr (std::is_same_v<NT, float>)
-        return wave_sum_f32(v);
-      else
-        return Parts<NT>::make(wave_reduce(Parts<NT>::first(v)), wave_reduce(Parts<NT>::second(v)));
-    }
+    template <class NT> struct Acc {
+      static constexpr int N = is_scalar_nt<NT> ? 1 : 2;
+      double v[N];
+      __host__ __device__ __forceinline__ Acc()
+      {
+        for (int i = 0; i < N; ++i) v[i] = 0.0;
+      }
+      __host__ __device__ __forceinline__ void add(const NT &x)
+      {
+        if constexpr (N == 1)
+          v[0] += double(x);
+        else {
+          v[0] += double(Parts<NT>::first(x));
+          v[1] += double(Parts<NT>::second(x));
+        }
+      }
+      static __host__ __device__ __forceinline__ NT value(const double (&s)[N])
+      {
+        if constexpr (N == 1)
+          return NT(s[0]);
+        else
+          return Parts<NT>::make(s[0], s[1]);
+      }
+    };
 
-    template <class NT> __device__ __forceinline__ NT block_reduce(NT v)
+    // ------------------------------------------------------------------------------------------
+    // node sets.  Radial nodes x_j of a Gauss-Legendre rule on [0,1] give q_j = k sqrt(x_j x_ext)
+    // with factor element(q_j) w_j x_ext; angular nodes come from a per-integrator table of
+    // (coordinate, weight factor); Matsubara nodes from the provider's rule.
+    // ------------------------------------------------------------------------------------------
+    template <class ctype> struct AngNode {
+      ctype c, a;
+    };
+
+    template <class ctype> struct Nodes {
+      const ctype *xq = nullptr, *wq = nullptr;    // radial rule on [0,1]
+      const AngNode<ctype> *ang = nullptr;          // angular tables, one per angular digit
+      const ctype *mx = nullptr, *mw = nullptr;     // Matsubara rule (positive nodes)
+      int nx = 0, na = 0, nm = 0;
+      ctype x_extent = 0, m_T = 0;
+    };
+
+    template <class ctype> constexpr ctype two_pi = ctype(2) * ctype(3.14159265358979323846);
+
+    // Each shape: ND digits (radial first, the fastest last, as the reference's flat loops),
+    // element(q, k) of the radial node, and the node term given the staged radial factor rf.
+    // IntegratorGPU<d> (integrator_gpu.hh:34-60)
+    template <int d> struct Radial {
+      static constexpr int ND = 1, NA = 0;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND]) { r[0] = n.nx; }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        return S_d_prec<C>(d) * (powr<d - 2>(q) / 2 * powr<2>(k)) / powr<d>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &, C q, C rf, const int (&)[ND], C k, const T &...t)
+      {
+        return NT(rf * KERNEL::kernel(q, k, t...));
+      }
+    };
+
+    // IntegratorAngleGPU<d> (integrator_angle_gpu.hh:21-40): (q, cos)
+    template <int d> struct Angle1 {
+      static constexpr int ND = 2, NA = 1;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = n.na;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        return S_d_prec<C>(d) * (powr<d - 2>(q) / C(2) * powr<2>(k)) * (C(1) / C(2)) / powr<d>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> a = n.ang[g[1]];
+        return NT((rf * a.a) * KERNEL::kernel(q, a.c, k, t...));
+      }
+    };
+
+    // Integrator4D2AngGPU (integrator_4D_2ang_gpu.hh:20-45): (q, cos1, cos2), the phi integral as 2 pi
+    struct Angle2 {
+      static constexpr int ND = 3, NA = 2;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = r[2] = n.na;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        return (powr<2>(q) * C(0.5) * powr<2>(k)) * two_pi<C> / powr<4>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> a1 = n.ang[g[1]], a2 = n.ang[n.na + g[2]];
+        return NT((rf * (a1.a * a2.a)) * KERNEL::kernel(q, a1.c, a2.c, k, t...));
+      }
+    };
+
+    // Integrator4DGPU (integrator_4D_gpu.hh:20-49): (q, cos1, phi, cos2)
+    struct Angle3 {
+      static constexpr int ND = 4, NA = 3;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = r[2] = r[3] = n.na;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        return (powr<2>(q) * C(0.5) * powr<2>(k)) / powr<4>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> a1 = n.ang[g[1]], ap = n.ang[n.na + g[2]], a2 = n.ang[2 * n.na + g[3]];
+        return NT((rf * (a1.a * ap.a * a2.a)) * KERNEL::kernel(q, a1.c, a2.c, ap.c, k, t...));
+      }
+    };
+
+    // Matsubara sum T sum_n f(q0_n) = T f(0) + sum_i w_i (f(x_i) + f(-x_i)) (matsubara.hh:102-108)
+    // IntegratorFiniteTq0GPU<d> (integrator_finiteTq0_gpu.hh:20-44): (q, n)
+    template <int d> struct FiniteT {
+      static constexpr int ND = 2, NA = 0;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = n.nm;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        return S_d_prec<C>(d - 1) * (powr<d - 3>(q) / C(2) * powr<2>(k)) / powr<d - 1>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const C q0 = n.mx[g[1]];
+        NT r = NT((rf * n.mw[g[1]]) * (KERNEL::kernel(q, q0, k, t...) + KERNEL::kernel(q, -q0, k, t...)));
+        if (n.m_T > 0 && g[1] == 0) r += NT((rf * n.m_T) * KERNEL::kernel(q, C(0), k, t...));
+        return r;
+      }
+    };
+
+    // IntegratorAngleFiniteTq0GPU<d> (integrator_angle_finiteTq0_gpu.hh:20-50): (q, cos, n)
+    template <int d> struct Angle1FiniteT {
+      static constexpr int ND = 3, NA = 1;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = n.na;
+        r[2] = n.nm;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        return S_d_prec<C>(d - 1) * (powr<d - 3>(q) / C(2) * powr<2>(k)) * (C(1) / C(2)) / powr<d - 1>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> a = n.ang[g[1]];
+        const C q0 = n.mx[g[2]], ra = rf * a.a;
+        NT r = NT((ra * n.mw[g[2]]) * (KERNEL::kernel(q, a.c, q0, k, t...) + KERNEL::kernel(q, a.c, -q0, k, t...)));
+        if (n.m_T > 0 && g[2] == 0) r += NT((ra * n.m_T) * KERNEL::kernel(q, a.c, C(0), k, t...));
+        return r;
+      }
+    };
+
+    // Integrator4DFiniteTq0GPU (integrator_4D_finiteTq0_gpu.hh:20-55): (q, cos, phi, n)
+    struct Angle2FiniteT {
+      static constexpr int ND = 4, NA = 2;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = r[2] = n.na;
+        r[3] = n.nm;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        return (q / C(2) * powr<2>(k)) / powr<3>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> a = n.ang[g[1]], ap = n.ang[n.na + g[2]];
+        const C q0 = n.mx[g[3]], ra = rf * (a.a * ap.a);
+        NT r = NT((ra * n.mw[g[3]]) *
+                  (KERNEL::kernel(q, a.c, ap.c, q0, k, t...) + KERNEL::kernel(q, a.c, ap.c, -q0, k, t...)));
+        if (n.m_T > 0 && g[3] == 0) r += NT((ra * n.m_T) * KERNEL::kernel(q, a.c, ap.c, C(0), k, t...));
+        return r;
+      }
+    };
+
+    // Angular tables of a shape from the Gauss-Legendre rule on [0,1] of order na (host nodes):
+    //   cos = 2(x - 1/2), weight 2w; cos1 of the 4D forms also carries the Jacobian sqrt(1 - cos1^2);
+    //   phi = 2 pi x, weight 2 pi w.
+    enum class AngKind { cos, cos_jacobian, phi };
+    template <class ctype>
+    std::vector<AngNode<ctype>> angle_table(const std::vector<ctype> &x, const std::vector<ctype> &w, AngKind kind)
     {
-      // raw storage: NT may have a user-provided constructor (complex)
-      __shared__ __attribute__((aligned(16))) unsigned char smem_raw[sizeof(NT) * (kBlock / 64)];
-      NT *smem = reinterpret_cast<NT *>(smem_raw);
-      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-      v = wave_reduce(v);
-      if (lane == 0) smem[wid] = v;
-      __syncthreads();
-      NT t = smem[0];
-      for (int i = 1; i < kBlock / 64; ++i) t += smem[i];
+      std::vector<AngNode<ctype>> t(x.size());
+      for (size_t i = 0; i < x.size(); ++i) {
+        if (kind == AngKind::phi) {
+          t[i] = {two_pi<ctype> * x[i], two_pi<ctype> * w[i]};
+        } else {
+          const ctype c = 2 * (x[i] - ctype(0.5));
+          t[i] = {c, kind == AngKind::cos_jacobian ? 2 * w[i] * std::sqrt(1 - c * c) : 2 * w[i]};
+        }
+      }
       return t;
     }
 
-    // Partials of type NT through device-scope atomics, component-wise (dfrg::dev::publish).  A
-    // two-component partial is stored as its two components at the start of its NT slot.
-    template <class NT> __device__ __forceinline__ void publish_nt(NT *p, const NT &v)
+    // The tables of several angular digits, one after the other (na entries each).
+    template <class ctype>
+    std::vector<AngNode<ctype>> angle_tables(const std::vector<ctype> &x, const std::vector<ctype> &w,
+                                             std::initializer_list<AngKind> kinds)
     {
-      if constexpr (is_scalar_nt<NT>)
-        dev::publish(p, v);
-      else {
-        using P = typename Parts<NT>::P;
-        static_assert(sizeof(NT) >= 2 * sizeof(P), "two-component number type smaller than its components");
-        P *raw = reinterpret_cast<P *>(p);
-        dev::publish(raw, Parts<NT>::first(v));
-        dev::publish(raw + 1, Parts<NT>::second(v));
+      std::vector<AngNode<ctype>> t;
+      for (AngKind k : kinds) {
+        const auto one = angle_table(x, w, k);
+        t.insert(t.end(), one.begin(), one.end());
       }
-    }
-    template <class NT> __device__ __forceinline__ NT fetch_nt(const NT *p)
-    {
-      if constexpr (is_scalar_nt<NT>)
-        return dev::fetch(p);
-      else {
-        using P = typename Parts<NT>::P;
-        const P *raw = reinterpret_cast<const P *>(p);
-        return Parts<NT>::make(dev::fetch(raw), dev::fetch(raw + 1));
-      }
+      return t;
     }
 
-    // partial slots of a single call: one 128-byte line each (dfrg::dev::kSlotStride)
-    template <class NT> constexpr int slot_stride = sizeof(NT) >= 128 ? 1 : int(128 / sizeof(NT));
-
-    template <class NT> __device__ __forceinline__ NT lane_nt(const NT &v, int i)
+    // A device copy of the angular tables, shared by the copies of an integrator.
+    template <class ctype> std::shared_ptr<AngNode<ctype>> upload_angles(const std::vector<AngNode<ctype>> &t)
     {
-      if constexpr (is_scalar_nt<NT>)
-        return dev::lane_value(v, i);
-      else
-        return Parts<NT>::make(dev::lane_value(Parts<NT>::first(v), i), dev::lane_value(Parts<NT>::second(v), i));
-    }
-
-    // Publish this block's partial in slot `slot` (at partials[slot * stride]) of a group of n_slots;
-    // the last block of the group adds the partials in slot order and writes *result (then resets
-    // the counter for the next use of the scratch block).  The first wave of that block fetches the partials one per lane, 64 at
-    // a time, and adds them lane by lane in slot order.
-    template <class NT>
-    __device__ __forceinline__ void finish_slots(NT part, NT *partials, int slot, int n_slots, unsigned *counter,
-                                                 NT *result, int stride = 1)
-    {
-      __shared__ bool last;
-      if (threadIdx.x == 0) {
-        publish_nt(&partials[(long long)slot * stride], part);
-        dev::wait_stores();
-        last = dev::take_ticket(counter) == (unsigned)(n_slots - 1);
+      void *p = nullptr;
+      check(dfrg_malloc(&p, sizeof(AngNode<ctype>) * std::max<size_t>(t.size(), 1)), "angular node table");
+      std::shared_ptr<AngNode<ctype>> d((AngNode<ctype> *)p, [](AngNode<ctype> *q) { dfrg_free(q); });
+      if (!t.empty()) {
+        check(dfrg_memcpy_h2d(p, t.data(), sizeof(AngNode<ctype>) * t.size(), nullptr), "angular node table");
+        check(dfrg_stream_sync(nullptr), "angular node table");
       }
-      __syncthreads();
-      if (last && threadIdx.x < 64) {
-        NT s = NT(0);
-        for (int c = 0; c < n_slots; c += 64) {
-          const int j = c + (int)threadIdx.x;
-          const NT v = j < n_slots ? fetch_nt(&partials[(long long)j * stride]) : NT(0);
-          const int m = n_slots - c < 64 ? n_slots - c : 64;
-          if (c == 0) {
-            s = lane_nt(v, 0);
-            for (int l = 1; l < m; ++l) s += lane_nt(v, l);
-          } else
-            for (int l = 0; l < m; ++l) s += lane_nt(v, l);
-        }
-        if (threadIdx.x == 0) {
-          *result = s;
-          dev::reset_counter(counter);
-        }
-      }
+      return d;
     }
 
     // ------------------------------------------------------------------------------------------
-    // per-thread node sums.  Block b of nb takes nodes b*kBlock + tid, stepping nb*kBlock; a
-    // single call launches nb blocks, a batch launches nb blocks per request with the same
-    // partition, so a batched result is bitwise the single call's.
+    // the tile plan: tile_evals nodes per tile, tpr tiles per integral, block threads per tile
     // ------------------------------------------------------------------------------------------
-    // IntegratorGPU<d> (integrator_gpu.hh:34-60)
-    // The digits of a flat node index (slowest first) in mixed radix r, walked with a fixed
-    // stride S: a thread's nodes i, i + S, i + 2S, ... of the flat loop, with the stride's digits
-    // added with carries instead of a division per node.  The slowest digit is clamped to its
-    // radix at the start, so a thread whose first node is past the end does not iterate.
+    struct Plan {
+      long long per_req = 0; // nodes of one integral
+      int ni = 1;            // inner nodes per radial node
+      int tile = 0;          // nodes per tile
+      int tpr = 1;           // tiles per integral
+      int block = 256;
+    };
+
+    inline int device_cus()
+    {
+      static const int cus = [] {
+        int c = 0;
+        if (dfrg_device_props(nullptr, 0, &c, nullptr, nullptr) != DFRG_OK || c <= 0) c = 256;
+        return c;
+      }();
+      return cus;
+    }
+
+    // The plan depends on the integral's node count only, so a request in any batch and get() sum
+    // the same tiles in the same order: about 32 tiles per integral (a batch of a few dozen
+    // integrals fills the chip with ~8 blocks per CU), 1024..65536 nodes per tile, at most kMaxQ
+    // radial nodes per tile and never more nodes than the integral has.
+    inline Plan make_plan(long long per_req, int ni)
+    {
+      Plan p;
+      p.per_req = per_req;
+      p.ni = ni;
+      long long t = (per_req + 31) / 32;
+      t = std::max(kMinTile, std::min(kMaxTile, t));
+      t = std::min(t, per_req);
+      t = std::min(t, (long long)(kMaxQ - 2) * ni);
+      t = std::max(t, 1LL);
+      p.tile = (int)t;
+      p.tpr = (int)((per_req + t - 1) / t);
+      p.block = t >= 1024 ? 256 : 64;
+      return p;
+    }
+
+    template <class S, class C> Plan plan_of(const Nodes<C> &n)
+    {
+      int rad[S::ND];
+      S::radix(n, rad);
+      long long per_req = 1;
+      for (int i = 0; i < S::ND; ++i) per_req *= rad[i];
+      return make_plan(per_req, (int)(per_req / rad[0]));
+    }
+
+    // The mixed-radix digits of a flat node index (slowest first), advanced by a fixed stride S with
+    // carries instead of a division per node.
     template <int N> struct NodeWalk {
       int dig[N], step[N], rad[N];
       __device__ __forceinline__ NodeWalk(long long i, long long S, const int (&r)[N])
@@ -200,10 +393,9 @@ namespace dfrg
           S /= r[j];
         }
         rad[0] = r[0];
-        dig[0] = (int)(i < r[0] ? i : r[0]);
+        dig[0] = (int)i;
         step[0] = (int)S;
       }
-      __device__ __forceinline__ bool valid() const { return dig[0] < rad[0]; }
       __device__ __forceinline__ void next()
       {
         int carry = 0;
@@ -217,530 +409,408 @@ namespace dfrg
       }
     };
 
-    template <int d, class ctype, class NT, class KERNEL, class... T>
-    __device__ __forceinline__ NT radial_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq, int nx,
-                                             ctype x_extent, int b, int nb, ctype k, const T &...t)
+    // ------------------------------------------------------------------------------------------
+    // one tile: request r, tile c of its tpr
+    // ------------------------------------------------------------------------------------------
+    template <class S, int B, class NT, class KERNEL, class C, class Args>
+    __device__ __forceinline__ void tile_body(const Nodes<C> &n, const Plan &p, int r, int c, const Args &args,
+                                              double *partials, unsigned *counters, NT *results)
     {
-      NT acc = NT(0);
-      constexpr ctype S_d = DiFfRG::S_d_prec<ctype>(d);
-      for (int i = b * kBlock + threadIdx.x; i < nx; i += nb * kBlock) {
-        const ctype weight = wq[i] * x_extent;
-        const ctype q = k * sqrt(xq[i] * x_extent);
-        const ctype int_element = S_d * (DiFfRG::powr<d - 2>(q) / 2 * DiFfRG::powr<2>(k)) /
-                                  DiFfRG::powr<d>(2 * (ctype)M_PI);
-        const NT res = KERNEL::kernel(q, k, t...);
-        acc += int_element * res * weight;
+      constexpr int NC = Acc<NT>::N;
+      __shared__ C s_q[kMaxQ], s_rf[kMaxQ];
+      __shared__ double s_red[NC][B / 64];
+      __shared__ bool s_last;
+      const long long f0 = (long long)c * p.tile;
+      const long long f1 = f0 + p.tile < p.per_req ? f0 + p.tile : p.per_req;
+      const int ix0 = (int)(f0 / p.ni), ix1 = (int)((f1 - 1) / p.ni);
+      const C k = std::get<0>(args);
+      for (int j = threadIdx.x; j <= ix1 - ix0; j += B) {
+        const int ix = ix0 + j;
+        const C q = k * sqrt(n.xq[ix] * n.x_extent);
+        s_q[j] = q;
+        s_rf[j] = S::element(q, k) * (n.wq[ix] * n.x_extent);
       }
-      return acc;
+      __syncthreads();
+
+      Acc<NT> acc;
+      int rad[S::ND];
+      S::radix(n, rad);
+      NodeWalk<S::ND> w(f0 + threadIdx.x, B, rad);
+      for (long long f = f0 + threadIdx.x; f < f1; f += B, w.next()) {
+        const int j = w.dig[0] - ix0;
+        acc.add(std::apply(
+            [&](const C &kk, const auto &...t) {
+              return S::template term<NT, KERNEL>(n, s_q[j], s_rf[j], w.dig, kk, t...);
+            },
+            args));
+      }
+
+      // block: wave sums, then wave 0's lanes in wave order
+      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        const double v = dev::wave_sum(acc.v[i]);
+        if (lane == 0) s_red[i][wid] = v;
+      }
+      __syncthreads();
+      double tot[NC];
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        tot[i] = s_red[i][0];
+        for (int j = 1; j < B / 64; ++j) tot[i] += s_red[i][j];
+      }
+      if (p.tpr == 1) {
+        if (threadIdx.x == 0) results[r] = Acc<NT>::value(tot);
+        return;
+      }
+      // tiles of the integral: publish, ticket, the last block adds the partials in slot order and
+      // resets the counter
+      double *slots = partials + (long long)r * p.tpr * dev::kSlotStride;
+      unsigned *counter = counters + (long long)r * dev::kCounterStride;
+      if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) dev::publish(&slots[(long long)c * dev::kSlotStride + i], tot[i]);
+        dev::wait_stores();
+        s_last = dev::take_ticket(counter) == (unsigned)(p.tpr - 1);
+      }
+      __syncthreads();
+      if (s_last && threadIdx.x < 64) {
+        double s[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) s[i] = 0.0;
+        for (int c0 = 0; c0 < p.tpr; c0 += 64) {
+          const int jj = c0 + (int)threadIdx.x;
+          const int m = p.tpr - c0 < 64 ? p.tpr - c0 : 64;
+#pragma unroll
+          for (int i = 0; i < NC; ++i) {
+            const double v = jj < p.tpr ? dev::fetch(&slots[(long long)jj * dev::kSlotStride + i]) : 0.0;
+            for (int l = 0; l < m; ++l) s[i] += dev::lane_value(v, l);
+          }
+        }
+        if (threadIdx.x == 0) {
+          results[r] = Acc<NT>::value(s);
+          dev::reset_counter(counter);
+        }
+      }
     }
 
-    // Integrator4D2AngGPU (integrator_4D_2ang_gpu.hh:20-45): node (q, cos1, cos2), cos2 fastest
-    template <class ctype, class NT, class KERNEL, class... T>
-    __device__ __forceinline__ NT angle2_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                             const ctype *__restrict__ xa, const ctype *__restrict__ wa, int nx,
-                                             int na, ctype x_extent, int b, int nb, ctype k, const T &...t)
+    // one integral, its (k, t...) passed by value
+    template <class S, int B, class NT, class KERNEL, class C, class Args>
+    __global__ __launch_bounds__(B) void get_kernel(Nodes<C> n, Plan p, Args args, double *partials,
+                                                    unsigned *counters, NT *results)
     {
-      NT acc = NT(0);
-      for (NodeWalk<3> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na, na}); w.valid();
-           w.next()) {
-        const int ix = w.dig[0], i1 = w.dig[1], i2 = w.dig[2];
-        const ctype q = k * sqrt(xq[ix] * x_extent);
-        const ctype cos1 = 2 * (xa[i1] - (ctype)0.5);
-        const ctype int_element = (DiFfRG::powr<2>(q) * (ctype)0.5 * DiFfRG::powr<2>(k)) // x = p^2 / k^2 integral
-                                  * sqrt(1. - DiFfRG::powr<2>(cos1))                      // cos1 Jacobian
-                                  * 2 * (ctype)M_PI                                       // phi integral
-                                  / DiFfRG::powr<4>(2 * (ctype)M_PI);                     // Fourier factor
-        const ctype weight = 2 * wa[i1] * wq[ix] * x_extent * 2 * wa[i2];
-        const ctype cos2 = 2 * (xa[i2] - (ctype)0.5);
-        acc += KERNEL::kernel(q, cos1, cos2, k, t...) * int_element * weight;
-      }
-      return acc;
+      tile_body<S, B, NT, KERNEL>(n, p, 0, (int)blockIdx.x, args, partials, counters, results);
     }
 
-    // Integrator4DGPU (integrator_4D_gpu.hh:20-49): node (q, cos1, phi, cos2), cos2 fastest
-    template <class ctype, class NT, class KERNEL, class... T>
-    __device__ __forceinline__ NT angle3_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                             const ctype *__restrict__ xa, const ctype *__restrict__ wa, int nx,
-                                             int na, ctype x_extent, int b, int nb, ctype k, const T &...t)
+    // a batch of integrals whose (k, t...) tuples sit in HBM, tpr consecutive blocks per integral
+    template <class S, int B, class NT, class KERNEL, class C, class Args>
+    __global__ __launch_bounds__(B) void batch_kernel(Nodes<C> n, Plan p, const Args *args, double *partials,
+                                                      unsigned *counters, NT *results)
     {
-      NT acc = NT(0);
-      for (NodeWalk<4> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na, na, na}); w.valid();
-           w.next()) {
-        const int ix = w.dig[0], i1 = w.dig[1], i3 = w.dig[2], i2 = w.dig[3]; // q, cos1, phi, cos2
-        const ctype q = k * sqrt(xq[ix] * x_extent);
-        const ctype cos1 = 2 * (xa[i1] - (ctype)0.5);
-        const ctype phi = 2 * (ctype)M_PI * xa[i3];
-        const ctype int_element = (DiFfRG::powr<2>(q) * (ctype)0.5 * DiFfRG::powr<2>(k)) *
-                                  sqrt(1. - DiFfRG::powr<2>(cos1)) / DiFfRG::powr<4>(2 * (ctype)M_PI);
-        const ctype weight = 2 * (ctype)M_PI * wa[i3] * 2 * wa[i1] * wq[ix] * x_extent;
-        const ctype cos2 = 2 * (xa[i2] - (ctype)0.5);
-        acc += KERNEL::kernel(q, cos1, cos2, phi, k, t...) * int_element * weight * 2 * wa[i2];
-      }
-      return acc;
-    }
-
-    // IntegratorFiniteTq0GPU<d> (integrator_finiteTq0_gpu.hh:20-44): node (q, Matsubara n)
-    template <int d, class ctype, class NT, class KERNEL, class... T>
-    __device__ __forceinline__ NT finiteTq0_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx,
-                                                int nm, ctype x_extent, ctype m_T, int b, int nb, ctype k,
-                                                const T &...t)
-    {
-      NT acc = NT(0);
-      constexpr ctype S_dm1 = DiFfRG::S_d_prec<ctype>(d - 1);
-      for (NodeWalk<2> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, nm}); w.valid(); w.next()) {
-        const int ix = w.dig[0], iy = w.dig[1];
-        const ctype q = k * sqrt(xq[ix] * x_extent);
-        const ctype x_weight = wq[ix] * x_extent;
-        const ctype int_element = S_dm1 * (DiFfRG::powr<d - 3>(q) / (ctype)2 * DiFfRG::powr<2>(k)) /
-                                  DiFfRG::powr<d - 1>(2 * (ctype)M_PI);
-        const ctype q0 = mx[iy];
-        const ctype weight = x_weight * mw[iy];
-        NT res = int_element * weight * (KERNEL::kernel(q, q0, k, t...) + KERNEL::kernel(q, -q0, k, t...));
-        if (m_T > 0 && iy == 0) res += int_element * x_weight * m_T * KERNEL::kernel(q, (ctype)0, k, t...);
-        acc += res;
-      }
-      return acc;
-    }
-
-    // IntegratorAngleGPU<d> (integrator_angle_gpu.hh:21-40): node (q, cos), cos fastest
-    template <int d, class ctype, class NT, class KERNEL, class... T>
-    __device__ __forceinline__ NT angle1_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                             const ctype *__restrict__ xa, const ctype *__restrict__ wa, int nx,
-                                             int na, ctype x_extent, int b, int nb, ctype k, const T &...t)
-    {
-      NT acc = NT(0);
-      constexpr ctype S_d = DiFfRG::S_d_prec<ctype>(d);
-      for (NodeWalk<2> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na}); w.valid(); w.next()) {
-        const int ix = w.dig[0], iy = w.dig[1];
-        const ctype q = k * sqrt(xq[ix] * x_extent);
-        const ctype cos = 2 * (xa[iy] - (ctype)0.5);
-        const ctype int_element = S_d                                                // solid angle
-                                  * (DiFfRG::powr<d - 2>(q) / (ctype)2 * DiFfRG::powr<2>(k)) // x = q^2/k^2
-                                  * (1 / (ctype)2)                                   // cos integral divided out
-                                  / DiFfRG::powr<d>(2 * (ctype)M_PI);                // Fourier factor
-        const ctype weight = 2 * wa[iy] * wq[ix] * x_extent;
-        acc += int_element * weight * KERNEL::kernel(q, cos, k, t...);
-      }
-      return acc;
-    }
-
-    // Integrator4DFiniteTq0GPU (integrator_4D_finiteTq0_gpu.hh:20-55): node (q, cos, phi, Matsubara n)
-    template <class ctype, class NT, class KERNEL, class... T>
-    __device__ __forceinline__ NT angle2_finiteT_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                     const ctype *__restrict__ xa, const ctype *__restrict__ wa,
-                                                     const ctype *__restrict__ mx, const ctype *__restrict__ mw,
-                                                     int nx, int na, int nm, ctype x_extent, ctype m_T, int b, int nb,
-                                                     ctype k, const T &...t)
-    {
-      NT acc = NT(0);
-      for (NodeWalk<4> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na, na, nm}); w.valid();
-           w.next()) {
-        const int ix = w.dig[0], iy = w.dig[1], iz = w.dig[2], in = w.dig[3]; // q, cos, phi, n
-        const ctype q = k * sqrt(xq[ix] * x_extent);
-        const ctype int_element = (DiFfRG::powr<1>(q) / (ctype)2 * DiFfRG::powr<2>(k)) // x = q^2/k^2, 3 dims
-                                  / DiFfRG::powr<3>(2 * (ctype)M_PI);                   // Fourier factor
-        const ctype cos = 2 * (xa[iy] - (ctype)0.5);
-        const ctype phi = 2 * (ctype)M_PI * xa[iz];
-        const ctype x_weight = 2 * (ctype)M_PI * wa[iz] * 2 * wa[iy] * wq[ix] * x_extent;
-        const ctype q0 = mx[in];
-        const ctype weight = x_weight * mw[in];
-        NT res = int_element * weight *
-                 (KERNEL::kernel(q, cos, phi, q0, k, t...) + KERNEL::kernel(q, cos, phi, -q0, k, t...));
-        if (m_T > 0 && in == 0) res += int_element * x_weight * m_T * KERNEL::kernel(q, cos, phi, (ctype)0, k, t...);
-        acc += res;
-      }
-      return acc;
-    }
-
-    // IntegratorAngleFiniteTq0GPU<d> (integrator_angle_finiteTq0_gpu.hh:20-50): node (q, cos, Matsubara n)
-    template <int d, class ctype, class NT, class KERNEL, class... T>
-    __device__ __forceinline__ NT angle1_finiteT_sum(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                     const ctype *__restrict__ xa, const ctype *__restrict__ wa,
-                                                     const ctype *__restrict__ mx, const ctype *__restrict__ mw,
-                                                     int nx, int na, int nm, ctype x_extent, ctype m_T, int b, int nb,
-                                                     ctype k, const T &...t)
-    {
-      NT acc = NT(0);
-      constexpr ctype S_dm1 = DiFfRG::S_d_prec<ctype>(d - 1);
-      for (NodeWalk<3> w((long long)b * kBlock + threadIdx.x, (long long)nb * kBlock, {nx, na, nm}); w.valid();
-           w.next()) {
-        const int ix = w.dig[0], iy = w.dig[1], in = w.dig[2];
-        const ctype q = k * sqrt(xq[ix] * x_extent);
-        const ctype int_element = S_dm1                                                // solid angle
-                                  * (DiFfRG::powr<d - 3>(q) / (ctype)2 * DiFfRG::powr<2>(k)) // x = q^2/k^2
-                                  * (1 / (ctype)2)                                     // cos integral divided out
-                                  / DiFfRG::powr<d - 1>(2 * (ctype)M_PI);              // Fourier factor
-        const ctype cos = 2 * (xa[iy] - (ctype)0.5);
-        const ctype x_weight = 2 * wa[iy] * wq[ix] * x_extent;
-        const ctype q0 = mx[in];
-        const ctype weight = x_weight * mw[in];
-        NT res = int_element * weight * (KERNEL::kernel(q, cos, q0, k, t...) + KERNEL::kernel(q, cos, -q0, k, t...));
-        if (m_T > 0 && in == 0) res += int_element * x_weight * m_T * KERNEL::kernel(q, cos, (ctype)0, k, t...);
-        acc += res;
-      }
-      return acc;
+      const int r = (int)(blockIdx.x / (unsigned)p.tpr), c = (int)(blockIdx.x - (unsigned)r * p.tpr);
+      tile_body<S, B, NT, KERNEL>(n, p, r, c, args[r], partials, counters, results);
     }
 
     // ------------------------------------------------------------------------------------------
-    // kernels: one call (arguments by value), or a batch of requests whose (k, t...) tuples sit in
-    // device memory, nb blocks per request
+    // host: the thread's scratch arena
     // ------------------------------------------------------------------------------------------
-    template <int d, class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void radial_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                            int nx, ctype x_extent, ctype k, NT *partials,
-                                                            unsigned *counter, NT *result, T... t)
+    class Arena
     {
-      const NT acc = radial_sum<d, ctype, NT, KERNEL>(xq, wq, nx, x_extent, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
-    }
-    template <int d, class ctype, class NT, class KERNEL, class Args>
-    __global__ __launch_bounds__(kBlock) void radial_batch_kernel(const ctype *__restrict__ xq,
-                                                                  const ctype *__restrict__ wq, int nx, ctype x_extent,
-                                                                  const Args *args, int nb, NT *partials,
-                                                                  unsigned *counters, NT *results)
-    {
-      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
-      const NT acc = std::apply(
-          [&](const ctype &k, const auto &...t) {
-            return radial_sum<d, ctype, NT, KERNEL>(xq, wq, nx, x_extent, b, nb, k, t...);
-          },
-          args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
-                   counters + (long long)r * dev::kCounterStride, results + r);
-    }
-
-    template <class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void angle2_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
-                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
-                                                            unsigned *counter, NT *result, T... t)
-    {
-      const NT acc = angle2_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
-    }
-    template <class ctype, class NT, class KERNEL, class Args>
-    __global__ __launch_bounds__(kBlock) void angle2_batch_kernel(const ctype *__restrict__ xq,
-                                                                  const ctype *__restrict__ wq,
-                                                                  const ctype *__restrict__ xa,
-                                                                  const ctype *__restrict__ wa, int nx, int na,
-                                                                  ctype x_extent, const Args *args, int nb,
-                                                                  NT *partials, unsigned *counters, NT *results)
-    {
-      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
-      const NT acc = std::apply(
-          [&](const ctype &k, const auto &...t) {
-            return angle2_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
-          },
-          args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
-                   counters + (long long)r * dev::kCounterStride, results + r);
-    }
-
-    template <class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void angle3_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
-                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
-                                                            unsigned *counter, NT *result, T... t)
-    {
-      const NT acc = angle3_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
-    }
-    template <class ctype, class NT, class KERNEL, class Args>
-    __global__ __launch_bounds__(kBlock) void angle3_batch_kernel(const ctype *__restrict__ xq,
-                                                                  const ctype *__restrict__ wq,
-                                                                  const ctype *__restrict__ xa,
-                                                                  const ctype *__restrict__ wa, int nx, int na,
-                                                                  ctype x_extent, const Args *args, int nb,
-                                                                  NT *partials, unsigned *counters, NT *results)
-    {
-      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
-      const NT acc = std::apply(
-          [&](const ctype &k, const auto &...t) {
-            return angle3_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
-          },
-          args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
-                   counters + (long long)r * dev::kCounterStride, results + r);
-    }
-
-    template <int d, class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void finiteTq0_kernel(const ctype *__restrict__ xq,
-                                                               const ctype *__restrict__ wq,
-                                                               const ctype *__restrict__ mx,
-                                                               const ctype *__restrict__ mw, int nx, int nm,
-                                                               ctype x_extent, ctype m_T, ctype k, NT *partials,
-                                                               unsigned *counter, NT *result, T... t)
-    {
-      const NT acc =
-          finiteTq0_sum<d, ctype, NT, KERNEL>(xq, wq, mx, mw, nx, nm, x_extent, m_T, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
-    }
-    template <int d, class ctype, class NT, class KERNEL, class Args>
-    __global__ __launch_bounds__(kBlock) void finiteTq0_batch_kernel(const ctype *__restrict__ xq,
-                                                                     const ctype *__restrict__ wq,
-                                                                     const ctype *__restrict__ mx,
-                                                                     const ctype *__restrict__ mw, int nx, int nm,
-                                                                     ctype x_extent, ctype m_T, const Args *args,
-                                                                     int nb, NT *partials, unsigned *counters,
-                                                                     NT *results)
-    {
-      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
-      const NT acc = std::apply(
-          [&](const ctype &k, const auto &...t) {
-            return finiteTq0_sum<d, ctype, NT, KERNEL>(xq, wq, mx, mw, nx, nm, x_extent, m_T, b, nb, k, t...);
-          },
-          args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
-                   counters + (long long)r * dev::kCounterStride, results + r);
-    }
-
-    template <int d, class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void angle1_kernel(const ctype *__restrict__ xq, const ctype *__restrict__ wq,
-                                                            const ctype *__restrict__ xa, const ctype *__restrict__ wa,
-                                                            int nx, int na, ctype x_extent, ctype k, NT *partials,
-                                                            unsigned *counter, NT *result, T... t)
-    {
-      const NT acc =
-          angle1_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
-    }
-    template <int d, class ctype, class NT, class KERNEL, class Args>
-    __global__ __launch_bounds__(kBlock) void angle1_batch_kernel(const ctype *__restrict__ xq,
-                                                                  const ctype *__restrict__ wq,
-                                                                  const ctype *__restrict__ xa,
-                                                                  const ctype *__restrict__ wa, int nx, int na,
-                                                                  ctype x_extent, const Args *args, int nb,
-                                                                  NT *partials, unsigned *counters, NT *results)
-    {
-      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
-      const NT acc = std::apply(
-          [&](const ctype &k, const auto &...t) {
-            return angle1_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, nx, na, x_extent, b, nb, k, t...);
-          },
-          args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
-                   counters + (long long)r * dev::kCounterStride, results + r);
-    }
-
-    template <class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void angle2_finiteT_kernel(
-        const ctype *__restrict__ xq, const ctype *__restrict__ wq, const ctype *__restrict__ xa,
-        const ctype *__restrict__ wa, const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx, int na,
-        int nm, ctype x_extent, ctype m_T, ctype k, NT *partials, unsigned *counter, NT *result, T... t)
-    {
-      const NT acc = angle2_finiteT_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T,
-                                                           blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
-    }
-    template <class ctype, class NT, class KERNEL, class Args>
-    __global__ __launch_bounds__(kBlock) void angle2_finiteT_batch_kernel(
-        const ctype *__restrict__ xq, const ctype *__restrict__ wq, const ctype *__restrict__ xa,
-        const ctype *__restrict__ wa, const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx, int na,
-        int nm, ctype x_extent, ctype m_T, const Args *args, int nb, NT *partials, unsigned *counters, NT *results)
-    {
-      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
-      const NT acc = std::apply(
-          [&](const ctype &k, const auto &...t) {
-            return angle2_finiteT_sum<ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T, b, nb, k,
-                                                         t...);
-          },
-          args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
-                   counters + (long long)r * dev::kCounterStride, results + r);
-    }
-
-    template <int d, class ctype, class NT, class KERNEL, class... T>
-    __global__ __launch_bounds__(kBlock) void angle1_finiteT_kernel(
-        const ctype *__restrict__ xq, const ctype *__restrict__ wq, const ctype *__restrict__ xa,
-        const ctype *__restrict__ wa, const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx, int na,
-        int nm, ctype x_extent, ctype m_T, ctype k, NT *partials, unsigned *counter, NT *result, T... t)
-    {
-      const NT acc = angle1_finiteT_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T,
-                                                              blockIdx.x, gridDim.x, k, t...);
-      finish_slots(block_reduce(acc), partials, blockIdx.x, gridDim.x, counter, result, slot_stride<NT>);
-    }
-    template <int d, class ctype, class NT, class KERNEL, class Args>
-    __global__ __launch_bounds__(kBlock) void angle1_finiteT_batch_kernel(
-        const ctype *__restrict__ xq, const ctype *__restrict__ wq, const ctype *__restrict__ xa,
-        const ctype *__restrict__ wa, const ctype *__restrict__ mx, const ctype *__restrict__ mw, int nx, int na,
-        int nm, ctype x_extent, ctype m_T, const Args *args, int nb, NT *partials, unsigned *counters, NT *results)
-    {
-      const int r = blockIdx.x / nb, b = blockIdx.x - r * nb;
-      const NT acc = std::apply(
-          [&](const ctype &k, const auto &...t) {
-            return angle1_finiteT_sum<d, ctype, NT, KERNEL>(xq, wq, xa, wa, mx, mw, nx, na, nm, x_extent, m_T, b, nb,
-                                                            k, t...);
-          },
-          args[r]);
-      finish_slots(block_reduce(acc), partials + (long long)r * nb, b, nb,
-                   counters + (long long)r * dev::kCounterStride, results + r);
-    }
-
-    // ------------------------------------------------------------------------------------------
-    // host side: per-call scratch, launch and result
-    // ------------------------------------------------------------------------------------------
-    inline int grid_for(long long nodes)
-    {
-      const long long per_block = (long long)kBlock * kNodesPerLane;
-      long long g = (nodes + per_block - 1) / per_block;
-      if (g < 1) g = 1;
-      if (g > kMaxBlocks) g = kMaxBlocks;
-      return (int)g;
-    }
-
-    // One in-flight evaluation: scratch (partials | counter | result) in HBM and a pinned result
-    // slot, enqueued on `stream`.
-    template <class NT> struct Pending {
-      hipStream_t stream = nullptr;
-      void *d_scratch = nullptr;
-      void *h_result = nullptr;
-      NT constant = NT(0);
-
-      static constexpr size_t counter_offset = ((kMaxBlocks * slot_stride<NT> * sizeof(NT) + 255) / 256) * 256;
-      static constexpr size_t result_offset = counter_offset + 256;
-      static constexpr size_t bytes = result_offset + 256;
-
-      NT *partials() const { return (NT *)d_scratch; }
-      unsigned *counter() const { return (unsigned *)((char *)d_scratch + counter_offset); }
-      NT *result() const { return (NT *)((char *)d_scratch + result_offset); }
-
-      static Pending make()
+      struct Block {
+        void *d = nullptr;
+        size_t cap_req = 0, cap_data = 0;
+        hipStream_t s = nullptr; // the stream that last used it
+      };
+      // Arenas of threads that have ended.  A thread's destructor makes no HIP call (it may run at
+      // process exit); the next arena that needs memory takes one of these instead, after waiting
+      // for the stream that used it.
+      static std::mutex &orphan_mutex()
       {
-        Pending p;
-        void *s = nullptr;
-        DiFfRG::dfrg_check(dfrg_thread_stream(&s), "thread stream");
-        p.stream = (hipStream_t)s;
-        DiFfRG::dfrg_check(dfrg_pool_alloc(&p.d_scratch, bytes), "scratch");
-        DiFfRG::dfrg_check(dfrg_pool_host_alloc(&p.h_result, sizeof(NT)), "result slot");
-        return p;
+        static std::mutex m;
+        return m;
       }
-      // after the kernel launch: copy the result back on the same stream
-      void enqueue_result()
+      static std::vector<Block> &orphans()
       {
-        DiFfRG::check_hip("integrator kernel launch");
-        if (hipMemcpyAsync(h_result, result(), sizeof(NT), hipMemcpyDeviceToHost, stream) != hipSuccess)
-          throw std::runtime_error("integrator: result copy failed");
+        static std::vector<Block> v;
+        return v;
       }
-      NT wait()
+
+    public:
+      ~Arena()
       {
-        if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("integrator: stream error");
-        NT v;
-        std::memcpy((void *)&v, h_result, sizeof(NT));
-        dfrg_pool_free(d_scratch);
-        dfrg_pool_host_free(h_result);
-        return constant + v;
+        if (b.d) {
+          std::lock_guard<std::mutex> lock(orphan_mutex());
+          orphans().push_back(b);
+        }
       }
+      // Room for n_req counters and `data` bytes; counters are zero at the start of every launch.
+      void ensure(size_t n_req, size_t data)
+      {
+        if (!s) {
+          void *sv = nullptr;
+          check(dfrg_thread_stream(&sv), "thread stream");
+          s = (hipStream_t)sv;
+        }
+        if (b.d && n_req <= b.cap_req && data <= b.cap_data) return;
+        const size_t want_req = std::max<size_t>({n_req, 2 * b.cap_req, 64});
+        const size_t want_data = std::max<size_t>({data, 2 * b.cap_data, size_t(1) << 16});
+        if (b.d) {
+          if (hipStreamSynchronize(s) != hipSuccess) throw std::runtime_error("integrator: stream error");
+          dfrg_pool_free(b.d);
+          b.d = nullptr;
+        }
+        {
+          std::lock_guard<std::mutex> lock(orphan_mutex());
+          auto &o = orphans();
+          for (size_t i = 0; i < o.size(); ++i)
+            if (o[i].cap_req >= n_req && o[i].cap_data >= data) {
+              b = o[i];
+              o.erase(o.begin() + (long)i);
+              break;
+            }
+        }
+        if (b.d) {
+          if (hipStreamSynchronize(b.s) != hipSuccess) throw std::runtime_error("integrator: stream error");
+        } else {
+          b.cap_req = want_req;
+          b.cap_data = want_data;
+          check(dfrg_pool_alloc(&b.d, counter_bytes() + b.cap_data), "integrator scratch");
+        }
+        b.s = s;
+        if (hipMemsetAsync(b.d, 0, counter_bytes(), s) != hipSuccess)
+          throw std::runtime_error("integrator: scratch initialisation failed");
+      }
+      unsigned *counters() const { return (unsigned *)b.d; }
+      char *data() const { return (char *)b.d + counter_bytes(); }
+      hipStream_t stream() const { return s; }
+
+    private:
+      size_t counter_bytes() const { return b.cap_req * dev::kCounterStride * sizeof(unsigned); }
+      Block b;
+      hipStream_t s = nullptr;
     };
+    inline Arena &thread_arena()
+    {
+      thread_local Arena a;
+      return a;
+    }
+    inline size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+
+    // the kernel instantiation for a plan's block size
+    template <class S, class NT, class KERNEL, class C, class Args>
+    void launch_batch(const Nodes<C> &n, const Plan &p, const Args *d_args, int n_req, double *partials,
+                      unsigned *counters, NT *results, hipStream_t s)
+    {
+      const dim3 grid((unsigned)((long long)n_req * p.tpr));
+      if (p.block == 256)
+        hipLaunchKernelGGL((batch_kernel<S, 256, NT, KERNEL, C, Args>), grid, dim3(256), 0, s, n, p, d_args, partials,
+                           counters, results);
+      else
+        hipLaunchKernelGGL((batch_kernel<S, 64, NT, KERNEL, C, Args>), grid, dim3(64), 0, s, n, p, d_args, partials,
+                           counters, results);
+    }
 
     // ------------------------------------------------------------------------------------------
-    // batched requests.  request() calls of one integrator are collected: their (k, t...) tuples
-    // wait on the host, and reading any of their futures launches one kernel for all of them (nb
-    // blocks per request, the single-call partition) and copies all results back at once.  This
-    // turns a request_data loop over a grid of external points into one launch and one sync.
+    // batches of requests
     // ------------------------------------------------------------------------------------------
     constexpr size_t kMaxBatch = size_t(1) << 16;
 
     struct BatchBase {
       virtual ~BatchBase() = default;
+      virtual void flush() = 0;
     };
 
+    // Batches opened on this thread that are not launched yet.
+    inline std::vector<std::weak_ptr<BatchBase>> &thread_open()
+    {
+      thread_local std::vector<std::weak_ptr<BatchBase>> v;
+      return v;
+    }
+    // Launch every open batch of the calling thread (in the order they were opened).
+    inline void flush_thread()
+    {
+      auto &v = thread_open();
+      if (v.empty()) return;
+      std::vector<std::weak_ptr<BatchBase>> w;
+      w.swap(v);
+      for (auto &x : w)
+        if (auto b = x.lock()) b->flush();
+    }
+
     template <class NT, class Args> struct RequestBatch : BatchBase {
-      using Launch = std::function<void(const Args *d_args, int n, int nb, NT *partials, unsigned *counters,
-                                        NT *results, hipStream_t stream)>;
+      using Launch = std::function<void(const Args *d_args, int n, const Plan &p, double *partials,
+                                        unsigned *counters, NT *results, hipStream_t s)>;
       std::mutex m;
       std::vector<Args> args;
       std::vector<NT> constants, results;
-      int nb = 1;
-      bool done = false;
       Launch launch;
+      Plan plan;
+      bool launched = false, ready = false;
+      std::exception_ptr error;
+      hipEvent_t ev = nullptr;
+      void *h = nullptr; // pinned: tuples, then results
+
+      ~RequestBatch() override
+      {
+        if (ev) {
+          hipEventSynchronize(ev);
+          hipEventDestroy(ev);
+        }
+        if (h) dfrg_pool_host_free(h);
+      }
+
+      void flush() override
+      {
+        std::lock_guard<std::mutex> lock(m);
+        if (!launched) start();
+      }
 
       NT get(size_t i)
       {
+        flush_thread();
         std::lock_guard<std::mutex> lock(m);
-        if (!done) {
-          run();
-          done = true;
+        if (!launched) start();
+        if (!ready && !error) {
+          if (hipEventSynchronize(ev) != hipSuccess)
+            error = std::make_exception_ptr(std::runtime_error("integrator: batched request failed"));
+          else {
+            results.resize(constants.size());
+            std::memcpy((void *)results.data(), (char *)h + up256(sizeof(Args) * constants.size()),
+                        sizeof(NT) * constants.size());
+            ready = true;
+          }
         }
+        if (error) std::rethrow_exception(error);
         return constants[i] + results[i];
       }
 
     private:
-      static size_t up(size_t b) { return (b + 255) / 256 * 256; }
-      void run()
+      // Enqueue on this thread's stream: tuples to HBM, the kernel, results back; record the event.
+      void start()
       {
+        launched = true;
         const int n = (int)args.size();
-        void *sv = nullptr;
-        DiFfRG::dfrg_check(dfrg_thread_stream(&sv), "thread stream");
-        const hipStream_t s = (hipStream_t)sv;
-        const size_t o_part = up(sizeof(Args) * n);
-        const size_t o_cnt = o_part + up(sizeof(NT) * (size_t)n * nb);
-        const size_t o_res = o_cnt + up(sizeof(unsigned) * dev::kCounterStride * n);
-        const size_t bytes = o_res + up(sizeof(NT) * n);
-        const size_t h_res_off = up(sizeof(Args) * n);
-        void *d = nullptr, *h = nullptr;
-        DiFfRG::dfrg_check(dfrg_pool_alloc(&d, bytes), "batch scratch");
-        DiFfRG::dfrg_check(dfrg_pool_host_alloc(&h, h_res_off + sizeof(NT) * n), "batch staging");
-        char *dc = (char *)d, *hc = (char *)h;
-        std::memcpy(hc, (const void *)args.data(), sizeof(Args) * n);
-        bool ok = hipMemcpyAsync(dc, hc, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
-                  hipMemsetAsync(dc + o_cnt, 0, sizeof(unsigned) * dev::kCounterStride * n, s) == hipSuccess;
-        if (ok) {
-          launch((const Args *)dc, n, nb, (NT *)(dc + o_part), (unsigned *)(dc + o_cnt), (NT *)(dc + o_res), s);
-          ok = hipGetLastError() == hipSuccess &&
-               hipMemcpyAsync(hc + h_res_off, dc + o_res, sizeof(NT) * n, hipMemcpyDeviceToHost, s) == hipSuccess &&
-               hipStreamSynchronize(s) == hipSuccess;
+        try {
+          const Plan &p = plan;
+          const size_t o_args = 0, o_part = up256(sizeof(Args) * n);
+          const size_t o_res = o_part + up256(sizeof(double) * dev::kSlotStride * (size_t)n * (p.tpr > 1 ? p.tpr : 0));
+          const size_t bytes = o_res + up256(sizeof(NT) * n);
+          Arena &A = thread_arena();
+          A.ensure((size_t)n, bytes);
+          const hipStream_t s = A.stream();
+          const size_t h_res = up256(sizeof(Args) * n);
+          check(dfrg_pool_host_alloc(&h, h_res + sizeof(NT) * n), "batch staging");
+          std::memcpy(h, (const void *)args.data(), sizeof(Args) * n);
+          char *dd = A.data();
+          bool ok = hipMemcpyAsync(dd + o_args, h, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess;
+          if (ok) {
+            launch((const Args *)(dd + o_args), n, p, (double *)(dd + o_part), A.counters(), (NT *)(dd + o_res), s);
+            ok = hipGetLastError() == hipSuccess &&
+                 hipMemcpyAsync((char *)h + h_res, dd + o_res, sizeof(NT) * n, hipMemcpyDeviceToHost, s) ==
+                     hipSuccess &&
+                 hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
+                 hipEventRecord(ev, s) == hipSuccess;
+          }
+          if (!ok) throw std::runtime_error("integrator: batched request launch failed");
+        } catch (...) {
+          error = std::current_exception();
         }
-        if (ok) {
-          results.resize(n);
-          std::memcpy((void *)results.data(), hc + h_res_off, sizeof(NT) * n);
-        }
-        dfrg_pool_free(d);
-        dfrg_pool_host_free(h);
         args.clear();
         args.shrink_to_fit();
-        if (!ok) throw std::runtime_error("integrator: batched request launch failed");
       }
     };
 
     // Collects the requests of one integrator into batches.  A new batch opens when the argument
-    // types or the integrator state that the launch depends on (`state`, e.g. the Matsubara rule)
-    // change, when the open batch has been read, or at kMaxBatch requests.
+    // types or the integrator state the launch depends on (`state`, e.g. the Matsubara rule)
+    // change, when the open batch has been launched, or at kMaxBatch requests (the full batch is
+    // launched at once).
     template <class NT> class RequestBatcher
     {
     public:
       template <class Args, class MakeLaunch>
-      std::future<NT> enqueue(unsigned long long state, int nb, NT constant, Args &&args, MakeLaunch &&make_launch)
+      std::future<NT> enqueue(unsigned long long state, const Plan &plan, NT constant, Args &&args,
+                              MakeLaunch &&make_launch)
       {
         using B = RequestBatch<NT, std::decay_t<Args>>;
         std::shared_ptr<B> b;
         size_t idx = 0;
+        bool full = false;
         {
           std::lock_guard<std::mutex> lock(m);
-          b = std::dynamic_pointer_cast<B>(open);
+          b = std::dynamic_pointer_cast<B>(open.lock());
           for (int attempt = 0;; ++attempt) {
             if (!b || state != open_state || attempt > 0) {
               b = std::make_shared<B>();
-              b->nb = nb;
               b->launch = make_launch();
+              b->plan = plan;
               open = b;
               open_state = state;
+              thread_open().push_back(b);
             }
             std::lock_guard<std::mutex> lb(b->m);
-            if (b->done || b->args.size() >= kMaxBatch) continue; // read already, or full: open another
+            if (b->launched || b->args.size() >= kMaxBatch) continue; // launched already, or full
             idx = b->args.size();
             b->args.push_back(std::forward<Args>(args));
             b->constants.push_back(constant);
+            full = b->args.size() == kMaxBatch;
             break;
           }
         }
+        if (full) b->flush();
         return std::async(std::launch::deferred, [b, idx]() { return b->get(idx); });
       }
 
     private:
       std::mutex m;
-      std::shared_ptr<BatchBase> open;
+      std::weak_ptr<BatchBase> open;
       unsigned long long open_state = 0;
     };
+
+    // ------------------------------------------------------------------------------------------
+    // the integrator side: get() and request() for a shape
+    // ------------------------------------------------------------------------------------------
+    template <class S, class NT, class KERNEL, class C, class... T>
+    NT get(const Nodes<C> &n, C k, const T &...t)
+    {
+      flush_thread(); // this thread's open batches go first
+      using Args = std::tuple<C, T...>;
+      const Plan p = plan_of<S>(n);
+      const NT constant = KERNEL::constant(k, t...);
+      Arena &A = thread_arena();
+      const size_t o_res = up256(sizeof(double) * dev::kSlotStride * (p.tpr > 1 ? p.tpr : 0));
+      A.ensure(1, o_res + up256(sizeof(NT)));
+      const hipStream_t s = A.stream();
+      double *partials = (double *)A.data();
+      NT *res = (NT *)(A.data() + o_res);
+      const Args args(k, t...);
+      if (p.block == 256)
+        hipLaunchKernelGGL((get_kernel<S, 256, NT, KERNEL, C, Args>), dim3(p.tpr), dim3(256), 0, s, n, p, args,
+                           partials, A.counters(), res);
+      else
+        hipLaunchKernelGGL((get_kernel<S, 64, NT, KERNEL, C, Args>), dim3(p.tpr), dim3(64), 0, s, n, p, args,
+                           partials, A.counters(), res);
+      check_hip("integrator kernel launch");
+      thread_local void *h_slot = nullptr;
+      if (!h_slot) check(dfrg_pool_host_alloc(&h_slot, 256), "result slot");
+      static_assert(sizeof(NT) <= 256, "number type larger than the result slot");
+      if (hipMemcpyAsync(h_slot, res, sizeof(NT), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        throw std::runtime_error("integrator: stream error");
+      NT v;
+      std::memcpy((void *)&v, h_slot, sizeof(NT));
+      return constant + v;
+    }
+
+    template <class S, class NT, class KERNEL, class C, class... T>
+    std::future<NT> request(RequestBatcher<NT> &batcher, unsigned long long state, const Nodes<C> &n, C k,
+                            const T &...t)
+    {
+      using Args = std::tuple<C, T...>;
+      return batcher.enqueue(state, plan_of<S>(n), KERNEL::constant(k, t...), Args(k, t...), [n]() {
+        return [n](const Args *a, int r, const Plan &p, double *part, unsigned *cnt, NT *res, hipStream_t s) {
+          launch_batch<S, NT, KERNEL>(n, p, a, r, part, cnt, res, s);
+        };
+      });
+    }
   } // namespace tmpl
 } // namespace dfrg

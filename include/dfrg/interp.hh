@@ -1,6 +1,6 @@
-// DiFfRG/physics/interpolation/common.hh -- shared pieces of the interpolators: the return type of
-// the FP32 ("Tex") tables (reference interpolation/common.hh:8-25), their derivative channel for
-// dual numbers (DiFfRG/common/dual_traits.hh), and the host + HBM table storage behind every
+// dfrg/interp.hh -- shared pieces of the drop-in interpolators: the return type of the FP32
+// ("Tex") tables (the role of the reference's interpolation/common.hh:8-25), their derivative
+// channel for dual numbers (dfrg/dual.hh), and the host + HBM table storage behind every
 // interpolator.
 //
 // Copies of an interpolator share one storage and are non-owning views: the copy an integrator
@@ -16,24 +16,24 @@
 #include <memory>
 #include <stdexcept>
 
-#include <DiFfRG/common/dual_traits.hh>
-#include <DiFfRG/common/hip_prefix.hh>
+#include "dfrg/dual.hh"
+#include "dfrg/check.hh"
 
-namespace DiFfRG
+namespace dfrg
 {
-  namespace internal
+  namespace interp
   {
     // value type of the FP32 tables: float for float and double data, the dual type itself for
     // dual data (reference interpolation/common.hh:10-24)
-    template <typename NT> struct __TLITypes {
+    template <typename NT> struct TexReturn {
       static_assert(DualTraits<NT>::is_dual, "FP32 table interpolators hold float, double or dual-number data");
-      using ReturnType = NT;
+      using type = NT;
     };
-    template <> struct __TLITypes<float> {
-      using ReturnType = float;
+    template <> struct TexReturn<float> {
+      using type = float;
     };
-    template <> struct __TLITypes<double> {
-      using ReturnType = float;
+    template <> struct TexReturn<double> {
+      using type = float;
     };
 
     // n values of type V on the host and in HBM
@@ -46,15 +46,15 @@ namespace DiFfRG
         ensure_device();
         host = std::shared_ptr<V[]>(new V[n]());
         void *p = nullptr;
-        dfrg_check(dfrg_malloc(&p, sizeof(V) * n), "interpolator table");
+        check(dfrg_malloc(&p, sizeof(V) * n), "interpolator table");
         dev = std::shared_ptr<void>(p, [](void *q) { dfrg_free(q); });
       }
       V *host_ptr() const { return host.get(); }
       const V *device_ptr() const { return (const V *)dev.get(); }
       void upload() const
       {
-        dfrg_check(dfrg_memcpy_h2d(dev.get(), host.get(), sizeof(V) * n, nullptr), "interpolator upload");
-        dfrg_check(dfrg_stream_sync(nullptr), "interpolator upload");
+        check(dfrg_memcpy_h2d(dev.get(), host.get(), sizeof(V) * n, nullptr), "interpolator upload");
+        check(dfrg_stream_sync(nullptr), "interpolator upload");
       }
       unsigned size() const { return n; }
 
@@ -152,7 +152,7 @@ namespace DiFfRG
     // One interpolated value of a Tex interpolator: lerp(values), or for dual data
     // make(lerp(values), lerp(derivatives)).  lerp maps a table pointer to the FP32 result.
     template <typename NT, typename L>
-    __forceinline__ __host__ __device__ typename __TLITypes<NT>::ReturnType tex_value(const float *v, const float *dv,
+    __forceinline__ __host__ __device__ typename TexReturn<NT>::type tex_value(const float *v, const float *dv,
                                                                                        const L &lerp)
     {
       if constexpr (DualTraits<NT>::is_dual)
@@ -166,6 +166,6 @@ namespace DiFfRG
       for (int i = 0; i < dim; ++i)
         if (n[i] < 2) throw std::runtime_error(std::string(who) + ": every axis needs at least 2 points");
     }
-  } // namespace internal
-} // namespace DiFfRG
+  } // namespace interp
+} // namespace dfrg
 

@@ -9,7 +9,7 @@
 #include <cmath>
 #include <limits>
 
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
 
 namespace dfrg
@@ -25,12 +25,12 @@ namespace dfrg
 
       void set_T(const ctype T_new, const ctype E_new = 0)
       {
-        const bool keep = DiFfRG::is_close(T_new, T) && (!device || E_new != 0) &&
+        const bool keep = dfrg::is_close(T_new, T) && (!device || E_new != 0) &&
                           (std::abs(E_new - E) / std::max(E_new, E) < 2.5e-2);
         if (keep) return;
         T = T_new;
-        E = DiFfRG::is_close(E_new, (ctype)0) ? 10 * T : E_new;
-        manual_E = !DiFfRG::is_close(E_new, (ctype)0);
+        E = dfrg::is_close(E_new, (ctype)0) ? 10 * T : E_new;
+        manual_E = !dfrg::is_close(E_new, (ctype)0);
         size = (unsigned)provider->template get_matsubara_points<ctype>(T, E).size();
         if constexpr (device) {
           x = provider->template get_device_matsubara_points<ctype>(T, E);

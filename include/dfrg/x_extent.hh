@@ -14,7 +14,7 @@
 #include <functional>
 #include <stdexcept>
 
-#include <DiFfRG/common/math.hh>
+#include "dfrg/math.hh"
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
 
 namespace dfrg
@@ -41,7 +41,7 @@ namespace dfrg
     {
       bool all_zero = true;
       for (int i = 0; i < 10; ++i)
-        if (!DiFfRG::is_close(optimize_x(1.0 + 1e-3 + (std::pow(1.5, (double)i) - 1.0)), 0.0)) all_zero = false;
+        if (!dfrg::is_close(optimize_x(1.0 + 1e-3 + (std::pow(1.5, (double)i) - 1.0)), 0.0)) all_zero = false;
       if (all_zero) return 1.0;
 
       const unsigned o1 = quadrature_factor * x_quadrature_order;

@@ -6,7 +6,7 @@
 #include <utility>
 
 #include <DiFfRG/common/complex_math.hh>
-#include <DiFfRG/common/dual_traits.hh>
+#include "dfrg/dual.hh"
 
 namespace DiFfRG
 {
@@ -21,9 +21,9 @@ namespace DiFfRG
       // a dual number (DiFfRG/common/dual_traits.hh): the type of its value, as for
       // autodiff::Real<1, T> in the reference (types.hh:89-96)
       template <typename CT> struct _ctype {
-        static_assert(DiFfRG::internal::DualTraits<CT>::is_dual, "ctype: unsupported number type");
+        static_assert(dfrg::DualTraits<CT>::is_dual, "ctype: unsupported number type");
         using value = std::remove_cv_t<
-            std::remove_reference_t<decltype(DiFfRG::internal::DualTraits<CT>::value(std::declval<CT>()))>>;
+            std::remove_reference_t<decltype(dfrg::DualTraits<CT>::value(std::declval<CT>()))>>;
       };
       template <> struct _ctype<float> { using value = float; };
       template <> struct _ctype<double> { using value = double; };

@@ -1,0 +1,156 @@
+// compile_boundary.hip -- compile-only checks of the drop-in boundary (built with -c, never
+// linked or run; tests/test_boundary_compile.py drives it).
+//
+// 1. Every GPU and host integrator, the QuadratureProvider and request_data are instantiated with
+//    a parameter object that has exactly the getters of the reference's JSONValue
+//    (DiFfRG/include/DiFfRG/common/json.hh:62-139) and nothing else, declared only: the JSON
+//    constructors bind to the reference's own JSONValue, whatever backs it.
+// 2. With DFRG_REFERENCE_KERNELS, the reference's generated kernels are included unchanged by
+//    include path -- Examples/ONfiniteT/flows/V/V.kernel and Examples/YangMills/SP/flows/AA/ZA.kernel
+//    -- and instantiated in the integrators their wrappers use (V.cu: IntegratorTBB<3>; ZA.cu:
+//    IntegratorAngleGPU<4>), with the argument types of those wrappers.
+#include <string>
+
+#include <DiFfRG/common/math.hh>
+#include <DiFfRG/discretization/grid/coordinates.hh>
+#include <DiFfRG/physics/regulators.hh>
+#include <DiFfRG/physics/thermodynamics.hh>
+
+#include <DiFfRG/common/quadrature/quadrature_provider.hh>
+#include <DiFfRG/physics/integration/integrator_4D_2ang_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_4D_2ang_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_4D_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_4D_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_angle_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_angle_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_gpu.hh>
+#include <DiFfRG/physics/interpolation/linear_interpolation_1D.hh>
+#include <DiFfRG/physics/interpolation/tex_linear_interpolation_1D.hh>
+#include <DiFfRG/physics/utils.hh>
+
+using namespace ::DiFfRG;
+
+// the getters of the reference's JSONValue, declarations only
+struct ReferenceJSONValue {
+  double get_double(const std::string &key) const;
+  int get_int(const std::string &key) const;
+  unsigned get_uint(const std::string &key) const;
+  std::string get_string(const std::string &key) const;
+  bool get_bool(const std::string &key) const;
+  double get_double(const std::string &key, const double def) const;
+  int get_int(const std::string &key, const int def) const;
+  unsigned get_uint(const std::string &key, const unsigned def) const;
+  std::string get_string(const std::string &key, const std::string &def) const;
+  bool get_bool(const std::string &key, const bool def) const;
+};
+
+#ifdef DFRG_REFERENCE_KERNELS
+#include "V.kernel"
+#include "ZA.kernel"
+#endif
+
+namespace
+{
+  struct K1 {
+    static __host__ __device__ double kernel(double q, double k, double a) { return a * q * k; }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct K1p {
+    static __host__ __device__ double kernel(double q, double k, double p, double a) { return a * q * k / (p + q); }
+    static __host__ __device__ double constant(double, double, double) { return 0.; }
+  };
+  struct KAngle {
+    static __host__ __device__ double kernel(double q, double c, double k, double a) { return a * q * c * k; }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct K2Ang {
+    static __host__ __device__ double kernel(double q, double c1, double c2, double k, double a) { return a * q * c1 * c2 * k; }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct K4D {
+    static __host__ __device__ double kernel(double q, double c1, double c2, double ph, double k, double a)
+    {
+      return a * q * c1 * c2 * ph * k;
+    }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct KT {
+    static __host__ __device__ double kernel(double q, double q0, double k, double a) { return a / (q * q + q0 * q0 + k); }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct KAngleT {
+    static __host__ __device__ double kernel(double q, double c, double q0, double k, double a)
+    {
+      return a * c / (q * q + q0 * q0 + k);
+    }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct K4DT {
+    static __host__ __device__ double kernel(double q, double c, double ph, double q0, double k, double a)
+    {
+      return a * c * ph / (q * q + q0 * q0 + k);
+    }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+
+  // a wrapper of the shape the generated X_integrator classes have (templated request<NT>)
+  template <class I> struct Wrapper {
+    I &integ;
+    template <typename NT, typename... T> std::future<NT> request(T &&...t) { return integ.request(t...); }
+  };
+} // namespace
+
+double instantiate_with_reference_json(const ReferenceJSONValue &json)
+{
+  QuadratureProvider qp(json);
+  double s = 0.;
+  IntegratorGPU<4, double, K1> a(qp, {{64}}, 20., json);
+  IntegratorAngleGPU<4, double, KAngle> b(qp, {{64, 16}}, 20., json);
+  Integrator4D2AngGPU<double, K2Ang> c(qp, {{64, 32, 32}}, 20., json);
+  Integrator4DGPU<double, K4D> d(qp, {{32, 8, 8, 8}}, 20., json);
+  IntegratorFiniteTq0GPU<4, double, KT> e(qp, {{32}}, 20., json);
+  IntegratorAngleFiniteTq0GPU<4, double, KAngleT> f(qp, {{32, 8}}, 20., json);
+  Integrator4DFiniteTq0GPU<double, K4DT> g(qp, {{32, 8, 8}}, 20., json);
+  s += a.get(1., 2.) + b.get(1., 2.) + c.get(1., 2.) + d.get(1., 2.) + e.get(1., 2.) + f.get(1., 2.) + g.get(1., 2.);
+  s += a.request(1., 2.).get() + c.request(1., 2.).get() + e.request(1., 2.).get();
+  IntegratorTBB<4, double, K1> at(qp, {{64}}, 20., json);
+  IntegratorAngleTBB<4, double, KAngle> bt(qp, {{64, 16}}, 20., json);
+  Integrator4D2AngTBB<double, K2Ang> ct(qp, {{64, 32, 32}}, 20., json);
+  Integrator4DTBB<double, K4D> dt(qp, {{32, 8, 8, 8}}, 20., json);
+  IntegratorFiniteTq0TBB<4, double, KT> et(qp, {{32}}, 20., json);
+  s += at.get(1., 2.) + bt.get(1., 2.) + ct.get(1., 2.) + dt.get(1., 2.) + et.get(1., 2.);
+  // request_data over a coordinate grid, through a generated-wrapper-shaped object
+  LogarithmicCoordinates1D<double> grid(16, 1e-3, 10., 3.);
+  IntegratorGPU<4, double, K1p> ap(qp, {{64}}, 20., json);
+  Wrapper<IntegratorGPU<4, double, K1p>> w{ap};
+  std::vector<double> out(16);
+  update_data(request_data<double>(w, grid, 1., 2.), out.data());
+  return s + out[0];
+}
+
+#ifdef DFRG_REFERENCE_KERNELS
+double instantiate_reference_kernels(const ReferenceJSONValue &json)
+{
+  QuadratureProvider qp(json);
+  using REG = PolynomialExpRegulator<>;
+  // ONfiniteT V.cu: IntegratorTBB<3, double, V_kernel<REG>>, request(k, N, T, rhoPhi, m2Pi, m2Sigma)
+  IntegratorTBB<3, double, V_kernel<REG>> vt(qp, {{64}}, 20., json);
+  IntegratorGPU<3, double, V_kernel<REG>> vg(qp, {{64}}, 20., json);
+  double s = vt.get(1., 4., 0.1, 0.2, 0.3, 0.4) + vt.request(1., 4., 0.1, 0.2, 0.3, 0.4).get();
+  s += vg.get(1., 4., 0.1, 0.2, 0.3, 0.4) + vg.request(1., 4., 0.1, 0.2, 0.3, 0.4).get();
+  // YangMills SP ZA.cu: IntegratorAngleGPU<4, double, ZA_kernel<REG>>, seven dressing functions
+  using Tex = TexLinearInterpolator1D<double, LogarithmicCoordinates1D<float>>;
+  LogarithmicCoordinates1D<float> coords(64, 1e-3f, 40.f, 5.f);
+  Tex Z(coords);
+  IntegratorAngleGPU<4, double, ZA_kernel<REG>> za(qp, {{64, 16}}, 20., json);
+  s += za.get(1., 0.5, Z, Z, Z, Z, Z, Z, Z, 0.1) + za.request(1., 0.5, Z, Z, Z, Z, Z, Z, Z, 0.1).get();
+  return s;
+}
+#endif

@@ -23,7 +23,15 @@
 #include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_cpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_finiteTq0_gpu.hh>
-#include <DiFfRG/physics/interpolation.hh>
+#include <DiFfRG/discretization/grid/combined_coordinates.hh>
+#include <DiFfRG/discretization/grid/stack_coordinates.hh>
+#include <DiFfRG/physics/interpolation/linear_interpolation_1D.hh>
+#include <DiFfRG/physics/interpolation/linear_interpolation_2D.hh>
+#include <DiFfRG/physics/interpolation/linear_interpolation_3D.hh>
+#include <DiFfRG/physics/interpolation/tex_linear_interpolation_1D.hh>
+#include <DiFfRG/physics/interpolation/tex_linear_interpolation_1D_stack.hh>
+#include <DiFfRG/physics/interpolation/tex_linear_interpolation_2D.hh>
+#include <DiFfRG/physics/interpolation/tex_linear_interpolation_3D.hh>
 #include <DiFfRG/physics/regulators.hh>
 #include <DiFfRG/physics/utils.hh>
 
@@ -46,7 +54,7 @@ __host__ __device__ inline Dual1 operator*(const Dual1 &a, double s) { return Du
 __host__ __device__ inline Dual1 operator*(double s, const Dual1 &a) { return Dual1(s * a.v, s * a.d); }
 __host__ __device__ inline Dual1 operator+(const Dual1 &a, const Dual1 &b) { return Dual1(a.v + b.v, a.d + b.d); }
 
-namespace DiFfRG::internal
+namespace dfrg
 {
   template <> struct DualTraits<Dual1> {
     static constexpr bool is_dual = true;
@@ -54,10 +62,11 @@ namespace DiFfRG::internal
     __host__ __device__ static double derivative(const Dual1 &x) { return x.d; }
     __host__ __device__ static Dual1 make(double v, double d) { return Dual1(v, d); }
   };
-} // namespace DiFfRG::internal
+} // namespace dfrg
 #include <dfrg/explicit_rk.hh>
 #include <dfrg/x_extent.hh>
 
+#include <algorithm>
 #include <chrono>
 #include <memory>
 #include <string>
@@ -321,6 +330,24 @@ namespace
     }
     static __forceinline__ __host__ __device__ double constant(const double, const double, const ZInterp &) { return 0.; }
   };
+  // the same self-energy integrand written for FP32 arguments and an FP32 table: with NT = float
+  // the integrator evaluates it in FP32 on FP32 nodes and accumulates in FP64
+  using ZInterpF = LinearInterpolator1D<float, LogarithmicCoordinates1D<float>>;
+  struct SelfEnergyKernelF {
+    static __forceinline__ __host__ __device__ float G(const float k2, const float y2, const ZInterpF &Z)
+    {
+      return 1.0f / (Z(sqrtf(y2)) * y2 + k2 * expf(-y2 / k2));
+    }
+    static __forceinline__ __host__ __device__ float kernel(const float q, const float cos1, const float,
+                                                            const float k, const float p, const ZInterpF &Z)
+    {
+      const float k2 = k * k, q2 = q * q, x = q2 / k2;
+      const float Gq = G(k2, q2, Z);
+      return 2.0f * expf(-x) * k2 * (1.0f + x) * Gq * Gq * G(k2, p * p + q2 - 2.0f * p * q * cos1, Z);
+    }
+    static __forceinline__ __host__ __device__ float constant(const float, const float, const ZInterpF &) { return 0.f; }
+  };
+
   // finite-T two-propagator integrand
   struct FiniteTKernel {
     static __forceinline__ __host__ __device__ double G(const double k2, const double w, const double s, const double m)
@@ -400,15 +427,15 @@ int kat_wave_sum8(const double *in, double *out)
 {
   return guard([&] {
     void *d_in = nullptr, *d_out = nullptr;
-    dfrg_check(dfrg_malloc(&d_in, sizeof(double) * 512), "in");
-    dfrg_check(dfrg_malloc(&d_out, sizeof(double) * 64), "out");
+    dfrg::check(dfrg_malloc(&d_in, sizeof(double) * 512), "in");
+    dfrg::check(dfrg_malloc(&d_out, sizeof(double) * 64), "out");
     void *s = nullptr;
-    dfrg_check(dfrg_thread_stream(&s), "thread stream");
-    dfrg_check(dfrg_memcpy_h2d(d_in, in, sizeof(double) * 512, s), "in");
+    dfrg::check(dfrg_thread_stream(&s), "thread stream");
+    dfrg::check(dfrg_memcpy_h2d(d_in, in, sizeof(double) * 512, s), "in");
     hipLaunchKernelGGL(wave_sum8_kernel, dim3(1), dim3(64), 0, (hipStream_t)s, (const double *)d_in, (double *)d_out);
-    check_hip("wave_sum8_kernel");
-    dfrg_check(dfrg_memcpy_d2h(out, d_out, sizeof(double) * 64, s), "out");
-    dfrg_check(dfrg_stream_sync(s), "out");
+    dfrg::check_hip("wave_sum8_kernel");
+    dfrg::check(dfrg_memcpy_d2h(out, d_out, sizeof(double) * 64, s), "out");
+    dfrg::check(dfrg_stream_sync(s), "out");
     dfrg_free(d_in);
     dfrg_free(d_out);
   });
@@ -611,6 +638,14 @@ namespace
     }
     template <typename NT, typename... T> NT get(double k, const T &...t) { return integ.get(k, t...); }
   };
+  // the FP32 form: the wrapper converts k and p to the integrator's ctype
+  struct SelfEnergyWrapperF {
+    Integrator4D2AngGPU<float, SelfEnergyKernelF> integ;
+    template <typename NT, typename... T> std::future<NT> request(double k, double p, const T &...t)
+    {
+      return integ.request((float)k, (float)p, t...);
+    }
+  };
 } // namespace
 
 extern "C" {
@@ -690,6 +725,118 @@ int kat_request_timing(int nq, int na, double x_extent, double k, const double *
     auto t2 = std::chrono::steady_clock::now();
     *t_get = std::chrono::duration<double>(t1 - t0).count() / reps;
     *t_req = std::chrono::duration<double>(t2 - t1).count() / reps;
+  });
+}
+
+// a large batch of two-angle requests, read back, then get() calls with the same integrator: the
+// get() scratch must not inherit the batch's partials or tickets
+int kat_selfenergy_after_batch(int nq, int na, double x_extent, double k, const double *p, int n_p, const double *Z,
+                               int n_tab, double start, double stop, double bias, int n_batch, double *out_get,
+                               double *out_batch)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<double> coords((unsigned)n_tab, start, stop, bias);
+    ZInterp interp(Z, coords);
+    Integrator4D2AngGPU<double, SelfEnergyKernel> integ(provider(), {{(unsigned)nq, (unsigned)na, (unsigned)na}},
+                                                        x_extent);
+    std::vector<std::future<double>> f;
+    for (int i = 0; i < n_batch; ++i) f.push_back(integ.request(k, p[i % n_p] * (1.0 + 1e-3 * i), interp));
+    for (int i = 0; i < n_batch; ++i) out_batch[i] = f[i].get();
+    for (int i = 0; i < n_p; ++i) out_get[i] = integ.get(k, p[i], interp);
+  });
+}
+
+// request_data over n_p external momenta, host work of sleep_us, then the reads: seconds spent in
+// the reads (t_read) against seconds of a request_data + read with no host work between (t_full)
+int kat_request_overlap(int nq, int na, double x_extent, double k, int n_p, const double *Z, int n_tab, double start,
+                        double stop, double bias, int sleep_us, int reps, double *t_full, double *t_read)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<double> coords((unsigned)n_tab, start, stop, bias);
+    LogarithmicCoordinates1D<double> pgrid((unsigned)n_p, 1e-3, 40.0, 6.0);
+    ZInterp interp(Z, coords);
+    SelfEnergyWrapper w{Integrator4D2AngGPU<double, SelfEnergyKernel>(
+        provider(), {{(unsigned)nq, (unsigned)na, (unsigned)na}}, x_extent)};
+    std::vector<double> out(n_p);
+    std::vector<double> full, read;
+    for (int r = 0; r <= reps; ++r) {
+      auto t0 = std::chrono::steady_clock::now();
+      update_data(request_data<double>(w, pgrid, k, interp), out.data());
+      auto t1 = std::chrono::steady_clock::now();
+      auto fut = request_data<double>(w, pgrid, k, interp);
+      std::this_thread::sleep_for(std::chrono::microseconds(sleep_us));
+      auto t2 = std::chrono::steady_clock::now();
+      update_data(fut, out.data());
+      auto t3 = std::chrono::steady_clock::now();
+      if (r > 0) { // the first round warms up
+        full.push_back(std::chrono::duration<double>(t1 - t0).count());
+        read.push_back(std::chrono::duration<double>(t3 - t2).count());
+      }
+    }
+    std::sort(full.begin(), full.end());
+    std::sort(read.begin(), read.end());
+    *t_full = full[full.size() / 2];
+    *t_read = read[read.size() / 2];
+  });
+}
+
+// The self-energy call set through request_data (log grid of n_p momenta on [1e-3, 40], bias 6):
+// results, and the median over reps of the device time of one batch (HIP events on the thread's
+// stream around the request loop and the reads: tuples to HBM, the kernel, results back) and of
+// the host time.  fp32 != 0 runs the FP32 form of the integrand (NT = float, FP64 accumulation).
+int kat_selfenergy_request_bench(int nq, int na, double x_extent, double k, int n_p, const double *Z, int n_tab,
+                                 double start, double stop, double bias, int fp32, int reps, double *out,
+                                 double *p_out, double *t_dev, double *t_host)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<double> pgrid((unsigned)n_p, 1e-3, 40.0, 6.0);
+    const auto grid = make_grid(pgrid);
+    for (int i = 0; i < n_p; ++i) p_out[i] = grid[i];
+    void *sv = nullptr;
+    dfrg::check(dfrg_thread_stream(&sv), "thread stream");
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+      throw std::runtime_error("event create failed");
+    std::vector<double> dev, host;
+    auto run = [&](auto &w, auto &grid_c, auto kk, auto &interp, auto *res) {
+      for (int r = 0; r <= reps; ++r) {
+        auto t0 = std::chrono::steady_clock::now();
+        hipEventRecord(e0, (hipStream_t)sv);
+        auto fut = request_data<std::remove_pointer_t<decltype(res)>>(w, grid_c, kk, interp);
+        update_data(fut, res);
+        hipEventRecord(e1, (hipStream_t)sv);
+        hipEventSynchronize(e1);
+        auto t1 = std::chrono::steady_clock::now();
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) {
+          dev.push_back(ms * 1e-3);
+          host.push_back(std::chrono::duration<double>(t1 - t0).count());
+        }
+      }
+    };
+    if (!fp32) {
+      LogarithmicCoordinates1D<double> coords((unsigned)n_tab, start, stop, bias);
+      ZInterp interp(Z, coords);
+      SelfEnergyWrapper w{Integrator4D2AngGPU<double, SelfEnergyKernel>(
+          provider(), {{(unsigned)nq, (unsigned)na, (unsigned)na}}, x_extent)};
+      run(w, pgrid, k, interp, out);
+    } else {
+      LogarithmicCoordinates1D<float> coords((unsigned)n_tab, (float)start, (float)stop, (float)bias);
+      std::vector<float> Zf(Z, Z + n_tab);
+      ZInterpF interp(Zf, coords);
+      SelfEnergyWrapperF w{Integrator4D2AngGPU<float, SelfEnergyKernelF>(
+          provider(), {{(unsigned)nq, (unsigned)na, (unsigned)na}}, (float)x_extent)};
+      std::vector<float> outf(n_p);
+      run(w, pgrid, k, interp, outf.data());
+      for (int i = 0; i < n_p; ++i) out[i] = outf[i];
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    std::sort(dev.begin(), dev.end());
+    std::sort(host.begin(), host.end());
+    *t_dev = dev[dev.size() / 2];
+    *t_host = host[host.size() / 2];
   });
 }
 
@@ -822,17 +969,17 @@ namespace
   template <int DIM, class I> void interp_eval(const I &interp, const double *pts, int n, double *out_dev, double *out_host)
   {
     void *d_pts = nullptr, *d_out = nullptr;
-    dfrg_check(dfrg_malloc(&d_pts, sizeof(double) * DIM * n), "pts");
-    dfrg_check(dfrg_malloc(&d_out, sizeof(double) * n), "out");
+    dfrg::check(dfrg_malloc(&d_pts, sizeof(double) * DIM * n), "pts");
+    dfrg::check(dfrg_malloc(&d_out, sizeof(double) * n), "out");
     // the library's copies run on this thread's stream: launch there too
     void *s = nullptr;
-    dfrg_check(dfrg_thread_stream(&s), "thread stream");
-    dfrg_check(dfrg_memcpy_h2d(d_pts, pts, sizeof(double) * DIM * n, s), "pts");
+    dfrg::check(dfrg_thread_stream(&s), "thread stream");
+    dfrg::check(dfrg_memcpy_h2d(d_pts, pts, sizeof(double) * DIM * n, s), "pts");
     hipLaunchKernelGGL((interp_eval_kernel<DIM, I>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)s, interp,
                        (const double *)d_pts, n, (double *)d_out);
-    check_hip("interp_eval_kernel");
-    dfrg_check(dfrg_memcpy_d2h(out_dev, d_out, sizeof(double) * n, s), "out");
-    dfrg_check(dfrg_stream_sync(s), "out");
+    dfrg::check_hip("interp_eval_kernel");
+    dfrg::check(dfrg_memcpy_d2h(out_dev, d_out, sizeof(double) * n, s), "out");
+    dfrg::check(dfrg_stream_sync(s), "out");
     dfrg_free(d_pts);
     dfrg_free(d_out);
     for (int i = 0; i < n; ++i) {
@@ -956,16 +1103,16 @@ namespace
   void interp_dual_eval(const I &interp, const double *pts, int n, double *out_dev, double *out_host)
   {
     void *d_pts = nullptr, *d_out = nullptr;
-    dfrg_check(dfrg_malloc(&d_pts, sizeof(double) * DIM * n), "pts");
-    dfrg_check(dfrg_malloc(&d_out, sizeof(double) * 2 * n), "out");
+    dfrg::check(dfrg_malloc(&d_pts, sizeof(double) * DIM * n), "pts");
+    dfrg::check(dfrg_malloc(&d_out, sizeof(double) * 2 * n), "out");
     void *s = nullptr;
-    dfrg_check(dfrg_thread_stream(&s), "thread stream");
-    dfrg_check(dfrg_memcpy_h2d(d_pts, pts, sizeof(double) * DIM * n, s), "pts");
+    dfrg::check(dfrg_thread_stream(&s), "thread stream");
+    dfrg::check(dfrg_memcpy_h2d(d_pts, pts, sizeof(double) * DIM * n, s), "pts");
     hipLaunchKernelGGL((interp_dual_kernel<DIM, I>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)s, interp,
                        (const double *)d_pts, n, (double *)d_out);
-    check_hip("interp_dual_kernel");
-    dfrg_check(dfrg_memcpy_d2h(out_dev, d_out, sizeof(double) * 2 * n, s), "out");
-    dfrg_check(dfrg_stream_sync(s), "out");
+    dfrg::check_hip("interp_dual_kernel");
+    dfrg::check(dfrg_memcpy_d2h(out_dev, d_out, sizeof(double) * 2 * n, s), "out");
+    dfrg::check(dfrg_stream_sync(s), "out");
     dfrg_free(d_pts);
     dfrg_free(d_out);
     for (int i = 0; i < n; ++i) {
@@ -1148,14 +1295,14 @@ int kat_rk_device(int method, const double *A, double *y, int n, double t0, doub
 {
   return guard([&] {
     void *s = nullptr;
-    dfrg_check(dfrg_thread_stream(&s), "thread stream");
+    dfrg::check(dfrg_thread_stream(&s), "thread stream");
     void *dA, *dy, *dsnap = nullptr, *dst;
-    dfrg_check(dfrg_malloc(&dA, sizeof(double) * 4 * n), "A");
-    dfrg_check(dfrg_malloc(&dy, sizeof(double) * 2 * n), "y");
-    dfrg_check(dfrg_malloc(&dst, sizeof(int) * n), "status");
-    if (n_snap > 0) dfrg_check(dfrg_malloc(&dsnap, sizeof(double) * 2 * n * n_snap), "snap");
-    dfrg_check(dfrg_memcpy_h2d(dA, A, sizeof(double) * 4 * n, s), "A");
-    dfrg_check(dfrg_memcpy_h2d(dy, y, sizeof(double) * 2 * n, s), "y");
+    dfrg::check(dfrg_malloc(&dA, sizeof(double) * 4 * n), "A");
+    dfrg::check(dfrg_malloc(&dy, sizeof(double) * 2 * n), "y");
+    dfrg::check(dfrg_malloc(&dst, sizeof(int) * n), "status");
+    if (n_snap > 0) dfrg::check(dfrg_malloc(&dsnap, sizeof(double) * 2 * n * n_snap), "snap");
+    dfrg::check(dfrg_memcpy_h2d(dA, A, sizeof(double) * 4 * n, s), "A");
+    dfrg::check(dfrg_memcpy_h2d(dy, y, sizeof(double) * 2 * n, s), "y");
     LinearRHS rhs{(const double *)dA, nullptr, nullptr, 0, c};
     if (nx > 0) {
       rhs.xq = provider().get_device_points<double>(nx);
@@ -1164,10 +1311,10 @@ int kat_rk_device(int method, const double *A, double *y, int n, double t0, doub
     }
     dfrg::tmpl::rk_ensemble<2>(rhs, method, (double *)dy, n, t0, t1, dt, output_dt, (double *)dsnap, n_snap,
                                (int *)dst, (hipStream_t)s);
-    dfrg_check(dfrg_memcpy_d2h(y, dy, sizeof(double) * 2 * n, s), "y");
-    dfrg_check(dfrg_memcpy_d2h(status, dst, sizeof(int) * n, s), "status");
-    if (n_snap > 0) dfrg_check(dfrg_memcpy_d2h(snap, dsnap, sizeof(double) * 2 * n * n_snap, s), "snap");
-    dfrg_check(dfrg_stream_sync(s), "sync");
+    dfrg::check(dfrg_memcpy_d2h(y, dy, sizeof(double) * 2 * n, s), "y");
+    dfrg::check(dfrg_memcpy_d2h(status, dst, sizeof(int) * n, s), "status");
+    if (n_snap > 0) dfrg::check(dfrg_memcpy_d2h(snap, dsnap, sizeof(double) * 2 * n * n_snap, s), "snap");
+    dfrg::check(dfrg_stream_sync(s), "sync");
     dfrg_free(dA);
     dfrg_free(dy);
     dfrg_free(dst);

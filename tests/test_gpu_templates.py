@@ -450,3 +450,59 @@ def test_request_batching_is_one_launch_per_batch(kat):
     print(f"  64 x get(): {tg.value * 1e3:.3f} ms, 64 requests in one batch: {tr.value * 1e3:.3f} ms "
           f"({tg.value / tr.value:.1f}x)", flush=True)
     assert tr.value < tg.value
+
+
+def test_get_after_large_batch_matches_oracle(kat, qp):
+    """get() after a batch of 256 two-angle requests at {64, 32, 32} with the same integrator: the
+    batch's partial slots and tickets must not leak into the scratch get() uses."""
+    c = W.selfenergy(n_p=4)
+    p, Z = arr(c["p"]), arr(c["Z"])
+    n_batch = 256
+    out_get, out_batch = np.empty(4), np.empty(n_batch)
+    print("[case] 256 requests {64,32,32}, then 4 get()", flush=True)
+    call(kat, "kat_selfenergy_after_batch", 64, 32, D(20.0), D(1.0), P(p), 4, P(Z), len(Z), D(1e-3), D(40.0),
+         D(6.0), n_batch, P(out_get), P(out_batch))
+    coord = O.logcoord(len(Z), 1e-3, 40.0, 6.0)
+    xq, xa = (qp.get_points(64), qp.get_weights(64)), (qp.get_points(32), qp.get_weights(32))
+    ref = O.c1_selfenergy(p, [Z] * 4, [coord] * 4, [xq] * 4, [xa] * 4)
+    assert np.max(np.abs(out_get - ref) / ref) <= TOL
+    pb = arr([p[i % 4] * (1.0 + 1e-3 * i) for i in range(n_batch)])
+    refb = O.c1_selfenergy(pb[:8], [Z] * 8, [coord] * 8, [xq] * 8, [xa] * 8)
+    assert np.max(np.abs(out_batch[:8] - refb) / refb) <= TOL
+
+
+def test_request_data_launches_before_the_reads(kat):
+    """request_data launches its batch when its loop ends: with host work between request_data and
+    the reads longer than the batch, the reads find the results ready."""
+    c = W.selfenergy()
+    Z = arr(c["Z"])
+    tf, tr = D(), D()
+    print("[case] request_data, 5 ms of host work, then the reads", flush=True)
+    call(kat, "kat_request_overlap", 64, 32, D(20.0), D(1.0), 64, P(Z), len(Z), D(1e-3), D(40.0), D(6.0), 5000, 5,
+         C.byref(tf), C.byref(tr))
+    print(f"  request_data + reads: {tf.value * 1e6:.1f} us; reads after host work: {tr.value * 1e6:.1f} us",
+          flush=True)
+    assert tf.value < 5e-3  # the batch is shorter than the host work
+    assert tr.value < 0.5 * tf.value
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_selfenergy_request_data_full_size(kat, qp, fp32):
+    """The 64-momentum self-energy call set at {64, 32, 32} (4,194,304 evaluations) through
+    Integrator4D2AngGPU and request_data, against the oracle: FP64 at 1e-12; the FP32 form
+    (NT = float, LinearInterpolator1D<float>, FP64 accumulation) at 1e-5, its measured error is
+    printed."""
+    c = W.selfenergy()
+    Z = arr(c["Z"])
+    out, pg, td, th = np.empty(64), np.empty(64), D(), D()
+    print(f"[case] self-energy call set through request_data, fp32={fp32}", flush=True)
+    call(kat, "kat_selfenergy_request_bench", 64, 32, D(20.0), D(1.0), 64, P(Z), len(Z), D(1e-3), D(40.0), D(6.0),
+         fp32, 3, P(out), P(pg), C.byref(td), C.byref(th))
+    assert np.allclose(pg, c["p"], rtol=1e-15, atol=0)
+    coord = O.logcoord(len(Z), 1e-3, 40.0, 6.0)
+    xq, xa = (qp.get_points(64), qp.get_weights(64)), (qp.get_points(32), qp.get_weights(32))
+    ref = O.c1_selfenergy(pg, [Z] * 64, [coord] * 64, [xq] * 64, [xa] * 64)
+    err = np.max(np.abs(out - ref) / np.abs(ref))
+    print(f"  device {td.value * 1e6:.1f} us, host {th.value * 1e6:.1f} us per call set; max rel err {err:.2e}",
+          flush=True)
+    assert err <= (1e-5 if fp32 else TOL)
