@@ -50,6 +50,21 @@ SE_OPS = {0: {"pairs": (223.5, 203.0), "single": (223.5, 205.0)},
 SE_OPS_BLOCK = {}  # (mode, block, loop) whose schedule differs from the others
 
 
+def frozen_weights():
+    """FLOP (and VALU) per evaluation frozen in BASELINE.md (the ```json flop-weights block)."""
+    text = open(os.path.join(ROOT, "BASELINE.md")).read()
+    start = text.index("```json flop-weights") + len("```json flop-weights")
+    return json.loads(text[start:text.index("```", start)])
+
+
+FROZEN = frozen_weights()
+
+
+def ceiling(w):
+    """Instruction-mix bound of a loop: FLOP / (2 VALU) per evaluation."""
+    return w["flop"] / (2.0 * w["valu"])
+
+
 def se_loop(na, block, unroll=1):
     if 2 * min(na * na, block) > na * na:
         return "single"
@@ -83,8 +98,11 @@ def parse():
     ap.add_argument("--only-headline", action="store_true", help="skip the other workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
-    ap.add_argument("--flows", type=int, default=48, help="flows in the serving run")
-    ap.add_argument("--workloads", default="threshold,finite_T,uneven,flows",
+    ap.add_argument("--flows", type=int, default=1000, help="flows in the steady-state serving run")
+    ap.add_argument("--flow-check-frac", type=float, default=0.0,
+                    help="fraction of the served flows checked against the CPU reference (one flow per CPU "
+                         "thread); 0: six flows spanning the grid sizes")
+    ap.add_argument("--workloads", default="template,threshold,finite_T,uneven,flows",
                     help="comma-separated workloads measured after the headline")
     ap.add_argument("--shape", action="append", default=[], metavar="MODE:BLOCK,TILE,GRID",
                     help="fixed launch shape instead of the autotuner for the headline (MODE fp64|mixed); "
@@ -112,14 +130,16 @@ def measured_traffic():
 
 
 def copy_calibration():
-    """Copies a 256 MiB buffer 5 times with torch's copy kernel: 256 MiB read + 256 MiB written."""
-    import torch
-    src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
+    """Copies a 256 MiB device buffer 5 times with dfrg_memcpy_d2d: 256 MiB read + 256 MiB written."""
+    import diffrg_amd as D
+    D.init(0)
+    n = 1 << 25
+    src = D.DeviceArray.from_host(np.ones(n))
+    dst = D.DeviceArray(n)
     for _ in range(5):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "copies": 5}))
+        D.memcpy_d2d(dst, src)
+    D.stream_sync()
+    print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 5}))
 
 
 def log(msg):
@@ -257,7 +277,8 @@ def selfenergy(D, qp, args):
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
         shape = bb.autotune(mode, reps=3)
         ms, _ = time_steps(D, lambda: bb.run(outb, mode), max(1, args.steps // BATCH), max(1, args.warmup // BATCH))
-        fl, va = se_ops(se_variant(bb, mode), c["na"], shape["block"], bb.unroll(mode))
+        va = se_ops(se_variant(bb, mode), c["na"], shape["block"], bb.unroll(mode))[1]
+        fl = FROZEN["C1_selfenergy"][name]["flop"]
         res[name][f"batched_x{BATCH}"] = {
             "integrals_per_s": nb / (ms * 1e-3), "us_per_launch": ms * 1e3, "shape": shape,
             "max_rel_err": rel_err(outb.to_host(), refb),
@@ -281,8 +302,11 @@ def threshold(D, qp, args):
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
         ms, _ = time_steps(D, lambda: D.threshold_batch(d_m2, d_x, d_w, out, c["x_extent"], c["k"], mode, c["d"]),
                            args.steps, args.warmup)
+        wt = FROZEN["C0_threshold"][name]
         r[name] = {"integrals_per_s": len(c["m2"]) / (ms * 1e-3), "us_per_step": ms * 1e3,
-                   "max_rel_err": rel_err(out.to_host(), ref)}
+                   "max_rel_err": rel_err(out.to_host(), ref),
+                   "roofline_frac": wt["flop"] * len(c["m2"]) * 64 / (ms * 1e-3) / PEAK_FP64_VALU,
+                   "ceiling": ceiling(wt)}
     # BATCH call sets in one launch (the same 256 integrals repeated): throughput without the fixed
     # per-launch latency that dominates a 256-integral launch
     d_m2b = D.DeviceArray.from_host(np.tile(c["m2"], BATCH))
@@ -330,8 +354,12 @@ def finite_T(D, qp, args):
         ms, _ = time_steps(D, run, args.steps, args.warmup)
         got = out.to_host()
         err = np.abs(got - ref) / np.abs(ref)
+        wt = FROZEN["C2_finite_T"][name]
+        pairs = int(sum(c["nx"] * len(c["rules"][t][0]) for t in c["T"]))
         r[name] = {"integrals_per_s": len(c["T"]) / (ms * 1e-3), "evals_per_s": evals / (ms * 1e-3),
                    "us_per_step": ms * 1e3, "max_rel_err": float(err.max()), "block": best[1],
+                   "roofline_frac": wt["flop"] * pairs / (ms * 1e-3) / PEAK_FP64_VALU, "ceiling": ceiling(wt),
+                   "valu_issue_frac": valu_issue_frac(wt["valu"], pairs, ms * 1e-3),
                    "max_rel_err_per_T": {str(t): float(err[c["T"] == t].max()) for t in W.TEMPS}}
     # BATCH call sets in one launch (the same 512 externals repeated)
     devb = dict(dev)
@@ -388,8 +416,10 @@ def uneven(D, qp, args):
         shape = b.autotune(mode, reps=2, cache_path=None)
         ms, _ = time_steps(D, lambda: b.graph_launch(out, mode, launches=per_graph), steps, 2)
         ms /= per_graph
+        wt = FROZEN["C3_uneven"][name]
         r[name] = {"integrals_per_s": n / (ms * 1e-3), "evals_per_s": b.evals / (ms * 1e-3), "ms_per_batch": ms,
                    "max_rel_err": rel_err(out.to_host(), ref), "shape": shape,
+                   "roofline_frac": wt["flop"] * b.evals / (ms * 1e-3) / PEAK_FP64_VALU, "ceiling": ceiling(wt),
                    "valu_issue_frac": sum(int(q) * int(a) ** 2 * se_ops(int(v), int(a), shape["block"],
                                                                           b.unroll(mode))[1]
                                           for q, a, v in zip(c["nq"], c["na"], b.variants(mode)))
@@ -403,60 +433,137 @@ def uneven(D, qp, args):
     return r
 
 
-def flows(D, qp, args):
-    """Flow serving: saturation throughput, then Poisson arrivals at 0.7x saturation."""
+KAT_LIB = os.path.join(ROOT, "tests", "native", "libdfrg_kat.so")
+
+
+def template(D, qp, args):
+    """C1 through the drop-in boundary: Integrator4D2AngGPU<NT, KERNEL> with the self-energy as a
+    user KERNEL (LinearInterpolator1D dressing), its 64 requests made as request_data makes them and
+    read back.  The C++ side is the template test library (tests/native/libdfrg_kat.so, a user of
+    the drop-in headers).  Device time: HIP events on the request thread's stream around the batch's
+    launch and the reads (tuples to HBM, the kernel, results back); host time: the whole call set
+    (64 requests, launch, reads)."""
+    import ctypes as C
     import diffrg_amd.workloads as W
     from oracle import oracle as O
-    n_flows = args.flows
+    kat = C.CDLL(KAT_LIB)
+    kat.kat_last_error.restype = C.c_char_p
+    c = W.selfenergy()
+    Z = np.ascontiguousarray(c["Z"])
+    n = len(c["p"])
+    evals = n * c["nq"] * c["na"] ** 2
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    coord = O.logcoord(c["n_tab"], *c["coord"])
+    xq = (qp.get_points(c["nq"]), qp.get_weights(c["nq"]))
+    xa = (qp.get_points(c["na"]), qp.get_weights(c["na"]))
+    r = {"evals_per_call_set": evals,
+         "path": "Integrator4D2AngGPU<NT, SelfEnergyKernel> + request_data/update_data (physics/utils.hh)"}
+    for fp32, name, tol in ((0, "fp64", 1e-12), (1, "fp32_eval_fp64_acc", 1e-5)):
+        out, pg = np.empty(n), np.empty(n)
+        td, th = C.c_double(), C.c_double()
+        rc = kat.kat_selfenergy_request_bench(c["nq"], c["na"], C.c_double(c["x_extent"]), C.c_double(c["k"]), n,
+                                              P(Z), len(Z), C.c_double(c["coord"][0]), C.c_double(c["coord"][1]),
+                                              C.c_double(c["coord"][2]), fp32, 20, P(out), P(pg), C.byref(td),
+                                              C.byref(th))
+        if rc != 0:
+            raise RuntimeError(kat.kat_last_error())
+        ref = O.c1_selfenergy(pg, [c["Z"]] * n, [coord] * n, [xq] * n, [xa] * n, c["x_extent"], c["k"])
+        err = rel_err(out, ref)
+        w = FROZEN["C1_template"]["fp64" if not fp32 else "fp32"]
+        t = td.value
+        r[name] = {"integrals_per_s": n / t, "evals_per_s": evals / t, "us_device_per_call_set": t * 1e6,
+                   "us_host_per_call_set": th.value * 1e6, "max_rel_err_vs_fp64_oracle": err,
+                   "within_tolerance": err <= tol, "tolerance": tol,
+                   "flop_per_eval": w["flop"], "valu_per_eval": w["valu"], "ceiling": ceiling(w),
+                   "roofline_frac": w["flop"] * evals / t / PEAK_FP64_VALU,
+                   "valu_issue_frac": valu_issue_frac(w["valu"], evals, t)}
+        log(f"template {name}: {t * 1e6:.1f} us device, {th.value * 1e6:.1f} us host per call set, err {err:.2e}")
+    return r
+
+
+def _flow_sample(reqs, m, seed):
+    """m flows spanning the grid sizes: the sorted sizes cut into m strata, one seeded pick in each."""
+    rng = np.random.default_rng(seed)
+    order = sorted(range(len(reqs)), key=lambda i: reqs[i]["n_grid"])
+    edges = np.linspace(0, len(order), m + 1).astype(int)
+    return sorted(order[int(rng.integers(edges[j], max(edges[j] + 1, edges[j + 1])))] for j in range(m))
+
+
+def flows(D, qp, args):
+    """Flow serving: saturation throughput (flows submitted at once), then a steady-state run of
+    Poisson arrivals at 0.7x saturation, with served/offered rate and p50/p99 latency over the
+    window that excludes the first and last 10% of arrivals, and final potentials of a seeded
+    sample of flows spanning the grid sizes against the CPU reference."""
+    import diffrg_amd.workloads as W
+    from oracle import oracle as O
     budget = 64 << 20
-    # saturation: all flows submitted at t=0
+    # saturation: n_sat flows submitted at t = 0
+    n_sat = min(args.flows, 512)
     fs = D.FlowServer(qp, hbm_budget=budget, max_resident=64, steps_per_launch=25)
-    reqs = W.flows(n_flows, rate=1.0)
+    reqs = W.flows(n_sat, rate=1.0)
     t0 = time.perf_counter()
-    ids = [fs.submit(q["n_grid"], q["T"], q["mu"], q["rho_max"], q["N"], q["m2_0"], q["lambda"]) for q in reqs]
+    for q in reqs:
+        fs.submit(q["n_grid"], q["T"], q["mu"], q["rho_max"], q["N"], q["m2_0"], q["lambda"])
     done = 0
-    while done < n_flows:
+    while done < n_sat:
         _, _, done = fs.progress()
-    sat = n_flows / (time.perf_counter() - t0)
-    results = {fid: fs.result(fid)[0] for fid in ids}
+    sat = n_sat / (time.perf_counter() - t0)
     fs.close()
-    # Poisson arrivals at 0.7 x saturation
+    # steady state: Poisson arrivals at 0.7 x saturation
+    n_flows = args.flows
     rate = 0.7 * sat
     reqs = W.flows(n_flows, rate=rate)
     fs = D.FlowServer(qp, hbm_budget=budget, max_resident=64, steps_per_launch=25)
-    pending = list(reqs)
-    ids2 = []
+    pending = list(range(n_flows))
+    ids = [None] * n_flows
     t0 = time.perf_counter()
     finished = 0
     while finished < n_flows:
         now = time.perf_counter() - t0
-        while pending and pending[0]["arrival"] <= now:
-            q = pending.pop(0)
-            ids2.append((fs.submit(q["n_grid"], q["T"], q["mu"], q["rho_max"], q["N"], q["m2_0"], q["lambda"]), q))
+        while pending and reqs[pending[0]]["arrival"] <= now:
+            i = pending.pop(0)
+            q = reqs[i]
+            ids[i] = fs.submit(q["n_grid"], q["T"], q["mu"], q["rho_max"], q["N"], q["m2_0"], q["lambda"])
         r, qd, finished = fs.progress()
         if r == 0 and qd == 0 and pending:
-            time.sleep(max(0.0, min(0.002, pending[0]["arrival"] - (time.perf_counter() - t0))))
+            time.sleep(max(0.0, min(0.002, reqs[pending[0]]["arrival"] - (time.perf_counter() - t0))))
     elapsed = time.perf_counter() - t0
-    st = fs.stats()
-    # CPU reference flows for a sample (smallest and largest grids) and the in-mix results
-    order = sorted(range(n_flows), key=lambda i: reqs[i]["n_grid"])
-    sample = sorted({order[0], order[1 % n_flows], order[-1]})
+    res = [fs.result(fid) for fid in ids]
+    lat = np.array([l for _, l in res])
+    arr = np.array([q["arrival"] for q in reqs])
+    # window: arrivals between the 10% and 90% quantiles; a flow completes at arrival + latency
+    lo, hi = np.quantile(arr, 0.1), np.quantile(arr, 0.9)
+    in_win = (arr >= lo) & (arr <= hi)
+    comp = arr + lat
+    offered = in_win.sum() / (hi - lo)
+    served = ((comp >= lo) & (comp <= hi)).sum() / (hi - lo)
+    # CPU reference for a seeded sample spanning the grid sizes
     x, w = qp.get_points(64), qp.get_weights(64)
-    max_diff, cpu_t = 0.0, []
-    for i in sample:
-        fid, q = ids2[i]
-        u, _ = fs.result(fid)
-        fp = O.flow_params(q["n_grid"], q["rho_max"], q["T"], q["mu"], q["N"], q["m2_0"], q["lambda"])
-        t1 = time.perf_counter()
-        uref = O.flow_lpa(fp, x, w)
-        cpu_t.append((time.perf_counter() - t1, q["n_grid"]))
-        max_diff = max(max_diff, float(np.max(np.abs(u - uref))))
+    m = max(4, int(math.ceil(args.flow_check_frac * n_flows))) if args.flow_check_frac > 0 else 6
+    sample = _flow_sample(reqs, m, seed=0xD1FF0004)
+    fps = [O.flow_params(reqs[i]["n_grid"], reqs[i]["rho_max"], reqs[i]["T"], reqs[i]["mu"], reqs[i]["N"],
+                         reqs[i]["m2_0"], reqs[i]["lambda"]) for i in sample]
+    t1 = time.perf_counter()
+    if args.flow_check_frac > 0:
+        urefs = O.flow_lpa_many(fps, x, w)  # one flow per thread
+    else:
+        urefs = [O.flow_lpa(fp, x, w) for fp in fps]  # each flow's right-hand side over the threads
+    cpu_s = time.perf_counter() - t1
+    diffs = [float(np.max(np.abs(res[i][0] - u))) for i, u in zip(sample, urefs)]
     fs.close()
-    cpu_flow_s = sum(t for t, _ in cpu_t) / len(cpu_t)
-    return {"flows": n_flows, "saturation_flows_per_s": sat, "offered_flows_per_s": rate,
-            "served_flows_per_s": n_flows / elapsed, "p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"],
-            "max_abs_u_diff_vs_cpu": max_diff, "cpu_reference_flow_s_mean": cpu_flow_s,
-            "cpu_sample": [{"n_grid": n, "seconds": t} for t, n in cpu_t],
+    return {"flows": n_flows, "saturation_flows_per_s": sat, "saturation_flows": n_sat,
+            "offered_flows_per_s": rate, "run_s": elapsed,
+            "steady_state": {"window_s": [float(lo), float(hi)], "flows_in_window": int(in_win.sum()),
+                             "offered_flows_per_s": float(offered), "served_flows_per_s": float(served),
+                             "served_over_offered": float(served / offered),
+                             "p50_latency_s": float(np.quantile(lat[in_win], 0.5)),
+                             "p99_latency_s": float(np.quantile(lat[in_win], 0.99))},
+            "p50_latency_s_all": float(np.quantile(lat, 0.5)), "p99_latency_s_all": float(np.quantile(lat, 0.99)),
+            "cpu_check": {"flows": len(sample), "fraction": len(sample) / n_flows,
+                          "n_grid": [reqs[i]["n_grid"] for i in sample], "max_abs_u_diff": max(diffs),
+                          "tolerance": 1e-8, "within_tolerance": max(diffs) <= 1e-8, "cpu_seconds": cpu_s,
+                          "threads": O.num_threads(),
+                          "parallel_over": "flows" if args.flow_check_frac > 0 else "grid points"},
             "config": "O(4) LPA flows, n_grid ~ U{64..1024}, T ~ U[0.01,0.3], mu ~ U[0,0.4], RK4 2000 steps "
                       "k=1 -> 1e-3, two 64-node threshold integrals per point and stage; HBM budget 64 MiB, "
                       "64 slots, 25 steps per graph launch"}
@@ -506,17 +613,23 @@ def main():
     }
     na = se["na"]
     v1 = se["variant"][1]
-    fl1, va1 = se_ops(v1, na, mx["shape"]["block"], mx["unroll"])
-    fl0, va0 = se_ops(se["variant"][0], na, fp["shape"]["block"], fp["unroll"])
+    # FLOP per evaluation: the weights frozen in BASELINE.md; VALU per evaluation: the current
+    # assembly of the loop that ran (for the issue fraction)
+    w1, w0 = FROZEN["C1_selfenergy"]["mixed"], FROZEN["C1_selfenergy"]["fp64"]
+    fl1, fl0 = w1["flop"], w0["flop"]
+    va1 = se_ops(v1, na, mx["shape"]["block"], mx["unroll"])[1]
+    va0 = se_ops(se["variant"][0], na, fp["shape"]["block"], fp["unroll"])[1]
     achieved = fl1 * evals / (mx["ms"] * 1e-3) / 1e12
     peak = PEAK_FP64_VALU / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                        "frac": achieved / peak, "traffic": measured_traffic(), "kernel": f"se_kernel<MIXED, {mx['shape']['block']}>",
                        "integrand_form": {1: "general", 2: "narrow"}[v1],
-                       "flop_per_eval": fl1, "valu_per_eval": va1, "loop": se_loop(na, mx["shape"]["block"], mx["unroll"]),
+                       "flop_per_eval": fl1, "flop_per_eval_source": "frozen (BASELINE.md flop-weights)",
+                       "ceiling": ceiling(w1), "valu_per_eval": va1,
+                       "loop": se_loop(na, mx["shape"]["block"], mx["unroll"]),
                        "evals_per_launch": evals,
                        "valu_issue_frac": valu_issue_frac(va1, evals, mx["ms"] * 1e-3),
-                       "plain_fp64": {"flop_per_eval": fl0, "valu_per_eval": va0,
+                       "plain_fp64": {"flop_per_eval": fl0, "valu_per_eval": va0, "ceiling": ceiling(w0),
                                       "loop": se_loop(na, fp["shape"]["block"], fp["unroll"]),
                                       "achieved": fl0 * evals / (fp["ms"] * 1e-3) / 1e12,
                                       "frac": fl0 * evals / (fp["ms"] * 1e-3) / PEAK_FP64_VALU,
@@ -537,7 +650,8 @@ def main():
     if not args.only_headline:
         work = {}
         chosen = set(args.workloads.split(","))
-        for name, fn in (("threshold", threshold), ("finite_T", finite_T), ("uneven", uneven), ("flows", flows)):
+        for name, fn in (("template", template), ("threshold", threshold), ("finite_T", finite_T),
+                         ("uneven", uneven), ("flows", flows)):
             if name not in chosen:
                 continue
             t0 = time.perf_counter()

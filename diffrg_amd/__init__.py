@@ -208,6 +208,13 @@ def _need(a: "DeviceArray", n: int, dtype, what: str) -> None:
         raise ValueError(f"{what}: holds {a.n} values, {n} are required")
 
 
+def memcpy_d2d(dst: "DeviceArray", src: "DeviceArray", stream=None) -> None:
+    """Device-to-device copy of src into dst (enqueued on `stream`)."""
+    if dst.dtype != src.dtype or dst.n < src.n:
+        raise ValueError("memcpy_d2d: destination smaller than source or of another dtype")
+    check(lib().dfrg_memcpy_d2d(dst.ptr, src.ptr, src.n * src.dtype.itemsize, stream))
+
+
 def stream_sync(stream=None) -> None:
     check(lib().dfrg_stream_sync(stream))
 
@@ -406,14 +413,24 @@ class SEBatch:
                  x_extent=20.0, k=1.0):
         self.ptr = None  # until the library object exists
         n = len(p)
+        arr = lambda v, dt: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dt), (n,)))
+        # one copy of each distinct table: integrals that pass the same table object on the same
+        # coordinates share its cells (one staging per table in HBM and L2)
+        start_a, stop_a, bias_a = arr(start, np.float64), arr(stop, np.float64), arr(bias, np.float64)
         tab_n = np.array([len(t) for t in tables], np.int32)
         tab_off = np.zeros(n, np.int32)
-        tab_off[1:] = np.cumsum(tab_n)[:-1]
+        seen, parts, size = {}, [], 0
+        for i, t in enumerate(tables):
+            key = (id(t), float(start_a[i]), float(stop_a[i]), float(bias_a[i]))
+            if key not in seen:
+                seen[key] = size
+                parts.append(np.asarray(t, np.float64))
+                size += len(t)
+            tab_off[i] = seen[key]
         # an empty call set reaches the library, which rejects it with its own message
-        flat = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float64) for t in tables] or [np.zeros(0)]))
-        arr = lambda v, dt: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dt), (n,)))
-        self._keep = [arr(p, np.float64), arr(nq, np.int32), arr(na, np.int32), tab_off, tab_n,
-                      arr(start, np.float64), arr(stop, np.float64), arr(bias, np.float64), flat]
+        flat = np.ascontiguousarray(np.concatenate(parts or [np.zeros(0)]))
+        self._keep = [arr(p, np.float64), arr(nq, np.int32), arr(na, np.int32), tab_off, tab_n, start_a, stop_a,
+                      bias_a, flat]
         P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
         k_ = self._keep
         h = _vp()

@@ -234,8 +234,19 @@ namespace
     unsigned *counters; // one line per integral (dev::kCounterStride)
     unsigned *queue;    // [0] next tile, [kCounterStride] exited blocks (persistent mode)
     int persistent;
+    int xcd_map; // one tile per block: give each XCD a contiguous run of tiles
     double *out;
   };
+
+  // Blocks are dispatched to the eight XCDs in turn (block b to XCD b mod 8).  With xcd_map, the
+  // blocks of XCD x take the x-th contiguous run of the tile list, so the tiles of an integral --
+  // consecutive in the list -- share one L2 (its table staged once, its partials local).  A bijection
+  // of [0, G): XCD x owns per + (x < rem) blocks and as many consecutive tiles.
+  __device__ __forceinline__ int xcd_tile(int b, int G)
+  {
+    const int per = G >> 3, rem = G & 7, x = b & 7, k = b >> 3;
+    return x * per + (x < rem ? x : rem) + k;
+  }
 
   constexpr int kMaxTab = 256;
   constexpr int kMaxAng = 64;
@@ -417,7 +428,7 @@ namespace
         __syncthreads();
         t = __builtin_amdgcn_readfirstlane(s_tile);
       } else {
-        t = iter == 0 ? (int)blockIdx.x : A.n_tiles;
+        t = iter == 0 ? (A.xcd_map ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) : A.n_tiles;
       }
       if (t >= A.n_tiles) break;
       const SETile T = A.tiles[t];
@@ -575,6 +586,7 @@ struct dfrg_se_batch {
   // shape
   int block = 256, tile_evals = 8192, grid = 0, n_tiles = 0;
   std::vector<int> narrow; // per integral: the mixed path may use se_G_mixed<true>
+  int uniform_tiles = 0;    // every tile has the same evaluation count
   std::string signature;
   // graph
   hipGraph_t graph = nullptr;
@@ -627,6 +639,9 @@ namespace
       }
     }
     b->n_tiles = (int)tiles.size();
+    b->uniform_tiles = 1;
+    for (const SETile &t : tiles)
+      if ((long long)t.nqt * t.na * t.na != (long long)tiles[0].nqt * tiles[0].na * tiles[0].na) b->uniform_tiles = 0;
     if ((size_t)b->n_tiles > b->tiles_cap) {
       hipFree(b->d_tiles);
       hipFree(b->d_partials);
@@ -656,6 +671,8 @@ namespace
     a.counters = b->d_counters;
     a.queue = b->d_queue;
     a.persistent = b->grid > 0 ? 1 : 0;
+    // equal tiles only: with uneven tiles the longest-first order balances the XCDs better
+    a.xcd_map = !a.persistent && b->uniform_tiles;
     a.out = d_out;
     return a;
   }
@@ -803,6 +820,22 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
                    tab_n[i], kMaxTab);
     DFRG_CHECK_ARG(tab_off[i] >= 0 && tab_off[i] + tab_n[i] <= tables_len, "integral %d: table out of bounds", i);
     DFRG_CHECK_ARG(bias[i] != 0.0 && stop[i] > start[i], "integral %d: invalid coordinates", i);
+  }
+  // integrals may share table cells (the same tab_off), then with the same size and coordinates;
+  // distinct tables must not overlap (the mixed path keeps per-cell coordinate data)
+  {
+    std::vector<int> idx(n_int);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int a, int c) { return tab_off[a] < tab_off[c]; });
+    for (int k = 1; k < n_int; ++k) {
+      const int a = idx[k - 1], c = idx[k];
+      if (tab_off[a] == tab_off[c]) {
+        DFRG_CHECK_ARG(tab_n[a] == tab_n[c] && start[a] == start[c] && stop[a] == stop[c] && bias[a] == bias[c],
+                       "integrals %d and %d share a table with different sizes or coordinates", a, c);
+      } else {
+        DFRG_CHECK_ARG(tab_off[a] + tab_n[a] <= tab_off[c], "tables of integrals %d and %d overlap", a, c);
+      }
+    }
   }
   dfrg_se_batch *b = new dfrg_se_batch;
   b->qp = qp;

@@ -153,7 +153,9 @@ int dfrg_threshold_batch(int d, const double *d_m2, int n_int, const double *d_x
  * each integral with its own radial order nq[i], angular order na[i] (both angles), external
  * momentum p[i] and dressing table Z_i (tables[tab_off[i] .. + tab_n[i]]) on
  * LogarithmicCoordinates1D(tab_n[i], start[i], stop[i], bias[i]).  Radial orders 1..1024, angular
- * orders powers of two in [2, 64], tables of 2..256 values.  Host arrays; copied to HBM once. */
+ * orders powers of two in [2, 64], tables of 2..256 values.  Integrals may share a table (the same
+ * tab_off, then the same tab_n and coordinates); distinct tables must not overlap.  Host arrays;
+ * copied to HBM once. */
 typedef struct dfrg_se_batch dfrg_se_batch;
 int dfrg_se_batch_create(dfrg_se_batch **b, dfrg_qp *qp, int n_int, const double *p, const int *nq,
                          const int *na, const int *tab_off, const int *tab_n, const double *start,

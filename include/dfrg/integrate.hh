@@ -1,24 +1,27 @@
-// dfrg/integrate.hh -- the tile engine behind the drop-in integrator templates.
+// dfrg/integrate.hh -- the engine behind the drop-in integrator templates.
 //
 // A user KERNEL (static __host__ __device__ kernel()/constant()) is device code of the caller's
 // translation unit, so these kernels are templates compiled there by hipcc.  One integral is the
-// flat node space (radial node, inner nodes...) of its integrator; a tile is a contiguous range of
-// that space.  A block takes one tile: it stages the tile's radial nodes q_j and their factors
-// (integration element x weight) in LDS, its lanes walk the tile's nodes in mixed radix (no division
-// per node) and read the inner nodes' coordinates and weight factors from a per-integrator table,
-// and each node adds KERNEL::kernel(...) x (radial factor x inner factors) to an FP64 accumulator.
-// Blocks reduce with a wave64 DPP tree and LDS; the tiles of one integral meet in per-tile partial
-// slots whose last-arriving block adds them in slot order.  The tile plan, and so the summation
-// order, depends on the integral's node counts only: results are bitwise reproducible, and a
-// request in any batch gives bitwise the value of get().
+// flat node space (radial node, inner nodes...) of its integrator, cut into units that depend on its
+// node counts only (Plan).  A block takes a unit: for rows of many inner nodes the unit is one
+// radial node, and its lanes walk the inner nodes with q fixed, so the KERNEL's q-only
+// subexpressions are loop-invariant and computed once per lane and row; smaller rows are grouped,
+// their radial factors staged in LDS, and walked flat.  Lanes walk nodes in mixed radix (no
+// division per node), read the inner nodes' coordinates and weight factors from a per-integrator
+// table, and add KERNEL::kernel(...) x (radial factor x inner factors) to an FP64 accumulator.
+// Blocks reduce with a wave64 DPP tree and LDS; the units of one integral meet in per-unit partial
+// slots whose last-arriving block adds them in unit order.  Units, their lane assignment and the
+// reduction order depend on the node counts only: results are bitwise reproducible, and a request
+// in any batch gives bitwise the value of get().  Launches keep as many blocks as fit on the device
+// resident and let them stride over the units.
 //
 // Accumulation is in FP64 for every number type: FP32 number types (float, complex<float>) evaluate
 // the KERNEL in FP32 on FP32 nodes and accumulate in FP64, the mixed form of the template path.
 //
-// get() launches at once and waits.  request() adds to the integrator's open batch; a batch is
-// launched (asynchronously: tuples to HBM, one kernel, results back into pinned memory, an event)
-// when any future of this thread is read, when request_data() finishes its loop, when the batch is
-// full, or when a request needs another launch configuration.  Reading a future launches every open
+// get() is a batch of one, launched at once and waited for.  request() adds to the integrator's
+// open batch; a batch is launched (asynchronously: tuples to HBM, one kernel, results back into
+// pinned memory, an event) when any future of this thread is read, when request_data() finishes its
+// loop, when the batch is full, or when a request needs another launch configuration.  Reading a future launches every open
 // batch of the calling thread first, so batches of different integrators run back to back before
 // the first wait.
 //
@@ -31,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -54,10 +58,8 @@ namespace dfrg
 {
   namespace tmpl
   {
-    constexpr int kMaxQ = 256;   // radial nodes a tile may span (LDS staging)
-    constexpr long long kMinTile = 1024;
-    constexpr long long kMaxTile = 65536;
-
+    constexpr int kMaxQ = 256; // radial nodes a flat unit may span (LDS staging)
+    
     // ------------------------------------------------------------------------------------------
     // number types: scalar (float, double) or two-component (complex: real/imag; dual numbers:
     // value/derivative), accumulated component-wise in FP64
@@ -329,14 +331,29 @@ namespace dfrg
     }
 
     // ------------------------------------------------------------------------------------------
-    // the tile plan: tile_evals nodes per tile, tpr tiles per integral, block threads per tile
+    // the unit plan.  An integral's flat node space (radial digit slowest, as the reference's flat
+    // loops) is cut into units that depend on its node counts only:
+    //   rows of ni >= kRowMin inner nodes: one unit per radial node (rows of more than kMaxUnit
+    //     nodes in equal parts).  The lanes walk the row's inner nodes with q fixed, so every
+    //     subexpression of the KERNEL that depends on q (and the arguments) only is loop-invariant:
+    //     after inlining the compiler evaluates it once per lane and row, not per node;
+    //   rows of fewer inner nodes: units of `rows` whole rows (about kRowMin nodes), their radial
+    //     factors staged in LDS, walked flat.
+    // A unit has a fixed block size and lane assignment and its own partial slot, and the integral
+    // is the sum of its unit partials in unit order, so how units are grouped into launches and
+    // blocks does not enter the result.
     // ------------------------------------------------------------------------------------------
+    constexpr int kRowMin = 256, kMaxUnit = 65536;
+    constexpr int kRowNodesPerLane = 16;
+
     struct Plan {
-      long long per_req = 0; // nodes of one integral
-      int ni = 1;            // inner nodes per radial node
-      int tile = 0;          // nodes per tile
-      int tpr = 1;           // tiles per integral
-      int block = 256;
+      int nx = 1, ni = 1;
+      int row_mode = 0;
+      int rows = 1;  // radial nodes per unit (flat mode)
+      int parts = 1; // units per row (row mode)
+      int unit = 1;  // nodes per unit (row mode: inner nodes)
+      int upr = 1;   // units per integral
+      int block = 64;
     };
 
     inline int device_cus()
@@ -349,23 +366,25 @@ namespace dfrg
       return cus;
     }
 
-    // The plan depends on the integral's node count only, so a request in any batch and get() sum
-    // the same tiles in the same order: about 32 tiles per integral (a batch of a few dozen
-    // integrals fills the chip with ~8 blocks per CU), 1024..65536 nodes per tile, at most kMaxQ
-    // radial nodes per tile and never more nodes than the integral has.
-    inline Plan make_plan(long long per_req, int ni)
+    inline Plan make_plan(int nx, int ni)
     {
       Plan p;
-      p.per_req = per_req;
+      p.nx = nx;
       p.ni = ni;
-      long long t = (per_req + 31) / 32;
-      t = std::max(kMinTile, std::min(kMaxTile, t));
-      t = std::min(t, per_req);
-      t = std::min(t, (long long)(kMaxQ - 2) * ni);
-      t = std::max(t, 1LL);
-      p.tile = (int)t;
-      p.tpr = (int)((per_req + t - 1) / t);
-      p.block = t >= 1024 ? 256 : 64;
+      if (ni >= kRowMin) {
+        p.row_mode = 1;
+        p.parts = (ni + kMaxUnit - 1) / kMaxUnit;
+        p.unit = (ni + p.parts - 1) / p.parts;
+        p.upr = nx * p.parts;
+        int b = 64; // at least kRowNodesPerLane nodes per lane
+        while (b < 256 && 2 * b * kRowNodesPerLane <= p.unit) b *= 2;
+        p.block = b;
+      } else {
+        p.rows = std::min({(kRowMin + ni - 1) / ni, kMaxQ, nx});
+        p.unit = p.rows * ni;
+        p.upr = (nx + p.rows - 1) / p.rows;
+        p.block = 64;
+      }
       return p;
     }
 
@@ -373,16 +392,17 @@ namespace dfrg
     {
       int rad[S::ND];
       S::radix(n, rad);
-      long long per_req = 1;
-      for (int i = 0; i < S::ND; ++i) per_req *= rad[i];
-      return make_plan(per_req, (int)(per_req / rad[0]));
+      long long ni = 1;
+      for (int i = 1; i < S::ND; ++i) ni *= rad[i];
+      if (ni > (1LL << 30) || rad[0] < 1 || ni < 1) throw std::invalid_argument("integrator: node counts out of range");
+      return make_plan(rad[0], (int)ni);
     }
 
     // The mixed-radix digits of a flat node index (slowest first), advanced by a fixed stride S with
     // carries instead of a division per node.
     template <int N> struct NodeWalk {
       int dig[N], step[N], rad[N];
-      __device__ __forceinline__ NodeWalk(long long i, long long S, const int (&r)[N])
+      __device__ __forceinline__ NodeWalk(long long i, long long S, const int *r)
       {
 #pragma unroll
         for (int j = N - 1; j >= 1; --j) {
@@ -410,43 +430,64 @@ namespace dfrg
     };
 
     // ------------------------------------------------------------------------------------------
-    // one tile: request r, tile c of its tpr
+    // one unit u of integral r, then the block's share of the cross-unit sum
     // ------------------------------------------------------------------------------------------
-    template <class S, int B, class NT, class KERNEL, class C, class Args>
-    __device__ __forceinline__ void tile_body(const Nodes<C> &n, const Plan &p, int r, int c, const Args &args,
+    constexpr int kMaxBlock = 256;
+
+    template <class S, class NT, class KERNEL, class C, class Args>
+    __device__ __forceinline__ void unit_body(const Nodes<C> &n, const Plan &p, int r, int u, const Args &args,
                                               double *partials, unsigned *counters, NT *results)
     {
       constexpr int NC = Acc<NT>::N;
       __shared__ C s_q[kMaxQ], s_rf[kMaxQ];
-      __shared__ double s_red[NC][B / 64];
+      __shared__ double s_red[NC][kMaxBlock / 64];
       __shared__ bool s_last;
-      const long long f0 = (long long)c * p.tile;
-      const long long f1 = f0 + p.tile < p.per_req ? f0 + p.tile : p.per_req;
-      const int ix0 = (int)(f0 / p.ni), ix1 = (int)((f1 - 1) / p.ni);
+      const int B = (int)blockDim.x, tid = (int)threadIdx.x;
       const C k = std::get<0>(args);
-      for (int j = threadIdx.x; j <= ix1 - ix0; j += B) {
-        const int ix = ix0 + j;
-        const C q = k * sqrt(n.xq[ix] * n.x_extent);
-        s_q[j] = q;
-        s_rf[j] = S::element(q, k) * (n.wq[ix] * n.x_extent);
-      }
-      __syncthreads();
-
-      Acc<NT> acc;
       int rad[S::ND];
       S::radix(n, rad);
-      NodeWalk<S::ND> w(f0 + threadIdx.x, B, rad);
-      for (long long f = f0 + threadIdx.x; f < f1; f += B, w.next()) {
-        const int j = w.dig[0] - ix0;
-        acc.add(std::apply(
-            [&](const C &kk, const auto &...t) {
-              return S::template term<NT, KERNEL>(n, s_q[j], s_rf[j], w.dig, kk, t...);
-            },
-            args));
+      Acc<NT> acc;
+      if constexpr (S::ND > 1) {
+        if (p.row_mode) {
+          const int ix = u / p.parts, part = u - ix * p.parts;
+          const int m0 = part * p.unit, m1 = m0 + p.unit < p.ni ? m0 + p.unit : p.ni;
+          const C q = k * sqrt(n.xq[ix] * n.x_extent);
+          const C rf = S::element(q, k) * (n.wq[ix] * n.x_extent);
+          auto eval = [&](const NodeWalk<S::ND - 1> &w) {
+            int g[S::ND];
+            g[0] = ix;
+#pragma unroll
+            for (int i = 1; i < S::ND; ++i) g[i] = w.dig[i - 1];
+            return std::apply(
+                [&](const C &kk, const auto &...t) { return S::template term<NT, KERNEL>(n, q, rf, g, kk, t...); },
+                args);
+          };
+          NodeWalk<S::ND - 1> w(m0 + tid, B, rad + 1);
+          for (int m = m0 + tid; m < m1; m += B, w.next()) acc.add(eval(w));
+        }
+      }
+      if (!p.row_mode) {
+        const int ix0 = u * p.rows, ix1 = ix0 + p.rows < p.nx ? ix0 + p.rows : p.nx;
+        for (int j = tid; j < ix1 - ix0; j += B) {
+          const C q = k * sqrt(n.xq[ix0 + j] * n.x_extent);
+          s_q[j] = q;
+          s_rf[j] = S::element(q, k) * (n.wq[ix0 + j] * n.x_extent);
+        }
+        __syncthreads();
+        const long long f1 = (long long)ix1 * p.ni;
+        NodeWalk<S::ND> w((long long)ix0 * p.ni + tid, B, rad);
+        for (long long f = (long long)ix0 * p.ni + tid; f < f1; f += B, w.next()) {
+          const int j = w.dig[0] - ix0;
+          acc.add(std::apply(
+              [&](const C &kk, const auto &...t) {
+                return S::template term<NT, KERNEL>(n, s_q[j], s_rf[j], w.dig, kk, t...);
+              },
+              args));
+        }
       }
 
-      // block: wave sums, then wave 0's lanes in wave order
-      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+      // block: wave sums, then wave 0's totals in wave order
+      const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
       for (int i = 0; i < NC; ++i) {
         const double v = dev::wave_sum(acc.v[i]);
@@ -459,56 +500,74 @@ namespace dfrg
         tot[i] = s_red[i][0];
         for (int j = 1; j < B / 64; ++j) tot[i] += s_red[i][j];
       }
-      if (p.tpr == 1) {
-        if (threadIdx.x == 0) results[r] = Acc<NT>::value(tot);
-        return;
-      }
-      // tiles of the integral: publish, ticket, the last block adds the partials in slot order and
-      // resets the counter
-      double *slots = partials + (long long)r * p.tpr * dev::kSlotStride;
-      unsigned *counter = counters + (long long)r * dev::kCounterStride;
-      if (threadIdx.x == 0) {
+      if (p.upr == 1) {
+        if (tid == 0) results[r] = Acc<NT>::value(tot);
+      } else {
+        // units of the integral: publish, ticket; the last block adds the partials in unit order
+        // and resets the counter
+        double *slots = partials + (long long)r * p.upr * dev::kSlotStride;
+        unsigned *counter = counters + (long long)r * dev::kCounterStride;
+        if (tid == 0) {
 #pragma unroll
-        for (int i = 0; i < NC; ++i) dev::publish(&slots[(long long)c * dev::kSlotStride + i], tot[i]);
-        dev::wait_stores();
-        s_last = dev::take_ticket(counter) == (unsigned)(p.tpr - 1);
-      }
-      __syncthreads();
-      if (s_last && threadIdx.x < 64) {
-        double s[NC];
+          for (int i = 0; i < NC; ++i) dev::publish(&slots[(long long)u * dev::kSlotStride + i], tot[i]);
+          dev::wait_stores();
+          s_last = dev::take_ticket(counter) == (unsigned)(p.upr - 1);
+        }
+        __syncthreads();
+        if (s_last && tid < 64) {
+          double sum[NC];
 #pragma unroll
-        for (int i = 0; i < NC; ++i) s[i] = 0.0;
-        for (int c0 = 0; c0 < p.tpr; c0 += 64) {
-          const int jj = c0 + (int)threadIdx.x;
-          const int m = p.tpr - c0 < 64 ? p.tpr - c0 : 64;
+          for (int i = 0; i < NC; ++i) sum[i] = 0.0;
+          for (int c0 = 0; c0 < p.upr; c0 += 64) {
+            const int jj = c0 + tid;
+            const int mm = p.upr - c0 < 64 ? p.upr - c0 : 64;
 #pragma unroll
-          for (int i = 0; i < NC; ++i) {
-            const double v = jj < p.tpr ? dev::fetch(&slots[(long long)jj * dev::kSlotStride + i]) : 0.0;
-            for (int l = 0; l < m; ++l) s[i] += dev::lane_value(v, l);
+            for (int i = 0; i < NC; ++i) {
+              const double v = jj < p.upr ? dev::fetch(&slots[(long long)jj * dev::kSlotStride + i]) : 0.0;
+              for (int l = 0; l < mm; ++l) sum[i] += dev::lane_value(v, l);
+            }
+          }
+          if (tid == 0) {
+            results[r] = Acc<NT>::value(sum);
+            dev::reset_counter(counter);
           }
         }
-        if (threadIdx.x == 0) {
-          results[r] = Acc<NT>::value(s);
-          dev::reset_counter(counter);
-        }
+      }
+      __syncthreads(); // LDS is free for the block's next unit
+    }
+
+    // a batch of integrals whose (k, t...) tuples sit in HBM; blocks stride over all units
+    template <class S, class NT, class KERNEL, class C, class Args>
+    __global__ __launch_bounds__(kMaxBlock) void batch_kernel(
+        Nodes<C> n, Plan p, const Args *args, int n_req, double *partials, unsigned *counters, NT *results)
+    {
+      const long long total = (long long)n_req * p.upr;
+      for (long long g = blockIdx.x; g < total; g += gridDim.x) {
+        const int r = (int)(g / p.upr), u = (int)(g - (long long)r * p.upr);
+        unit_body<S, NT, KERNEL>(n, p, r, u, args[r], partials, counters, results);
       }
     }
 
-    // one integral, its (k, t...) passed by value
-    template <class S, int B, class NT, class KERNEL, class C, class Args>
-    __global__ __launch_bounds__(B) void get_kernel(Nodes<C> n, Plan p, Args args, double *partials,
-                                                    unsigned *counters, NT *results)
+    // Blocks of `block` threads one launch keeps resident on the device: at most that many are
+    // launched, each striding over the units.
+    template <class F> int resident_blocks(F kernel, int block)
     {
-      tile_body<S, B, NT, KERNEL>(n, p, 0, (int)blockIdx.x, args, partials, counters, results);
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+      return per_cu * device_cus();
     }
-
-    // a batch of integrals whose (k, t...) tuples sit in HBM, tpr consecutive blocks per integral
-    template <class S, int B, class NT, class KERNEL, class C, class Args>
-    __global__ __launch_bounds__(B) void batch_kernel(Nodes<C> n, Plan p, const Args *args, double *partials,
-                                                      unsigned *counters, NT *results)
+    // the same, queried once per kernel and block size (64, 128 or 256)
+    template <auto kernel> int resident_cached(int block)
     {
-      const int r = (int)(blockIdx.x / (unsigned)p.tpr), c = (int)(blockIdx.x - (unsigned)r * p.tpr);
-      tile_body<S, B, NT, KERNEL>(n, p, r, c, args[r], partials, counters, results);
+      static std::atomic<int> cache[3] = {0, 0, 0};
+      const int i = block == 64 ? 0 : block == 128 ? 1 : 2;
+      int v = cache[i].load(std::memory_order_relaxed);
+      if (v == 0) {
+        v = resident_blocks(kernel, block);
+        cache[i].store(v, std::memory_order_relaxed);
+      }
+      return v;
     }
 
     // ------------------------------------------------------------------------------------------
@@ -596,18 +655,14 @@ namespace dfrg
     }
     inline size_t up256(size_t b) { return (b + 255) / 256 * 256; }
 
-    // the kernel instantiation for a plan's block size
     template <class S, class NT, class KERNEL, class C, class Args>
     void launch_batch(const Nodes<C> &n, const Plan &p, const Args *d_args, int n_req, double *partials,
                       unsigned *counters, NT *results, hipStream_t s)
     {
-      const dim3 grid((unsigned)((long long)n_req * p.tpr));
-      if (p.block == 256)
-        hipLaunchKernelGGL((batch_kernel<S, 256, NT, KERNEL, C, Args>), grid, dim3(256), 0, s, n, p, d_args, partials,
-                           counters, results);
-      else
-        hipLaunchKernelGGL((batch_kernel<S, 64, NT, KERNEL, C, Args>), grid, dim3(64), 0, s, n, p, d_args, partials,
-                           counters, results);
+      const long long total = (long long)n_req * p.upr;
+      constexpr auto kern = &batch_kernel<S, NT, KERNEL, C, Args>;
+      const int grid = (int)std::min<long long>(total, resident_cached<kern>(p.block));
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(p.block), 0, s, n, p, d_args, n_req, partials, counters, results);
     }
 
     // ------------------------------------------------------------------------------------------
@@ -693,7 +748,7 @@ namespace dfrg
         try {
           const Plan &p = plan;
           const size_t o_args = 0, o_part = up256(sizeof(Args) * n);
-          const size_t o_res = o_part + up256(sizeof(double) * dev::kSlotStride * (size_t)n * (p.tpr > 1 ? p.tpr : 0));
+          const size_t o_res = o_part + up256(sizeof(double) * dev::kSlotStride * (size_t)n * (p.upr > 1 ? p.upr : 0));
           const size_t bytes = o_res + up256(sizeof(NT) * n);
           Arena &A = thread_arena();
           A.ensure((size_t)n, bytes);
@@ -769,6 +824,8 @@ namespace dfrg
     // ------------------------------------------------------------------------------------------
     // the integrator side: get() and request() for a shape
     // ------------------------------------------------------------------------------------------
+    // get(): a batch of one, launched at once and waited for -- the same kernel as a batch of
+    // requests, so get() and request() give bitwise the same value
     template <class S, class NT, class KERNEL, class C, class... T>
     NT get(const Nodes<C> &n, C k, const T &...t)
     {
@@ -777,27 +834,34 @@ namespace dfrg
       const Plan p = plan_of<S>(n);
       const NT constant = KERNEL::constant(k, t...);
       Arena &A = thread_arena();
-      const size_t o_res = up256(sizeof(double) * dev::kSlotStride * (p.tpr > 1 ? p.tpr : 0));
+      const size_t o_part = up256(sizeof(Args));
+      const size_t o_res = o_part + up256(sizeof(double) * dev::kSlotStride * (p.upr > 1 ? p.upr : 0));
       A.ensure(1, o_res + up256(sizeof(NT)));
       const hipStream_t s = A.stream();
-      double *partials = (double *)A.data();
-      NT *res = (NT *)(A.data() + o_res);
+      // pinned staging of this thread: the tuple, then the result
+      thread_local void *h = nullptr;
+      thread_local size_t h_size = 0;
+      const size_t h_res = up256(sizeof(Args)), need = h_res + sizeof(NT);
+      if (h_size < need) {
+        if (h) dfrg_pool_host_free(h);
+        h = nullptr;
+        h_size = 0;
+        check(dfrg_pool_host_alloc(&h, need), "get() staging");
+        h_size = need;
+      }
       const Args args(k, t...);
-      if (p.block == 256)
-        hipLaunchKernelGGL((get_kernel<S, 256, NT, KERNEL, C, Args>), dim3(p.tpr), dim3(256), 0, s, n, p, args,
-                           partials, A.counters(), res);
-      else
-        hipLaunchKernelGGL((get_kernel<S, 64, NT, KERNEL, C, Args>), dim3(p.tpr), dim3(64), 0, s, n, p, args,
-                           partials, A.counters(), res);
+      std::memcpy(h, (const void *)&args, sizeof(Args));
+      char *dd = A.data();
+      if (hipMemcpyAsync(dd, h, sizeof(Args), hipMemcpyHostToDevice, s) != hipSuccess)
+        throw std::runtime_error("integrator: argument copy failed");
+      launch_batch<S, NT, KERNEL>(n, p, (const Args *)dd, 1, (double *)(dd + o_part), A.counters(),
+                                  (NT *)(dd + o_res), s);
       check_hip("integrator kernel launch");
-      thread_local void *h_slot = nullptr;
-      if (!h_slot) check(dfrg_pool_host_alloc(&h_slot, 256), "result slot");
-      static_assert(sizeof(NT) <= 256, "number type larger than the result slot");
-      if (hipMemcpyAsync(h_slot, res, sizeof(NT), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      if (hipMemcpyAsync((char *)h + h_res, dd + o_res, sizeof(NT), hipMemcpyDeviceToHost, s) != hipSuccess ||
           hipStreamSynchronize(s) != hipSuccess)
         throw std::runtime_error("integrator: stream error");
       NT v;
-      std::memcpy((void *)&v, h_slot, sizeof(NT));
+      std::memcpy((void *)&v, (char *)h + h_res, sizeof(NT));
       return constant + v;
     }
 

@@ -677,11 +677,35 @@ double orc_lpa_threshold(double m2, double T, double k, const double *x, const d
   return value;
 }
 
+/* orc_lpa_threshold with the node-only factors computed once per right-hand side: a_j = element
+ * x weight, r_j = RBdot, s_j = q^2 + RB (the same values and operation order, so bitwise the same). */
+static double lpa_threshold_tab(double m2, double T, const double *a, const double *r, const double *s, int nx)
+{
+  double value = 0.0;
+  for (int j = 0; j < nx; ++j) {
+    const double E = sqrt(s[j] + m2);
+    value += a[j] * (r[j] * bose_S2(E, T));
+  }
+  return value;
+}
+
 void orc_flow_rhs(const orc_flow_params *fp, const double *u, double k, const double *x, const double *w,
                   int nx, double x_extent, double b, double *dudt)
 {
   const int n = fp->n_grid;
   const double h = fp->rho_max / (n - 1);
+  double *tab = (double *)malloc(sizeof(double) * 3 * nx);
+  double *ta = tab, *tr = tab + nx, *ts = tab + 2 * nx;
+  {
+    const double k2 = k * k, S3 = S_d(3);
+    for (int j = 0; j < nx; ++j) {
+      const double q = k * sqrt(x[j] * x_extent);
+      const double q2 = q * q;
+      ta[j] = (S3 * (q / 2.0 * k2) / ipow(2.0 * M_PI, 3)) * (w[j] * x_extent);
+      tr[j] = orc_exp_RBdot(b, k2, q2);
+      ts[j] = q2 + orc_exp_RB(b, k2, q2);
+    }
+  }
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     double up, upp;
@@ -698,10 +722,11 @@ void orc_flow_rhs(const orc_flow_params *fp, const double *u, double k, const do
     const double rho = i * h;
     const double m2pi = u[i];
     const double m2sigma = u[i] + 2.0 * rho * up;
-    const double Ipi = orc_lpa_threshold(m2pi, fp->T, k, x, w, nx, x_extent, b);
-    const double Isig = orc_lpa_threshold(m2sigma, fp->T, k, x, w, nx, x_extent, b);
+    const double Ipi = lpa_threshold_tab(m2pi, fp->T, ta, tr, ts, nx);
+    const double Isig = lpa_threshold_tab(m2sigma, fp->T, ta, tr, ts, nx);
     dudt[i] = 0.5 * ((fp->N - 1.0) * up * Ipi + (3.0 * up + 2.0 * rho * upp) * Isig);
   }
+  free(tab);
 }
 
 int orc_flow_lpa(const orc_flow_params *fp, const double *x, const double *w, int nx, double x_extent,
@@ -736,4 +761,13 @@ int orc_flow_lpa(const orc_flow_params *fp, const double *x, const double *w, in
   }
   free(k1); free(k2); free(k3); free(k4); free(tmp);
   return rc;
+}
+
+/* Many flows, one flow per thread (OpenMP over flows; the right-hand side of each flow then runs
+ * on its thread alone).  u_out[f] receives flow f's n_grid values; rc[f] its return code. */
+void orc_flow_lpa_many(const orc_flow_params *fps, int n_flows, const double *x, const double *w, int nx,
+                       double x_extent, double b, double k_end, int n_steps, double **u_out, int *rc)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int f = 0; f < n_flows; ++f) rc[f] = orc_flow_lpa(&fps[f], x, w, nx, x_extent, b, k_end, n_steps, u_out[f]);
 }

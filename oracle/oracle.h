@@ -139,6 +139,9 @@ typedef struct {
 int orc_flow_lpa(const orc_flow_params *fp, const double *x, const double *w, int nx, double x_extent,
                  double b, double k_end, int n_steps, double *u_out);
 /* One RHS evaluation (exposed for tests). */
+/* Many flows at once, OpenMP over the flows. */
+void orc_flow_lpa_many(const orc_flow_params *fps, int n_flows, const double *x, const double *w, int nx,
+                       double x_extent, double b, double k_end, int n_steps, double **u_out, int *rc);
 void orc_flow_rhs(const orc_flow_params *fp, const double *u, double k, const double *x, const double *w,
                   int nx, double x_extent, double b, double *dudt);
 

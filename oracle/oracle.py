@@ -105,6 +105,8 @@ def lib():
                                    C.c_double, C.c_int, _dp]
         L.orc_flow_rhs.argtypes = [C.POINTER(FlowParams), _dp, C.c_double, _dp, _dp, C.c_int, C.c_double,
                                    C.c_double, _dp]
+        L.orc_flow_lpa_many.argtypes = [C.POINTER(FlowParams), C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, C.POINTER(_dp), C.POINTER(C.c_int)]
         L.orc_num_threads.restype = C.c_int
     return _lib
 
@@ -322,6 +324,20 @@ def flow_lpa(fp: FlowParams, x, w, x_extent=20.0, b=1.0, k_end=1e-3, n_steps=200
     if rc != 0:
         raise FloatingPointError("flow diverged")
     return u
+
+
+def flow_lpa_many(fps, x, w, x_extent=20.0, b=1.0, k_end=1e-3, n_steps=2000):
+    """Final u of several flows, one flow per OpenMP thread."""
+    x, w = _f64(x), _f64(w)
+    n = len(fps)
+    arr = (FlowParams * n)(*fps)
+    outs = [np.empty(fp.n_grid) for fp in fps]
+    ptrs = (_dp * n)(*[_p(u) for u in outs])
+    rc = (C.c_int * n)()
+    lib().orc_flow_lpa_many(arr, n, _p(x), _p(w), len(x), x_extent, b, k_end, n_steps, ptrs, rc)
+    if any(rc):
+        raise FloatingPointError("flow diverged")
+    return outs
 
 
 def num_threads() -> int:

@@ -33,6 +33,7 @@
 #include <DiFfRG/physics/interpolation/tex_linear_interpolation_2D.hh>
 #include <DiFfRG/physics/interpolation/tex_linear_interpolation_3D.hh>
 #include <DiFfRG/physics/regulators.hh>
+#include <DiFfRG/physics/thermodynamics.hh>
 #include <DiFfRG/physics/utils.hh>
 
 // A minimal forward-mode dual number with the interface autodiff::real offers (value, derivative,
@@ -780,10 +781,11 @@ int kat_request_overlap(int nq, int na, double x_extent, double k, int n_p, cons
   });
 }
 
-// The self-energy call set through request_data (log grid of n_p momenta on [1e-3, 40], bias 6):
-// results, and the median over reps of the device time of one batch (HIP events on the thread's
-// stream around the request loop and the reads: tuples to HBM, the kernel, results back) and of
-// the host time.  fp32 != 0 runs the FP32 form of the integrand (NT = float, FP64 accumulation).
+// The self-energy call set as request_data makes it (one request per momentum of a log grid of n_p
+// on [1e-3, 40], bias 6, then the reads): results, and the medians over reps of the device time of
+// one batch -- HIP events on the thread's stream around its launch and the reads: tuples to HBM,
+// the kernel, results back -- and of the host time of the whole call set (requests, launch,
+// reads).  fp32 != 0 runs the FP32 form of the integrand (NT = float, FP64 accumulation).
 int kat_selfenergy_request_bench(int nq, int na, double x_extent, double k, int n_p, const double *Z, int n_tab,
                                  double start, double stop, double bias, int fp32, int reps, double *out,
                                  double *p_out, double *t_dev, double *t_host)
@@ -792,6 +794,7 @@ int kat_selfenergy_request_bench(int nq, int na, double x_extent, double k, int 
     LogarithmicCoordinates1D<double> pgrid((unsigned)n_p, 1e-3, 40.0, 6.0);
     const auto grid = make_grid(pgrid);
     for (int i = 0; i < n_p; ++i) p_out[i] = grid[i];
+    provider(); // initialises the library on device 0
     void *sv = nullptr;
     dfrg::check(dfrg_thread_stream(&sv), "thread stream");
     hipEvent_t e0, e1;
@@ -799,11 +802,16 @@ int kat_selfenergy_request_bench(int nq, int na, double x_extent, double k, int 
       throw std::runtime_error("event create failed");
     std::vector<double> dev, host;
     auto run = [&](auto &w, auto &grid_c, auto kk, auto &interp, auto *res) {
+      using R = std::remove_pointer_t<decltype(res)>;
+      const auto g = make_grid(grid_c);
       for (int r = 0; r <= reps; ++r) {
         auto t0 = std::chrono::steady_clock::now();
+        // the requests of request_data's loop; the batch is launched between the events
+        std::vector<std::future<R>> fut;
+        for (unsigned i = 0; i < g.size(); ++i) fut.push_back(w.template request<R>(kk, g[i], interp));
         hipEventRecord(e0, (hipStream_t)sv);
-        auto fut = request_data<std::remove_pointer_t<decltype(res)>>(w, grid_c, kk, interp);
-        update_data(fut, res);
+        dfrg::tmpl::flush_thread();
+        for (unsigned i = 0; i < g.size(); ++i) res[i] = fut[i].get();
         hipEventRecord(e1, (hipStream_t)sv);
         hipEventSynchronize(e1);
         auto t1 = std::chrono::steady_clock::now();
@@ -1373,6 +1381,201 @@ int kat_radial_float(int nx, float x_extent, float k, const double *c, float con
     const float cf[6] = {(float)c[0], (float)c[1], (float)c[2], (float)c[3], (float)c[4], (float)c[5]};
     *out_get = integ.get(k, constant, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5]);
     *out_req = integ.request(k, constant, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5]).get();
+  });
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// regulators and thermodynamic helpers (DiFfRG/physics/regulators.hh, thermodynamics.hh),
+// evaluated on the device and on the host, and regulators inside a user KERNEL
+// ---------------------------------------------------------------------------------------------
+namespace
+{
+  struct BosB1 {
+    static constexpr double b = 1.;
+  };
+  struct PolyOrder1 {
+    static constexpr int order = 1;
+  };
+  struct PolyOrder60 {
+    static constexpr int order = 60;
+  };
+  struct SmoothAlpha1e4 {
+    static constexpr double alpha = 1e-4;
+  };
+
+  // RB, RBdot, dq2RB, RF, RFdot, dq2RF (NaN where the regulator does not define the function)
+  template <class R> __host__ __device__ void reg_eval(double k2, double q2, double *o)
+  {
+    const double nan = __builtin_nan("");
+    o[0] = R::RB(k2, q2);
+    o[1] = R::RBdot(k2, q2);
+    if constexpr (requires { R::dq2RB(k2, q2) + 0.; })
+      o[2] = R::dq2RB(k2, q2);
+    else
+      o[2] = nan;
+    o[3] = R::RF(k2, q2);
+    o[4] = R::RFdot(k2, q2);
+    if constexpr (requires { R::dq2RF(k2, q2) + 0.; })
+      o[5] = R::dq2RF(k2, q2);
+    else
+      o[5] = nan;
+  }
+  template <class R> __global__ void reg_kernel(const double *k2, const double *q2, int n, double *out)
+  {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) reg_eval<R>(k2[i], q2[i], out + 6 * i);
+  }
+  template <class R> void reg_run(int n, const double *k2, const double *q2, double *out_dev, double *out_host)
+  {
+    for (int i = 0; i < n; ++i) reg_eval<R>(k2[i], q2[i], out_host + 6 * i);
+    void *d = nullptr, *s = nullptr;
+    dfrg::check(dfrg_malloc(&d, sizeof(double) * 8 * n), "reg buffers");
+    double *dk = (double *)d, *dq = dk + n, *dout = dq + n;
+    dfrg::check(dfrg_thread_stream(&s), "thread stream");
+    dfrg::check(dfrg_memcpy_h2d(dk, k2, sizeof(double) * n, s), "k2");
+    dfrg::check(dfrg_memcpy_h2d(dq, q2, sizeof(double) * n, s), "q2");
+    hipLaunchKernelGGL(reg_kernel<R>, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)s, dk, dq, n, dout);
+    dfrg::check_hip("reg_kernel");
+    dfrg::check(dfrg_memcpy_d2h(out_dev, dout, sizeof(double) * 6 * n, s), "out");
+    dfrg::check(dfrg_stream_sync(s), "out");
+    dfrg_free(d);
+  }
+
+  // the LPA threshold integrand of a regulator, RBdot / (q^2 + RB + m2)^2
+  template <class R> struct ThresholdOf {
+    static __forceinline__ __host__ __device__ double kernel(const double q, const double k, const double m2)
+    {
+      const double k2 = powr<2>(k), q2 = powr<2>(q);
+      return R::RBdot(k2, q2) / powr<2>(q2 + R::RB(k2, q2) + m2);
+    }
+    static __forceinline__ __host__ __device__ double constant(const double, const double) { return 0.; }
+  };
+  template <class R> void reg_threshold(int nx, double x_extent, double k, const double *m2, int n, double *gpu,
+                                        double *host)
+  {
+    IntegratorGPU<4, double, ThresholdOf<R>> g(provider(), {{(unsigned)nx}}, x_extent);
+    IntegratorTBB<4, double, ThresholdOf<R>> h(provider(), {{(unsigned)nx}}, x_extent);
+    std::vector<std::future<double>> f;
+    for (int i = 0; i < n; ++i) f.push_back(g.request(k, m2[i]));
+    for (int i = 0; i < n; ++i) {
+      gpu[i] = f[i].get();
+      host[i] = h.get(k, m2[i]);
+    }
+  }
+
+  template <class F> int reg_dispatch(int kind, F &&f)
+  {
+    switch (kind) {
+    case 0: f.template operator()<LitimRegulator<>>(); return 0;
+    case 1: f.template operator()<BosonicRegulator<>>(); return 0;
+    case 2: f.template operator()<BosonicRegulator<BosB1>>(); return 0;
+    case 3: f.template operator()<ExponentialRegulator<B1>>(); return 0;
+    case 4: f.template operator()<ExponentialRegulator<>>(); return 0;
+    case 5: f.template operator()<SmoothedLitimRegulator<>>(); return 0;
+    case 6: f.template operator()<SmoothedLitimRegulator<SmoothAlpha1e4>>(); return 0;
+    case 7: f.template operator()<PolynomialExpRegulator<>>(); return 0;
+    case 8: f.template operator()<PolynomialExpRegulator<PolyOrder1>>(); return 0;
+    case 9: f.template operator()<PolynomialExpRegulator<PolyOrder60>>(); return 0;
+    }
+    throw std::invalid_argument("unknown regulator kind");
+  }
+
+  constexpr int kThermo = 18;
+  __host__ __device__ void thermo_eval(double x, double T, double *o)
+  {
+    o[0] = CothFiniteT(x, T);
+    o[1] = TanhFiniteT(x, T);
+    o[2] = SechFiniteT(x, T);
+    o[3] = CschFiniteT(x, T);
+    o[4] = cothS(x, T);
+    o[5] = dcothS(x, T);
+    o[6] = ddcothS(x, T);
+    o[7] = dddcothS(x, T);
+    o[8] = tanhS(x, T);
+    o[9] = dtanhS(x, T);
+    o[10] = ddtanhS(x, T);
+    o[11] = sechS(x, T);
+    o[12] = cschS(x, T);
+    o[13] = nB(x, T);
+    o[14] = dnB(x, T);
+    o[15] = ddnB(x, T);
+    o[16] = nF(x, T);
+    o[17] = dnF(x, T);
+  }
+  __global__ void thermo_kernel(const double *x, const double *T, int n, double *out)
+  {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) thermo_eval(x[i], T[i], out + kThermo * i);
+  }
+
+  // the bosonic finite-T loop kernel of the reference's O(N) example in its own form
+  // (Examples/ONfiniteT/flows/V/V.kernel:4-15 with a single field): CothFiniteT(E, T)/E RBdot
+  template <class R> struct CothKernel {
+    static __forceinline__ __host__ __device__ double kernel(const double q, const double k, const double T,
+                                                             const double m2)
+    {
+      const double k2 = powr<2>(k), q2 = powr<2>(q);
+      const double E = sqrt(q2 + R::RB(k2, q2) + m2);
+      return CothFiniteT(E, T) / E * R::RBdot(k2, q2);
+    }
+    static __forceinline__ __host__ __device__ double constant(const double, const double, const double)
+    {
+      return 0.;
+    }
+  };
+} // namespace
+
+extern "C" {
+
+int kat_num_thermo(void) { return kThermo; }
+
+int kat_regulator_eval(int kind, int n, const double *k2, const double *q2, double *out_dev, double *out_host)
+{
+  return guard([&] {
+    reg_dispatch(kind, [&]<class R>() { reg_run<R>(n, k2, q2, out_dev, out_host); });
+  });
+}
+
+int kat_regulator_threshold(int kind, int nx, double x_extent, double k, const double *m2, int n, double *gpu,
+                            double *host)
+{
+  return guard([&] {
+    reg_dispatch(kind, [&]<class R>() { reg_threshold<R>(nx, x_extent, k, m2, n, gpu, host); });
+  });
+}
+
+int kat_thermo_eval(int n, const double *x, const double *T, double *out_dev, double *out_host)
+{
+  return guard([&] {
+    for (int i = 0; i < n; ++i) thermo_eval(x[i], T[i], out_host + kThermo * i);
+    void *d = nullptr, *s = nullptr;
+    dfrg::check(dfrg_malloc(&d, sizeof(double) * (2 + kThermo) * n), "thermo buffers");
+    double *dx = (double *)d, *dT = dx + n, *dout = dT + n;
+    dfrg::check(dfrg_thread_stream(&s), "thread stream");
+    dfrg::check(dfrg_memcpy_h2d(dx, x, sizeof(double) * n, s), "x");
+    dfrg::check(dfrg_memcpy_h2d(dT, T, sizeof(double) * n, s), "T");
+    hipLaunchKernelGGL(thermo_kernel, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)s, dx, dT, n, dout);
+    dfrg::check_hip("thermo_kernel");
+    dfrg::check(dfrg_memcpy_d2h(out_dev, dout, sizeof(double) * kThermo * n, s), "out");
+    dfrg::check(dfrg_stream_sync(s), "out");
+    dfrg_free(d);
+  });
+}
+
+// sum_j over the radial GL rule of CothFiniteT(E, T)/E RBdot for the exponential regulator (b = 1),
+// 3-dimensional (IntegratorGPU<3>), on the GPU and with the host template, for n (T, m2) pairs
+int kat_coth_threshold(int nx, double x_extent, double k, const double *T, const double *m2, int n, double *gpu,
+                       double *host)
+{
+  return guard([&] {
+    IntegratorGPU<3, double, CothKernel<Reg>> g(provider(), {{(unsigned)nx}}, x_extent);
+    IntegratorTBB<3, double, CothKernel<Reg>> h(provider(), {{(unsigned)nx}}, x_extent);
+    for (int i = 0; i < n; ++i) {
+      gpu[i] = g.get(k, T[i], m2[i]);
+      host[i] = h.get(k, T[i], m2[i]);
+    }
   });
 }
 

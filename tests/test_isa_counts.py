@@ -51,3 +51,31 @@ def test_se_loop_choice():
     assert bench.se_loop(32, 256) == "pairs" and bench.se_loop(32, 512) == "pairs"
     assert bench.se_loop(32, 1024) == "single" and bench.se_loop(16, 256) == "single"
     assert bench.se_loop(64, 1024) == "pairs"
+
+
+def _kernel_body(asm_text, symbol_prefix):
+    import re
+    m = re.search(r"^(" + symbol_prefix + r"\w*):(.*?)s_endpgm", asm_text, re.M | re.S)
+    assert m, symbol_prefix
+    return [l.strip() for l in m.group(2).split("\n")]
+
+
+@pytest.mark.skipif(shutil.which(I.B.HIPCC) is None and not os.path.exists(I.B.HIPCC), reason="hipcc not found")
+def test_cross_block_publication_lowering():
+    """The cross-block finish (dfrg::dev::publish / wait_stores / take_ticket / fetch) uses relaxed
+    agent-scope atomics, not release/acquire; its correctness rests on how gfx950 lowers them:
+    the partial's store and the finishing block's load carry sc1 (device scope: written through
+    to, and read from, the level where the XCDs are coherent), the store is complete (vmcnt
+    counts stores on gfx9) before the ticket atomic is issued, and the ticket is an atomic with a
+    returned value (sc0).  This test fails if a compiler or flag change alters that lowering."""
+    with tempfile.TemporaryDirectory() as d:
+        text = open(I.compile_asm(os.path.join(ROOT, "diffrg_amd", "csrc", "integrals.hip"), d)).read()
+    for sym in ("_ZN12_GLOBAL__N_19se_kernelILi1ELi512ELi2E", "_ZN12_GLOBAL__N_19se_kernelILi0ELi256ELi1E"):
+        body = _kernel_body(text, sym)
+        stores = [k for k, l in enumerate(body) if l.startswith("global_store_dwordx2") and l.endswith(" sc1")]
+        assert stores, (sym, "no device-scope store of a partial")
+        i = stores[0]
+        j = next(k for k in range(i + 1, len(body)) if body[k].startswith("global_atomic_add"))
+        assert "s_waitcnt vmcnt(0)" in body[i + 1:j], (sym, body[i:j + 1])
+        assert body[j].endswith(" sc0"), body[j]  # returns the pre-increment ticket
+        assert any(l.startswith("global_load_dwordx2") and l.endswith(" sc1") for l in body[j:]), sym
