@@ -99,9 +99,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--flows", type=int, default=1000, help="flows in the steady-state serving run")
-    ap.add_argument("--flow-check-frac", type=float, default=0.0,
-                    help="fraction of the served flows checked against the CPU reference (one flow per CPU "
-                         "thread); 0: six flows spanning the grid sizes")
+    ap.add_argument("--flow-check-frac", type=float, default=0.1,
+                    help="fraction of the served flows checked against the CPU reference (stratified over the "
+                         "grid sizes, one flow per CPU thread); 0: six flows spanning the grid sizes")
     ap.add_argument("--workloads", default="template,threshold,finite_T,uneven,flows",
                     help="comma-separated workloads measured after the headline")
     ap.add_argument("--shape", action="append", default=[], metavar="MODE:BLOCK,TILE,GRID",
@@ -489,6 +489,21 @@ def _flow_sample(reqs, m, seed):
     return sorted(order[int(rng.integers(edges[j], max(edges[j] + 1, edges[j + 1])))] for j in range(m))
 
 
+def flow_step_roofline(tm, nx=64):
+    """flow_step_kernel over the steady-state run, from the server's HIP events around each graph
+    launch (25 step launches and the base-step node per graph, so the gaps between the launches
+    inside a graph count as kernel time): frozen FLOP per node of both threshold integrals x 64
+    nodes x 4 RK stages per grid-point step."""
+    wt = FROZEN["C4_flow_node"]["fp64"]
+    node_evals = tm["point_steps"] * 4 * nx
+    achieved = wt["flop"] * node_evals / tm["device_s"] if tm["device_s"] > 0 else 0.0
+    return {"graph_launches": tm["launches"], "device_s": tm["device_s"], "point_steps": tm["point_steps"],
+            "node_evals": node_evals, "flop_per_node": wt["flop"], "achieved_tflops": achieved / 1e12,
+            "roofline_frac": achieved / PEAK_FP64_VALU, "ceiling": ceiling(wt),
+            "valu_issue_frac": valu_issue_frac(wt["valu"], node_evals, tm["device_s"]),
+            "timing": "HIP events around each graph launch on the server's stream"}
+
+
 def flows(D, qp, args):
     """Flow serving: saturation throughput (flows submitted at once), then a steady-state run of
     Poisson arrivals at 0.7x saturation, with served/offered rate and p50/p99 latency over the
@@ -528,6 +543,7 @@ def flows(D, qp, args):
         if r == 0 and qd == 0 and pending:
             time.sleep(max(0.0, min(0.002, reqs[pending[0]]["arrival"] - (time.perf_counter() - t0))))
     elapsed = time.perf_counter() - t0
+    tm = fs.timing()
     res = [fs.result(fid) for fid in ids]
     lat = np.array([l for _, l in res])
     arr = np.array([q["arrival"] for q in reqs])
@@ -559,6 +575,7 @@ def flows(D, qp, args):
                              "p50_latency_s": float(np.quantile(lat[in_win], 0.5)),
                              "p99_latency_s": float(np.quantile(lat[in_win], 0.99))},
             "p50_latency_s_all": float(np.quantile(lat, 0.5)), "p99_latency_s_all": float(np.quantile(lat, 0.99)),
+            "step_kernel": flow_step_roofline(tm),
             "cpu_check": {"flows": len(sample), "fraction": len(sample) / n_flows,
                           "n_grid": [reqs[i]["n_grid"] for i in sample], "max_abs_u_diff": max(diffs),
                           "tolerance": 1e-8, "within_tolerance": max(diffs) <= 1e-8, "cpu_seconds": cpu_s,

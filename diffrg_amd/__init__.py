@@ -101,6 +101,7 @@ SIGNATURES = {
     "dfrg_fs_poll": [_vp, C.c_longlong, _ip],
     "dfrg_fs_result": [_vp, C.c_longlong, _dp, C.c_int, _dp],
     "dfrg_fs_stats": [_vp, _ip, _dp, _dp, _dp],
+    "dfrg_fs_timing": [_vp, C.POINTER(C.c_longlong), _dp, C.POINTER(C.c_longlong)],
     "dfrg_table_create": [C.POINTER(_vp), C.c_int, C.c_double, C.c_double, C.c_double, _dp],
     "dfrg_table_update": [_vp, _dp, _vp],
     "dfrg_table_view": [_vp, C.POINTER(_vp), _ip],
@@ -556,6 +557,12 @@ class FlowServer:
         f, p50, p99, busy = C.c_int(), C.c_double(), C.c_double(), C.c_double()
         check(lib().dfrg_fs_stats(self.ptr, C.byref(f), C.byref(p50), C.byref(p99), C.byref(busy)))
         return {"finished": f.value, "p50_s": p50.value, "p99_s": p99.value, "busy_s": busy.value}
+
+    def timing(self) -> dict:
+        """Graph launches, their device time (HIP events) and the grid-point steps they advanced."""
+        n, ps, t = C.c_longlong(), C.c_longlong(), C.c_double()
+        check(lib().dfrg_fs_timing(self.ptr, C.byref(n), C.byref(t), C.byref(ps)))
+        return {"launches": n.value, "device_s": t.value, "point_steps": ps.value}
 
     def close(self):
         if self.ptr:

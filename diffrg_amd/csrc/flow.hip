@@ -265,6 +265,11 @@ struct dfrg_fs {
   std::vector<double> latencies;
   double busy_s = 0;
   int finished = 0;
+  // graph launches: HIP events around each on the server's stream, their summed device time, and
+  // the grid-point RK4 steps they advanced
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  long long launches = 0, point_steps = 0;
+  double device_s = 0;
 };
 
 namespace
@@ -439,6 +444,15 @@ extern "C" {
 
 size_t dfrg_fs_flow_bytes(int n_grid) { return (size_t)2 * n_grid * sizeof(double); }
 
+int dfrg_fs_timing(dfrg_fs *fs, long long *launches, double *device_s, long long *point_steps)
+{
+  DFRG_CHECK_ARG(fs != nullptr, "NULL server");
+  if (launches) *launches = fs->launches;
+  if (device_s) *device_s = fs->device_s;
+  if (point_steps) *point_steps = fs->point_steps;
+  return DFRG_OK;
+}
+
 int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_resident, int nx, int n_steps,
                    double k_end, int steps_per_launch)
 {
@@ -516,7 +530,8 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
     dfrg_fs_destroy(fs);
     return DFRG_ERR_HIP;
   }
-  if (hipStreamCreateWithFlags(&fs->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&fs->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&fs->ev0) != hipSuccess || hipEventCreate(&fs->ev1) != hipSuccess) {
     set_error("flow server: stream creation failed");
     dfrg_fs_destroy(fs);
     return DFRG_ERR_HIP;
@@ -532,6 +547,8 @@ int dfrg_fs_destroy(dfrg_fs *fs)
   hipDeviceSynchronize();
   if (fs->graph_exec) hipGraphExecDestroy(fs->graph_exec);
   if (fs->graph) hipGraphDestroy(fs->graph);
+  if (fs->ev0) hipEventDestroy(fs->ev0);
+  if (fs->ev1) hipEventDestroy(fs->ev1);
   if (fs->stream) hipStreamDestroy(fs->stream);
   FlowDev &d = fs->dev;
   for (void *p : {(void *)d.status, (void *)d.bad, (void *)d.h,
@@ -568,11 +585,25 @@ int dfrg_fs_progress(dfrg_fs *fs, int *resident, int *queued, int *done)
   DFRG_TRY(fs_admit(fs));
   if (resident_count(fs) > 0) {
     DFRG_TRY(fs_capture(fs));
+    // grid-point steps this launch advances: each resident flow takes the steps it has left, up to
+    // steps_per_launch
+    for (long long id : fs->slot_flow)
+      if (id >= 0) {
+        const FlowRecord &r = fs->flows[id];
+        const int left = fs->n_steps - (fs->base - r.admit);
+        fs->point_steps += (long long)r.desc.n_grid * std::max(0, std::min(left, fs->steps_per_launch));
+      }
     const double t0 = now_s();
+    DFRG_HIP(hipEventRecord(fs->ev0, fs->stream));
     DFRG_HIP(hipGraphLaunch(fs->graph_exec, fs->stream));
+    DFRG_HIP(hipEventRecord(fs->ev1, fs->stream));
     DFRG_HIP(hipStreamSynchronize(fs->stream));
     fs->base += fs->steps_per_launch;
     fs->busy_s += now_s() - t0;
+    float ms = 0.f;
+    DFRG_HIP(hipEventElapsedTime(&ms, fs->ev0, fs->ev1));
+    fs->device_s += 1e-3 * ms;
+    ++fs->launches;
     DFRG_TRY(fs_retire(fs));
     DFRG_TRY(fs_admit(fs));
   }

@@ -258,6 +258,10 @@ int dfrg_fs_result(dfrg_fs *fs, long long id, double *u_out, int n, double *late
 int dfrg_fs_stats(dfrg_fs *fs, int *finished, double *p50_s, double *p99_s, double *busy_s);
 /* Bytes of HBM one flow of n_grid points occupies while resident. */
 size_t dfrg_fs_flow_bytes(int n_grid);
+/* Graph launches so far, their summed device time in seconds (HIP events around each launch on
+ * the server's stream) and the grid-point RK4 steps they advanced (each resident flow counts the
+ * steps it had left, up to steps_per_launch).  No reference counterpart (measurement). */
+int dfrg_fs_timing(dfrg_fs *fs, long long *launches, double *device_s, long long *point_steps);
 
 #ifdef __cplusplus
 }

@@ -19,14 +19,14 @@
 // the KERNEL in FP32 on FP32 nodes and accumulate in FP64, the mixed form of the template path.
 //
 // get() is a batch of one, launched at once and waited for.  request() adds to the integrator's
-// open batch; a batch is launched (asynchronously: tuples to HBM, one kernel, results back into
-// pinned memory, an event) when any future of this thread is read, when request_data() finishes its
+// open batch; a batch is launched (asynchronously: tuples to HBM, one kernel whose finishing blocks
+// store the results into pinned host memory, an event) when any future of this thread is read, when request_data() finishes its
 // loop, when the batch is full, or when a request needs another launch configuration.  Reading a future launches every open
 // batch of the calling thread first, so batches of different integrators run back to back before
 // the first wait.
 //
 // Scratch: every host thread owns an arena in HBM on its stream (ticket counters, partial slots,
-// argument tuples, results).  Counters are zeroed when a thread takes an arena (new, grown, or left
+// argument tuples).  Counters are zeroed when a thread takes an arena (new, grown, or left
 // by a thread that ended) and every launch's finishing blocks reset the counters they used, so
 // consecutive launches on the stream share it without clearing.
 #pragma once
@@ -740,7 +740,8 @@ namespace dfrg
       }
 
     private:
-      // Enqueue on this thread's stream: tuples to HBM, the kernel, results back; record the event.
+      // Enqueue on this thread's stream: tuples to HBM, the kernel (results into the pinned
+      // staging), the event.
       void start()
       {
         launched = true;
@@ -748,8 +749,7 @@ namespace dfrg
         try {
           const Plan &p = plan;
           const size_t o_args = 0, o_part = up256(sizeof(Args) * n);
-          const size_t o_res = o_part + up256(sizeof(double) * dev::kSlotStride * (size_t)n * (p.upr > 1 ? p.upr : 0));
-          const size_t bytes = o_res + up256(sizeof(NT) * n);
+          const size_t bytes = o_part + up256(sizeof(double) * dev::kSlotStride * (size_t)n * (p.upr > 1 ? p.upr : 0));
           Arena &A = thread_arena();
           A.ensure((size_t)n, bytes);
           const hipStream_t s = A.stream();
@@ -757,12 +757,13 @@ namespace dfrg
           check(dfrg_pool_host_alloc(&h, h_res + sizeof(NT) * n), "batch staging");
           std::memcpy(h, (const void *)args.data(), sizeof(Args) * n);
           char *dd = A.data();
-          bool ok = hipMemcpyAsync(dd + o_args, h, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess;
+          void *dh = nullptr; // the staging's device address: the finishing blocks store the results there
+          bool ok = hipHostGetDevicePointer(&dh, h, 0) == hipSuccess &&
+                    hipMemcpyAsync(dd + o_args, h, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess;
           if (ok) {
-            launch((const Args *)(dd + o_args), n, p, (double *)(dd + o_part), A.counters(), (NT *)(dd + o_res), s);
+            launch((const Args *)(dd + o_args), n, p, (double *)(dd + o_part), A.counters(),
+                   (NT *)((char *)dh + h_res), s);
             ok = hipGetLastError() == hipSuccess &&
-                 hipMemcpyAsync((char *)h + h_res, dd + o_res, sizeof(NT) * n, hipMemcpyDeviceToHost, s) ==
-                     hipSuccess &&
                  hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
                  hipEventRecord(ev, s) == hipSuccess;
           }
@@ -835,18 +836,23 @@ namespace dfrg
       const NT constant = KERNEL::constant(k, t...);
       Arena &A = thread_arena();
       const size_t o_part = up256(sizeof(Args));
-      const size_t o_res = o_part + up256(sizeof(double) * dev::kSlotStride * (p.upr > 1 ? p.upr : 0));
-      A.ensure(1, o_res + up256(sizeof(NT)));
+      A.ensure(1, o_part + up256(sizeof(double) * dev::kSlotStride * (p.upr > 1 ? p.upr : 0)));
       const hipStream_t s = A.stream();
-      // pinned staging of this thread: the tuple, then the result
-      thread_local void *h = nullptr;
+      // pinned staging of this thread: the tuple, then the result, which the finishing block
+      // stores through the staging's device address
+      thread_local void *h = nullptr, *dh = nullptr;
       thread_local size_t h_size = 0;
       const size_t h_res = up256(sizeof(Args)), need = h_res + sizeof(NT);
       if (h_size < need) {
         if (h) dfrg_pool_host_free(h);
-        h = nullptr;
+        h = dh = nullptr;
         h_size = 0;
         check(dfrg_pool_host_alloc(&h, need), "get() staging");
+        if (hipHostGetDevicePointer(&dh, h, 0) != hipSuccess) {
+          dfrg_pool_host_free(h);
+          h = dh = nullptr;
+          throw std::runtime_error("integrator: staging not mapped");
+        }
         h_size = need;
       }
       const Args args(k, t...);
@@ -855,11 +861,9 @@ namespace dfrg
       if (hipMemcpyAsync(dd, h, sizeof(Args), hipMemcpyHostToDevice, s) != hipSuccess)
         throw std::runtime_error("integrator: argument copy failed");
       launch_batch<S, NT, KERNEL>(n, p, (const Args *)dd, 1, (double *)(dd + o_part), A.counters(),
-                                  (NT *)(dd + o_res), s);
+                                  (NT *)((char *)dh + h_res), s);
       check_hip("integrator kernel launch");
-      if (hipMemcpyAsync((char *)h + h_res, dd + o_res, sizeof(NT), hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
-        throw std::runtime_error("integrator: stream error");
+      if (hipStreamSynchronize(s) != hipSuccess) throw std::runtime_error("integrator: stream error");
       NT v;
       std::memcpy((void *)&v, (char *)h + h_res, sizeof(NT));
       return constant + v;

@@ -1006,7 +1006,8 @@ extern "C" {
 
 // kind: 0 LinearInterpolator2D<double>, 1 LinearInterpolator3D<double>, 2 TexLinearInterpolator1D
 // (log coordinates), 3 TexLinearInterpolator2D, 4 TexLinearInterpolator3D, 5 TexLinearInterpolator1DStack
-// (bosonic), 6 TexLinearInterpolator1DStack (fermionic), 7 LinearInterpolator2D<float>.
+// (bosonic), 6 TexLinearInterpolator1DStack (fermionic), 7 LinearInterpolator2D<float>,
+// 8 LinearInterpolator1D<double> (FP64 log coordinates).
 // cp: linear axes (n, start, stop) per axis; log (n, start, stop, bias); stacks (m_start, m_stop, T,
 // n, start, stop, bias).  pts: n_pts points of the interpolator's dimension.
 int kat_interp(int kind, const double *cp, const double *data, int n_data, const double *pts, int n_pts,
@@ -1079,6 +1080,13 @@ int kat_interp(int kind, const double *cp, const double *data, int n_data, const
       LinearInterpolator2D<float, decltype(c)> I(std::vector<float>(c.size(), 0.f), c);
       I.update(data);
       interp_eval<2>(I, pts, n_pts, out_dev, out_host);
+      break;
+    }
+    case 8: {
+      LogarithmicCoordinates1D<double> c((unsigned)cp[0], cp[1], cp[2], cp[3]);
+      need(c.size());
+      LinearInterpolator1D<double, decltype(c)> I(data, c);
+      interp_eval<1>(I, pts, n_pts, out_dev, out_host);
       break;
     }
     default: throw std::invalid_argument("unknown interpolator kind");

@@ -221,6 +221,46 @@ def test_dual_number_channels(kat):
     assert np.array_equal(dev[:, 1], run(kat, kp, cp, der, pts)[0])
 
 
+def log_index64(x, n, start, stop, bias):
+    """The reference's FP64 index log1p((x - c)/b) (n-1)/a clamped to [0, n-1], NaN -> n-1."""
+    b = (stop - start) / math.expm1(bias)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        idx = np.log1p((x - start) / b) * (n - 1.0) / bias
+    return np.clip(np.nan_to_num(idx, nan=n - 1.0, posinf=n - 1.0, neginf=0.0), 0, n - 1)
+
+
+@pytest.mark.parametrize("n,bias", [(128, 6.0), (256, 1.0), (32, 6.0), (2000, 3.0), (10, 6.0), (2, 0.2)])
+def test_fp64_log_coordinates_on_the_device(kat, n, bias):
+    """LinearInterpolator1D<double> on FP64 log coordinates: on the device the index comes from an
+    FP32 cell estimate and an FP64 in-cell series (grids with a/(n-1) <= 1/4), elsewhere from the
+    reference's log1p.  A table linear in the index pins the index itself; the points include grid
+    nodes, both sides of cell boundaries at 1e-7 and 1e-13, both ends, far below the start (where
+    log1p has no real value and the NaN index takes the last point), NaN and +inf."""
+    rng = np.random.default_rng(n)
+    start, stop = 1e-3, 40.0
+    cp = [n, start, stop, bias]
+    b = (stop - start) / math.expm1(bias)
+    nodes = b * np.expm1(bias * np.arange(n) / (n - 1.0)) + start
+    near = np.concatenate([nodes * (1 + e) for e in (1e-7, -1e-7, 1e-13, -1e-13)])
+    pts = np.concatenate([nodes, near, rng.uniform(start, stop, 400), np.exp(rng.uniform(np.log(start), np.log(stop), 400)),
+                          [0.0, start - 0.3 * b, start - 0.5 * b, start - 0.7 * b, start - 2 * b, stop, 1.5 * stop,
+                           1e300, np.inf, np.nan]])
+    # the index within the reference's own FP64 rounding: the input (x - c)/b carries an error of
+    # a few ulp, which (n-1)/a log1p turns into ~1e-16 (n-1)/a cells; the log's own rounding ~1e-16 n
+    tol_idx = 8 * 2.0 ** -52 * ((n - 1.0) / bias + n)
+    lin = np.arange(n, dtype=np.float64) * 0.75 - 3.0
+    dev, host = run(kat, 8, cp, lin, pts)
+    want = log_index64(pts, n, start, stop, bias) * 0.75 - 3.0
+    assert np.all(np.isfinite(dev)) and np.all(np.isfinite(host))
+    assert np.max(np.abs(dev - want)) <= 0.75 * tol_idx + 4 * 2.0 ** -52 * 3.0 * n
+    assert np.max(np.abs(host - want)) <= 0.75 * tol_idx + 4 * 2.0 ** -52 * 3.0 * n
+    # a smooth table: device against the host's reference formula, at the table's cell slopes
+    smooth = 1.0 + 0.5 * np.tanh(np.sin(np.arange(n) * 0.37) * 2.0)
+    dev, host = run(kat, 8, cp, smooth, pts)
+    slope = np.max(np.abs(np.diff(smooth))) if n > 1 else 0.0
+    assert np.max(np.abs(dev - host)) <= slope * tol_idx + 4 * 2.0 ** -52
+
+
 def test_dual_number_fp64_interpolator(kat):
     # LinearInterpolator1D over dual data (reference linear_interpolation_1D.cu:76-91 instantiates
     # it for autodiff::real): FP64 lerp of both channels.  Value data linear in the index is
