@@ -15,8 +15,9 @@
 // with the atanh series to z^17 (|z| <= 0.125 when a/(N-1) <= 1/4: truncation below 1e-17), and
 // the estimate's cell moved by one where the position falls outside [0, 1).  The index agrees with
 // the reference's to its own FP64 rounding (~1e-14 of a cell); the clamp to [0, N-1] and its NaN
-// rule (an index that is NaN takes the last point) are the reference's.  Coarser grids, points
-// with (x - c)/b <= -1/2 and NaN take the reference's formula.
+// rule (an index that is NaN takes the last point) are the reference's: points with (x - c)/b <= -1/2
+// (negative index, or NaN below -1) and NaN take the clamp's endpoint.  Coarser grids take the
+// reference's formula.
 #pragma once
 
 #include <cmath>
@@ -89,8 +90,7 @@ namespace DiFfRG
       using ct = typename Coordinates::ctype;
 #ifdef __HIP_DEVICE_COMPILE__
       if constexpr (kLogFP64) {
-        const double u = (x - lc_c) * lc_inv_b;
-        if (cell_ptr && u > -0.5) return log_cell_lerp(u);
+        if (cell_ptr) return log_cell_lerp((x - lc_c) * lc_inv_b);
       }
 #endif
       ct idx = coordinates.backward(x);
@@ -130,10 +130,14 @@ namespace DiFfRG
       lc_s_ln2 = float((size - 1.) / a * 0.69314718055994530942);
     }
 
-    // u = (x - c)/b > -1/2: the lerp at the index of u (device)
-    __device__ __forceinline__ NT log_cell_lerp(const double u) const
+    // the lerp at the index of u = (x - c)/b (device).  For u <= -1/2 the reference's index is
+    // negative (-1 <= u: the first point) or NaN (u < -1, NaN: the last point, its NaN clamp); those
+    // take the clamp's endpoint by selects, so the loop that calls this has no branch
+    __device__ __forceinline__ NT log_cell_lerp(const double u_in) const
     {
       const int last = (int)size - 2;
+      const bool inside = u_in > -0.5;
+      const double u = inside ? u_in : 0.;
       const float est = __builtin_amdgcn_logf(1.f + (float)u) * lc_s_ln2; // v_log_f32 is log2
       int i = (int)floorf(fmaxf(0.f, fminf(est, (float)last)));
       const double *cl = cell_ptr + 2 * i;
@@ -163,6 +167,11 @@ namespace DiFfRG
         f -= 1.;
       }
       f = fmax(0., fmin(f, 1.));
+      if (!inside) {
+        const bool first = u_in >= -1.; // false for NaN
+        i = first ? 0 : last;
+        f = first ? 0. : 1.;
+      }
       const NT *d = device_data_ptr + i;
       return d[0] * (1. - f) + d[1] * f;
     }

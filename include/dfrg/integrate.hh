@@ -400,9 +400,10 @@ namespace dfrg
 
     // The mixed-radix digits of a flat node index (slowest first), advanced by a fixed stride S with
     // carries instead of a division per node.
-    template <int N> struct NodeWalk {
+    // I: the index type (int where the flat index of a unit fits, as in row mode: 32-bit divisions)
+    template <int N, class I = long long> struct NodeWalk {
       int dig[N], step[N], rad[N];
-      __device__ __forceinline__ NodeWalk(long long i, long long S, const int *r)
+      __device__ __forceinline__ NodeWalk(I i, I S, const int *r)
       {
 #pragma unroll
         for (int j = N - 1; j >= 1; --j) {
@@ -434,7 +435,7 @@ namespace dfrg
     // ------------------------------------------------------------------------------------------
     constexpr int kMaxBlock = 256;
 
-    template <class S, class NT, class KERNEL, class C, class Args>
+    template <class S, class NT, class KERNEL, bool ROW, class C, class Args>
     __device__ __forceinline__ void unit_body(const Nodes<C> &n, const Plan &p, int r, int u, const Args &args,
                                               double *partials, unsigned *counters, NT *results)
     {
@@ -447,13 +448,13 @@ namespace dfrg
       int rad[S::ND];
       S::radix(n, rad);
       Acc<NT> acc;
-      if constexpr (S::ND > 1) {
-        if (p.row_mode) {
+      if constexpr (S::ND > 1 && ROW) {
+        {
           const int ix = u / p.parts, part = u - ix * p.parts;
           const int m0 = part * p.unit, m1 = m0 + p.unit < p.ni ? m0 + p.unit : p.ni;
           const C q = k * sqrt(n.xq[ix] * n.x_extent);
           const C rf = S::element(q, k) * (n.wq[ix] * n.x_extent);
-          auto eval = [&](const NodeWalk<S::ND - 1> &w) {
+          auto eval = [&](const NodeWalk<S::ND - 1, int> &w) {
             int g[S::ND];
             g[0] = ix;
 #pragma unroll
@@ -462,11 +463,11 @@ namespace dfrg
                 [&](const C &kk, const auto &...t) { return S::template term<NT, KERNEL>(n, q, rf, g, kk, t...); },
                 args);
           };
-          NodeWalk<S::ND - 1> w(m0 + tid, B, rad + 1);
+          NodeWalk<S::ND - 1, int> w(m0 + tid, B, rad + 1);
           for (int m = m0 + tid; m < m1; m += B, w.next()) acc.add(eval(w));
         }
       }
-      if (!p.row_mode) {
+      if constexpr (!ROW || S::ND == 1) {
         const int ix0 = u * p.rows, ix1 = ix0 + p.rows < p.nx ? ix0 + p.rows : p.nx;
         for (int j = tid; j < ix1 - ix0; j += B) {
           const C q = k * sqrt(n.xq[ix0 + j] * n.x_extent);
@@ -537,14 +538,15 @@ namespace dfrg
     }
 
     // a batch of integrals whose (k, t...) tuples sit in HBM; blocks stride over all units
-    template <class S, class NT, class KERNEL, class C, class Args>
+    // ROW: the plan's row mode (one instantiation per mode, so each has its own register budget)
+    template <class S, class NT, class KERNEL, bool ROW, class C, class Args>
     __global__ __launch_bounds__(kMaxBlock) void batch_kernel(
         Nodes<C> n, Plan p, const Args *args, int n_req, double *partials, unsigned *counters, NT *results)
     {
       const long long total = (long long)n_req * p.upr;
       for (long long g = blockIdx.x; g < total; g += gridDim.x) {
         const int r = (int)(g / p.upr), u = (int)(g - (long long)r * p.upr);
-        unit_body<S, NT, KERNEL>(n, p, r, u, args[r], partials, counters, results);
+        unit_body<S, NT, KERNEL, ROW>(n, p, r, u, args[r], partials, counters, results);
       }
     }
 
@@ -660,9 +662,17 @@ namespace dfrg
                       unsigned *counters, NT *results, hipStream_t s)
     {
       const long long total = (long long)n_req * p.upr;
-      constexpr auto kern = &batch_kernel<S, NT, KERNEL, C, Args>;
-      const int grid = (int)std::min<long long>(total, resident_cached<kern>(p.block));
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(p.block), 0, s, n, p, d_args, n_req, partials, counters, results);
+      auto go = [&](auto kern, int resident) {
+        const int grid = (int)std::min<long long>(total, resident);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(p.block), 0, s, n, p, d_args, n_req, partials, counters, results);
+      };
+      if (p.row_mode && S::ND > 1) {
+        constexpr auto kern = &batch_kernel<S, NT, KERNEL, (S::ND > 1), C, Args>;
+        go(kern, resident_cached<kern>(p.block));
+      } else {
+        constexpr auto kern = &batch_kernel<S, NT, KERNEL, false, C, Args>;
+        go(kern, resident_cached<kern>(p.block));
+      }
     }
 
     // ------------------------------------------------------------------------------------------
