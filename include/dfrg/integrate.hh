@@ -713,13 +713,17 @@ namespace dfrg
       bool launched = false, ready = false;
       std::exception_ptr error;
       hipEvent_t ev = nullptr;
-      void *h = nullptr; // pinned: tuples, then results
+      hipStream_t stream = nullptr; // where the batch was enqueued
+      void *h = nullptr;            // pinned: tuples, then results (stored by the kernel)
 
       ~RequestBatch() override
       {
+        // the staging returns to the pool only after the work that writes into it has finished
         if (ev) {
           hipEventSynchronize(ev);
           hipEventDestroy(ev);
+        } else if (stream) {
+          hipStreamSynchronize(stream);
         }
         if (h) dfrg_pool_host_free(h);
       }
@@ -768,14 +772,21 @@ namespace dfrg
           std::memcpy(h, (const void *)args.data(), sizeof(Args) * n);
           char *dd = A.data();
           void *dh = nullptr; // the staging's device address: the finishing blocks store the results there
+          stream = s;
           bool ok = hipHostGetDevicePointer(&dh, h, 0) == hipSuccess &&
                     hipMemcpyAsync(dd + o_args, h, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess;
           if (ok) {
             launch((const Args *)(dd + o_args), n, p, (double *)(dd + o_part), A.counters(),
                    (NT *)((char *)dh + h_res), s);
-            ok = hipGetLastError() == hipSuccess &&
-                 hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
-                 hipEventRecord(ev, s) == hipSuccess;
+            ok = hipGetLastError() == hipSuccess;
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+              ev = nullptr;
+              ok = false;
+            } else if (hipEventRecord(ev, s) != hipSuccess) { // the destructor then waits on the stream
+              hipEventDestroy(ev);
+              ev = nullptr;
+              ok = false;
+            }
           }
           if (!ok) throw std::runtime_error("integrator: batched request launch failed");
         } catch (...) {
