@@ -18,7 +18,7 @@ namespace DiFfRG
   {
     namespace internal
     {
-      // a dual number (DiFfRG/common/dual_traits.hh): the type of its value, as for
+      // a dual number (dfrg/dual.hh): the type of its value, as for
       // autodiff::Real<1, T> in the reference (types.hh:89-96)
       template <typename CT> struct _ctype {
         static_assert(dfrg::DualTraits<CT>::is_dual, "ctype: unsupported number type");
