@@ -116,3 +116,18 @@ def test_diverging_flow_is_evicted(gpu, qp):
     u, _ = fs.result(good)
     ref = oracle_flow(qp, 32, 0.1, 100, 0.5)
     assert np.max(np.abs(u - ref)) <= TOL_FLOW
+
+
+def test_timing_counts_the_steps_taken(gpu, qp):
+    # dfrg_fs_timing: each graph launch advances the resident flows by the steps they have left
+    # (a flow admitted at the base step takes steps_per_launch per launch until its n_steps)
+    fs = gpu.FlowServer(qp, hbm_budget=1 << 20, max_resident=4, n_steps=130, k_end=0.2, steps_per_launch=25)
+    print("[case] flow timing: flows of 40 and 64 points, 130 steps, 25 per launch", flush=True)
+    a = fs.submit(40, 0.15)
+    b = fs.submit(64, 0.2)
+    run_all(fs, [a, b])
+    t = fs.timing()
+    # both admitted together: 6 launches (5 x 25 + the last 5 steps), each flow takes 130 steps
+    assert t["launches"] == 6, t
+    assert t["point_steps"] == (40 + 64) * 130, t
+    assert t["device_s"] > 0, t
