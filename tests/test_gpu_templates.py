@@ -100,10 +100,11 @@ def test_radial_complex_and_requests(kat, qp):
         assert out[i] == pytest.approx(O.kat_radial(4, x, w, 1.5, k, c, 0.0), rel=TOL)
 
 
-def test_many_requests_in_one_batch(kat, qp):
-    # 20,000 futures read only after all are requested: one batch, its tuple and result buffers
-    # grown well past any earlier batch of the integrator
-    n = 20000
+@pytest.mark.parametrize("n", [20000, 65536 + 100])
+def test_many_requests_in_one_batch(kat, qp, n):
+    # n futures read only after all are requested: one batch, its tuple and result buffers grown
+    # well past any earlier batch of the integrator; beyond 65,536 requests the full batch is
+    # launched at once and the rest go to the next batch
     ks = arr(np.linspace(0.05, 2.0, n))
     c = arr([0.3, -0.2, 0.5, 0.1, -0.4, 0.2])
     out = np.empty(n)
@@ -203,11 +204,14 @@ def test_three_angle_template(kat, qp):
         assert out.value == pytest.approx(ref, rel=TOL, abs=TOL * abs(const))
 
 
-@pytest.mark.parametrize("nx,na", [(37, 13), (5, 3), (129, 7), (300, 11)])
+@pytest.mark.parametrize("nx,na", [(37, 13), (5, 3), (129, 7), (300, 11), (33, 16), (3, 41)])
 def test_angle_templates_odd_sizes(kat, qp, nx, na):
     # Orders whose node counts do not divide the thread stride, so every digit of a thread's
     # node walk carries (the stride's lower digits are nonzero); a few sizes give more nodes than
-    # one pass of the grid, so threads take several strides.
+    # one pass of the grid, so threads take several strides.  Unit plans: rows of fewer than 256
+    # inner nodes are grouped (flat units), 256 or more make a unit per radial node (na = 16:
+    # exactly 256 for the 2-angle form), and more than 65,536 split each row (na = 41: 68,921
+    # inner nodes of the 3-angle form, two units per radial node).
     rng = np.random.default_rng(nx * 100 + na)
     xq, wq = qp.get_points(nx), qp.get_weights(nx)
     xa, wa = qp.get_points(na), qp.get_weights(na)
