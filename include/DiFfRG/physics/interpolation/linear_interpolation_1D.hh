@@ -7,17 +7,21 @@
 // on the device it reads the HBM copy.  It replaces both LinearInterpolator1D and the FP32
 // texture interpolator TexLinearInterpolator1D of the reference.
 //
-// FP64 logarithmic coordinates on the device: the reference's index log1p((x - c)/b) (N-1)/a costs
-// an FP64 division and log1p per call.  Here the cell comes from an FP32 log estimate and the
-// position in the cell from the tabulated cell start u_i = expm1(a i/(N-1)):
-//   idx - i = (N-1)/a log((1 + u)/(1 + u_i)) = (N-1)/a 2 atanh(z),  z = delta/(2 + delta),
-//   delta = (u - u_i)/(1 + u_i),
-// with the atanh series to z^17 (|z| <= 0.125 when a/(N-1) <= 1/4: truncation below 1e-17), and
-// the estimate's cell moved by one where the position falls outside [0, 1).  The index agrees with
-// the reference's to its own FP64 rounding (~1e-14 of a cell); the clamp to [0, N-1] and its NaN
-// rule (an index that is NaN takes the last point) are the reference's: points with (x - c)/b <= -1/2
-// (negative index, or NaN below -1) and NaN take the clamp's endpoint.  Coarser grids take the
-// reference's formula.
+// FP64 logarithmic coordinates on the device: the reference's index idx = log1p(u) (N-1)/a,
+// u = (x - c)/b, costs an FP64 division and log1p per call.  Here it comes from a table of sub-cell
+// starts: K sub-cells per cell (K the smallest power of two with |a|/(K(N-1)) <= 1/4), their starts
+// u_m = expm1(a m/(K(N-1))) and 1/(1 + u_m).  An FP32 log2 of the mantissa of 1 + u plus its exponent
+// locates the sub-cell m, and the position in it is
+//   K(N-1)/a log((1 + u)/(1 + u_m)) = K(N-1)/a 2 atanh(z),  z = delta/(2 + delta),
+//   delta = (u - u_m)/(1 + u_m),
+// with the atanh series to z^17 (|z| <= 0.125: truncation below 1e-17); an estimate one sub-cell off
+// (within ~1e-6 of a boundary) is moved by one.  The index agrees with the reference's to its own FP64
+// rounding (~1e-14 of a cell); the clamp to [0, N-1] and the reference's NaN rule (fmax(0, fmin(NaN,
+// N-1)): a NaN index takes the last point) give the ends: u <= -1 takes the first point when
+// -1 <= u and a > 0 (index -inf or negative), the last point otherwise (NaN, or +inf for a < 0);
+// N = 1, bias 0 and NaN bias have a NaN or zero-width index: the last point.  The device code has no
+// branch on the grid, so a user KERNEL's loop-invariant lookups (at q alone) stay hoistable.  Biases
+// beyond +-700 (exp overflows) and grids of more than 2^21 sub-cells per unit of log2 are refused.
 #pragma once
 
 #include <cmath>
@@ -50,19 +54,19 @@ namespace DiFfRG
         : size(coordinates.size()), coordinates(coordinates), owner(true)
     {
       dfrg::ensure_device();
+      if constexpr (kLogFP64) make_cells(); // may refuse the grid: before the table's allocation
       m_data = std::shared_ptr<NT[]>(new NT[size]);
       for (unsigned i = 0; i < size; ++i) m_data[i] = NT(in_data[i]);
       void *p = nullptr;
-      dfrg::check(dfrg_malloc(&p, sizeof(NT) * size), "LinearInterpolator1D");
+      // at least two points in HBM: a one-point table's lerp reads its point twice
+      dfrg::check(dfrg_malloc(&p, sizeof(NT) * (size < 2 ? 2 : size)), "LinearInterpolator1D");
       device_data = std::shared_ptr<void>(p, [](void *q) { dfrg_free(q); });
       device_data_ptr = (const NT *)p;
       update();
-      if constexpr (kLogFP64) make_cells();
     }
     LinearInterpolator1D(const LinearInterpolator1D &other)
         : size(other.size), coordinates(other.coordinates), m_data(other.m_data), device_data(other.device_data),
-          device_data_ptr(other.device_data_ptr), owner(false), cells(other.cells), cell_ptr(other.cell_ptr),
-          lc_c(other.lc_c), lc_inv_b(other.lc_inv_b), lc_2s(other.lc_2s), lc_s_ln2(other.lc_s_ln2)
+          device_data_ptr(other.device_data_ptr), owner(false), cells(other.cells), lc(other.lc)
     {
     }
 
@@ -77,6 +81,9 @@ namespace DiFfRG
       if (!owner) throw std::runtime_error("Cannot update data of non-owner interpolator");
       dfrg::check(dfrg_memcpy_h2d((void *)device_data_ptr, m_data.get(), sizeof(NT) * size, nullptr),
                  "LinearInterpolator1D::update");
+      if (size == 1)
+        dfrg::check(dfrg_memcpy_h2d((void *)(device_data_ptr + 1), m_data.get(), sizeof(NT), nullptr),
+                    "LinearInterpolator1D::update");
       dfrg::check(dfrg_stream_sync(nullptr), "LinearInterpolator1D::update");
     }
 
@@ -89,9 +96,7 @@ namespace DiFfRG
     {
       using ct = typename Coordinates::ctype;
 #ifdef __HIP_DEVICE_COMPILE__
-      if constexpr (kLogFP64) {
-        if (cell_ptr) return log_cell_lerp((x - lc_c) * lc_inv_b);
-      }
+      if constexpr (kLogFP64) return log_cell_lerp((x - lc.c) * lc.inv_b);
 #endif
       ct idx = coordinates.backward(x);
       idx = fmax(ct(0), fmin(idx, ct(size - 1)));
@@ -107,42 +112,77 @@ namespace DiFfRG
   private:
     static constexpr bool kLogFP64 = std::is_same_v<Coordinates, LogarithmicCoordinates1D<double>>;
 
-    // cell starts {u_i, 1/(1 + u_i)} of FP64 log coordinates, when a/(N-1) <= 1/4 and N <= 65536
-    // (the FP32 estimate then lands within one cell)
+    // the device index of FP64 log coordinates: sub-cell table and constants
+    struct LogCells {
+      const double2 *cell = nullptr; // {u_m, 1/(1 + u_m)}, m = 0 .. K(N-1)
+      double c = 0, inv_b = 0;       // u = (x - c) inv_b
+      double two_s = 0, s_ln2 = 0;   // 2 K(N-1)/a, K(N-1)/a ln 2
+      double inv_k = 1;              // 1/K
+      int m_last = 0, k_shift = 0;   // last sub-cell (K(N-1) - 1), log2 K
+      int last = 0;                  // last cell (N - 2; 0 for N = 1)
+      int const_last = 0;            // 1: every point takes the last point (N = 1, bias 0 or NaN)
+      int a_pos = 1;                 // bias > 0
+    };
+
     void make_cells()
     {
-      const double a = coordinates.bias, gem1inv = 1. / (size - 1.);
-      if (size < 2 || size > 65536 || !(a * gem1inv <= 0.25) || !(a > 0)) return;
-      std::vector<double> h(2 * size);
-      for (unsigned i = 0; i < size; ++i) {
-        h[2 * i] = std::expm1(a * i * gem1inv);
-        h[2 * i + 1] = 1. / (1. + h[2 * i]);
+      const double a = coordinates.bias;
+      const unsigned N = size;
+      LogCells L;
+      L.last = N >= 2 ? int(N) - 2 : 0;
+      std::vector<double2> h;
+      if (N >= 2 && a == a && a != 0) {
+        if (!(std::fabs(a) <= 700.))
+          throw std::invalid_argument("LinearInterpolator1D: the bias of LogarithmicCoordinates1D must be within "
+                                      "[-700, 700] (exp(bias) overflows beyond)");
+        int k_shift = 0;
+        while (std::fabs(a) / ((double)(1u << k_shift) * (N - 1.)) > 0.25) ++k_shift;
+        const double K = double(1u << k_shift), subs = K * (N - 1.);
+        if (subs / std::fabs(a) * 0.69314718055994530942 > double(1 << 21) || subs > double(1 << 24))
+          throw std::invalid_argument("LinearInterpolator1D: too many points for this bias of "
+                                      "LogarithmicCoordinates1D (more than 2^21 sub-cells per unit of log2)");
+        h.resize((size_t)subs + 1);
+        for (size_t m = 0; m < h.size(); ++m) {
+          const double t = a * (double)m / subs;
+          h[m] = make_double2(std::expm1(t), std::exp(-t));
+        }
+        L.c = coordinates.start;
+        L.inv_b = 1. / ((coordinates.stop - coordinates.start) / std::expm1(a));
+        L.two_s = 2. * subs / a;
+        L.s_ln2 = subs / a * 0.69314718055994530942;
+        L.inv_k = 1. / K;
+        L.m_last = int(subs) - 1;
+        L.k_shift = k_shift;
+        L.a_pos = a > 0;
+      } else {
+        // one point, bias 0 or NaN: the reference's index is 0 x (...)/a with a NaN or infinite
+        // factor, or the one point; all points take the last point.  A valid dummy table keeps the
+        // device code branch-free.
+        h = {make_double2(0., 1.), make_double2(0., 1.)};
+        L.const_last = 1;
       }
       void *p = nullptr;
-      dfrg::check(dfrg_malloc(&p, sizeof(double) * h.size()), "LinearInterpolator1D cells");
+      dfrg::check(dfrg_malloc(&p, sizeof(double2) * h.size()), "LinearInterpolator1D cells");
       cells = std::shared_ptr<void>(p, [](void *q) { dfrg_free(q); });
-      dfrg::check(dfrg_memcpy_h2d(p, h.data(), sizeof(double) * h.size(), nullptr), "LinearInterpolator1D cells");
+      dfrg::check(dfrg_memcpy_h2d(p, h.data(), sizeof(double2) * h.size(), nullptr), "LinearInterpolator1D cells");
       dfrg::check(dfrg_stream_sync(nullptr), "LinearInterpolator1D cells");
-      cell_ptr = (const double *)p;
-      lc_c = coordinates.start;
-      lc_inv_b = 1. / ((coordinates.stop - coordinates.start) / std::expm1(a));
-      lc_2s = 2. * (size - 1.) / a;
-      lc_s_ln2 = float((size - 1.) / a * 0.69314718055994530942);
+      L.cell = (const double2 *)p;
+      lc = L;
     }
 
-    // the lerp at the index of u = (x - c)/b (device).  For u <= -1/2 the reference's index is
-    // negative (-1 <= u: the first point) or NaN (u < -1, NaN: the last point, its NaN clamp); those
-    // take the clamp's endpoint by selects, so the loop that calls this has no branch
+    // the lerp at the reference's index of u = (x - c)/b (device, no branch on the grid)
     __device__ __forceinline__ NT log_cell_lerp(const double u_in) const
     {
-      const int last = (int)size - 2;
-      const bool inside = u_in > -0.5;
+      const bool inside = u_in > -1.; // false for NaN
       const double u = inside ? u_in : 0.;
-      const float est = __builtin_amdgcn_logf(1.f + (float)u) * lc_s_ln2; // v_log_f32 is log2
-      int i = (int)floorf(fmaxf(0.f, fminf(est, (float)last)));
-      const double *cl = cell_ptr + 2 * i;
-      double dl = (u - cl[0]) * cl[1];
-      dl = fmax(-0.5, fmin(dl, 1.0)); // beyond the cell by more than one: the clamp decides
+      // sub-cell estimate: log2(1 + u) = e + log2(mant), mant in [1/2, 1), FP32 log of the mantissa
+      int e = 0;
+      const double mant = frexp(1. + u, &e);
+      const double est = ((double)e + (double)__builtin_amdgcn_logf((float)mant)) * lc.s_ln2;
+      int m = (int)floor(fmax(0., fmin(est, (double)lc.m_last)));
+      const double2 cl = dfrg::gld(lc.cell + m); // {u_m, 1/(1 + u_m)}
+      double dl = (u - cl.x) * cl.y;
+      dl = fmax(-0.5, fmin(dl, 1.0)); // beyond the sub-cell by more than one: the clamp decides
       // z = dl/(2 + dl): FP32 seed, two FP64 Newton steps
       const double den = 2. + dl;
       double r = (double)__builtin_amdgcn_rcpf((float)den);
@@ -157,23 +197,22 @@ namespace DiFfRG
       pz = fma(pz, w, 1. / 5);
       pz = fma(pz, w, 1. / 3);
       pz = fma(pz, w, 1.);
-      double f = lc_2s * (z * pz);
-      // the FP32 estimate can be one cell off where the index is within ~1e-5 of an integer
-      if (f < 0. && i > 0) {
-        --i;
-        f += 1.;
-      } else if (f >= 1. && i < last) {
-        ++i;
-        f -= 1.;
-      }
-      f = fmax(0., fmin(f, 1.));
-      if (!inside) {
-        const bool first = u_in >= -1.; // false for NaN
-        i = first ? 0 : last;
-        f = first ? 0. : 1.;
-      }
+      double fs = lc.two_s * (z * pz); // position in the sub-cell
+      // the estimate can be one sub-cell off where the index is within ~1e-6 of a boundary
+      const bool down = fs < 0. && m > 0, up = !down && fs >= 1. && m < lc.m_last;
+      m += up ? 1 : down ? -1 : 0;
+      fs += down ? 1. : up ? -1. : 0.;
+      fs = fmax(0., fmin(fs, 1.));
+      // sub-cell -> cell (K a power of two: exact)
+      int i = m >> lc.k_shift;
+      double f = ((double)(m & ((1 << lc.k_shift) - 1)) + fs) * lc.inv_k;
+      // the clamp's ends: u <= -1 or NaN, and the degenerate grids
+      const bool first = !lc.const_last && !inside && lc.a_pos && u_in >= -1.;
+      const bool to_last = lc.const_last || (!inside && !first);
+      i = first ? 0 : to_last ? lc.last : i;
+      f = first ? 0. : to_last ? 1. : f;
       const NT *d = device_data_ptr + i;
-      return d[0] * (1. - f) + d[1] * f;
+      return dfrg::gld(d) * (1. - f) + dfrg::gld(d + 1) * f;
     }
 
     unsigned size;
@@ -183,8 +222,6 @@ namespace DiFfRG
     const NT *device_data_ptr = nullptr;
     bool owner;
     std::shared_ptr<void> cells;
-    const double *cell_ptr = nullptr;
-    double lc_c = 0, lc_inv_b = 0, lc_2s = 0;
-    float lc_s_ln2 = 0;
+    LogCells lc;
   };
 } // namespace DiFfRG

@@ -168,7 +168,7 @@ namespace dfrg
       template <class NT, class KERNEL, class C, class... T>
       static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
       {
-        const AngNode<C> a = n.ang[g[1]];
+        const AngNode<C> a = gld(n.ang + g[1]);
         return NT((rf * a.a) * KERNEL::kernel(q, a.c, k, t...));
       }
     };
@@ -188,7 +188,7 @@ namespace dfrg
       template <class NT, class KERNEL, class C, class... T>
       static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
       {
-        const AngNode<C> a1 = n.ang[g[1]], a2 = n.ang[n.na + g[2]];
+        const AngNode<C> a1 = gld(n.ang + g[1]), a2 = gld(n.ang + n.na + g[2]);
         return NT((rf * (a1.a * a2.a)) * KERNEL::kernel(q, a1.c, a2.c, k, t...));
       }
     };
@@ -208,7 +208,7 @@ namespace dfrg
       template <class NT, class KERNEL, class C, class... T>
       static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
       {
-        const AngNode<C> a1 = n.ang[g[1]], ap = n.ang[n.na + g[2]], a2 = n.ang[2 * n.na + g[3]];
+        const AngNode<C> a1 = gld(n.ang + g[1]), ap = gld(n.ang + n.na + g[2]), a2 = gld(n.ang + 2 * n.na + g[3]);
         return NT((rf * (a1.a * ap.a * a2.a)) * KERNEL::kernel(q, a1.c, a2.c, ap.c, k, t...));
       }
     };
@@ -229,8 +229,8 @@ namespace dfrg
       template <class NT, class KERNEL, class C, class... T>
       static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
       {
-        const C q0 = n.mx[g[1]];
-        NT r = NT((rf * n.mw[g[1]]) * (KERNEL::kernel(q, q0, k, t...) + KERNEL::kernel(q, -q0, k, t...)));
+        const C q0 = gld(n.mx + g[1]);
+        NT r = NT((rf * gld(n.mw + g[1])) * (KERNEL::kernel(q, q0, k, t...) + KERNEL::kernel(q, -q0, k, t...)));
         if (n.m_T > 0 && g[1] == 0) r += NT((rf * n.m_T) * KERNEL::kernel(q, C(0), k, t...));
         return r;
       }
@@ -252,9 +252,9 @@ namespace dfrg
       template <class NT, class KERNEL, class C, class... T>
       static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
       {
-        const AngNode<C> a = n.ang[g[1]];
-        const C q0 = n.mx[g[2]], ra = rf * a.a;
-        NT r = NT((ra * n.mw[g[2]]) * (KERNEL::kernel(q, a.c, q0, k, t...) + KERNEL::kernel(q, a.c, -q0, k, t...)));
+        const AngNode<C> a = gld(n.ang + g[1]);
+        const C q0 = gld(n.mx + g[2]), ra = rf * a.a;
+        NT r = NT((ra * gld(n.mw + g[2])) * (KERNEL::kernel(q, a.c, q0, k, t...) + KERNEL::kernel(q, a.c, -q0, k, t...)));
         if (n.m_T > 0 && g[2] == 0) r += NT((ra * n.m_T) * KERNEL::kernel(q, a.c, C(0), k, t...));
         return r;
       }
@@ -276,9 +276,9 @@ namespace dfrg
       template <class NT, class KERNEL, class C, class... T>
       static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
       {
-        const AngNode<C> a = n.ang[g[1]], ap = n.ang[n.na + g[2]];
-        const C q0 = n.mx[g[3]], ra = rf * (a.a * ap.a);
-        NT r = NT((ra * n.mw[g[3]]) *
+        const AngNode<C> a = gld(n.ang + g[1]), ap = gld(n.ang + n.na + g[2]);
+        const C q0 = gld(n.mx + g[3]), ra = rf * (a.a * ap.a);
+        NT r = NT((ra * gld(n.mw + g[3])) *
                   (KERNEL::kernel(q, a.c, ap.c, q0, k, t...) + KERNEL::kernel(q, a.c, ap.c, -q0, k, t...)));
         if (n.m_T > 0 && g[3] == 0) r += NT((ra * n.m_T) * KERNEL::kernel(q, a.c, ap.c, C(0), k, t...));
         return r;
@@ -720,10 +720,10 @@ namespace dfrg
       {
         // the staging returns to the pool only after the work that writes into it has finished
         if (ev) {
-          hipEventSynchronize(ev);
-          hipEventDestroy(ev);
+          (void)hipEventSynchronize(ev);
+          (void)hipEventDestroy(ev);
         } else if (stream) {
-          hipStreamSynchronize(stream);
+          (void)hipStreamSynchronize(stream);
         }
         if (h) dfrg_pool_host_free(h);
       }
@@ -783,7 +783,7 @@ namespace dfrg
               ev = nullptr;
               ok = false;
             } else if (hipEventRecord(ev, s) != hipSuccess) { // the destructor then waits on the stream
-              hipEventDestroy(ev);
+              (void)hipEventDestroy(ev);
               ev = nullptr;
               ok = false;
             }

@@ -7,6 +7,8 @@
 //   ctype_t<NT>    the real type of a number type (types.hh:130-167)
 //   JsonLike       what the integrator and provider constructors read from a parameter object
 //                  (the getters of the reference's JSONValue, common/json.hh:62-139)
+//   gld(p)         device load of *p as global memory (for pointers the compiler cannot place:
+//                  members of argument tuples and node tables point into HBM)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,6 +23,29 @@
 
 namespace dfrg
 {
+  // *p through a global-address-space pointer: a global load with a scalar base and 32-bit offsets
+  // instead of a flat load with a 64-bit address per lane.  p must point to device-visible global
+  // memory (HBM or mapped host memory), as every table of the integrators and interpolators does.
+  template <class T> __device__ __forceinline__ T gld(const T *p)
+  {
+    if constexpr (std::is_scalar_v<T>) {
+      return *(const __attribute__((address_space(1))) T *)p;
+    } else {
+      // class types (complex, dual numbers, node records): their words, then the bytes
+      static_assert(std::is_trivially_copyable_v<T>, "gld: the type must be trivially copyable");
+      using W = std::conditional_t<sizeof(T) % 8 == 0 && alignof(T) >= 8, unsigned long long, unsigned>;
+      static_assert(sizeof(T) % sizeof(W) == 0, "gld: the size must be a multiple of 4 bytes");
+      constexpr int n = sizeof(T) / sizeof(W);
+      const __attribute__((address_space(1))) W *q = (const __attribute__((address_space(1))) W *)p;
+      W w[n];
+#pragma unroll
+      for (int i = 0; i < n; ++i) w[i] = q[i];
+      T r;
+      __builtin_memcpy(&r, w, sizeof(T));
+      return r;
+    }
+  }
+
   template <int n, typename NT> constexpr __forceinline__ __host__ __device__ NT powr(const NT x)
   {
     if constexpr (n == 0)

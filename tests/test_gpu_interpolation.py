@@ -222,43 +222,65 @@ def test_dual_number_channels(kat):
 
 
 def log_index64(x, n, start, stop, bias):
-    """The reference's FP64 index log1p((x - c)/b) (n-1)/a clamped to [0, n-1], NaN -> n-1."""
-    b = (stop - start) / math.expm1(bias)
-    with np.errstate(invalid="ignore", divide="ignore"):
-        idx = np.log1p((x - start) / b) * (n - 1.0) / bias
-    return np.clip(np.nan_to_num(idx, nan=n - 1.0, posinf=n - 1.0, neginf=0.0), 0, n - 1)
+    """The reference's FP64 index log1p((x - c)/b) (n-1)/a clamped to [0, n-1], a NaN index -> n-1
+    (fmax(0, fmin(NaN, n-1)), the reference's device clamp), in IEEE arithmetic as the C++ does."""
+    f = np.float64
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        b = (f(stop) - f(start)) / f(np.expm1(f(bias)))
+        idx = np.log1p((x - f(start)) / b) * (f(n) - 1.0) / f(bias)
+    return np.clip(np.where(np.isnan(idx), n - 1.0, idx), 0, n - 1)
 
 
-@pytest.mark.parametrize("n,bias", [(128, 6.0), (256, 1.0), (32, 6.0), (2000, 3.0), (10, 6.0), (2, 0.2)])
-def test_fp64_log_coordinates_on_the_device(kat, n, bias):
-    """LinearInterpolator1D<double> on FP64 log coordinates: on the device the index comes from an
-    FP32 cell estimate and an FP64 in-cell series (grids with a/(n-1) <= 1/4), elsewhere from the
-    reference's log1p.  A table linear in the index pins the index itself; the points include grid
-    nodes, both sides of cell boundaries at 1e-7 and 1e-13, both ends, far below the start (where
-    log1p has no real value and the NaN index takes the last point), NaN and +inf."""
-    rng = np.random.default_rng(n)
-    start, stop = 1e-3, 40.0
+@pytest.mark.parametrize("n,bias,start,stop", [(128, 6.0, 1e-3, 40.0), (256, 1.0, 1e-3, 40.0), (32, 6.0, 1e-3, 40.0),
+                                               (2000, 3.0, 1e-3, 40.0), (10, 6.0, 1e-3, 40.0), (2, 0.2, 1e-3, 40.0),
+                                               (64, -3.0, 1e-3, 40.0), (8, 60.0, 1e-3, 40.0), (5, 300.0, 0.0, 1.0),
+                                               (3, 5.0, 2.0, 1.0), (20, 4.0, 1.0, 1.0), (1, 5.0, 0.0, 1.0),
+                                               (20, 0.0, 0.0, 1.0)])
+def test_fp64_log_coordinates_on_the_device(kat, n, bias, start, stop):
+    """LinearInterpolator1D<double> on FP64 log coordinates: on the device the index comes from a
+    table of sub-cell starts (an FP32 estimate of the sub-cell, an FP64 series in it), with no
+    branch on the grid.  A table linear in the index pins the index itself; the grids include
+    sub-cells (bias/(n-1) > 1/4), negative bias, biases whose 1 + u leaves the FP32 range, a
+    reversed axis, start == stop, one point and bias 0; the points include grid nodes, both sides of
+    cell boundaries at 1e-7 and 1e-13, both ends, below the start (where log1p has no real value and
+    the NaN index takes the last point), +-1e300, +-inf and NaN."""
+    rng = np.random.default_rng(n + int(abs(bias) * 10))
     cp = [n, start, stop, bias]
-    b = (stop - start) / math.expm1(bias)
-    nodes = b * np.expm1(bias * np.arange(n) / (n - 1.0)) + start
+    f = np.float64
+    with np.errstate(all="ignore"):
+        b = (f(stop) - f(start)) / f(np.expm1(f(bias)))
+        nodes = b * np.expm1(bias * np.arange(n) / max(n - 1.0, 1.0)) + start
+    nodes = nodes[np.isfinite(nodes)]
     near = np.concatenate([nodes * (1 + e) for e in (1e-7, -1e-7, 1e-13, -1e-13)])
-    pts = np.concatenate([nodes, near, rng.uniform(start, stop, 400), np.exp(rng.uniform(np.log(start), np.log(stop), 400)),
-                          [0.0, start - 0.3 * b, start - 0.5 * b, start - 0.7 * b, start - 2 * b, stop, 1.5 * stop,
-                           1e300, np.inf, np.nan]])
+    lo, hi = min(start, stop), max(start, stop)
+    below = [start - s * b for s in (0.3, 0.5, 0.99, 1.0, 2.0)] if np.isfinite(b) else []
+    pts = np.concatenate([nodes, near, rng.uniform(lo, hi, 400), np.exp(rng.uniform(np.log(lo + 1e-3), np.log(hi + 1e-3), 400)),
+                          below, [0.0, start, stop, 1.5 * stop, 1e300, -1e300, np.inf, -np.inf, np.nan]])
     # the index within the reference's own FP64 rounding: the input (x - c)/b carries an error of
-    # a few ulp, which (n-1)/a log1p turns into ~1e-16 (n-1)/a cells; the log's own rounding ~1e-16 n
-    tol_idx = 8 * 2.0 ** -52 * ((n - 1.0) / bias + n)
+    # a few ulp, which (n-1)/a log1p turns into ~1e-16 (n-1)/|a| cells (times 1/(1 + u) <= e^|a| for
+    # a < 0, where the top cells crowd towards u = -1); the log's own rounding ~1e-16 n
+    amp = math.exp(-bias) if bias < 0 else 1.0
+    tol_idx = 8 * 2.0 ** -52 * ((max(n, 2) - 1.0) / max(abs(bias), 1e-300) * amp + n) if bias != 0 else 0.0
     lin = np.arange(n, dtype=np.float64) * 0.75 - 3.0
     dev, host = run(kat, 8, cp, lin, pts)
     want = log_index64(pts, n, start, stop, bias) * 0.75 - 3.0
     assert np.all(np.isfinite(dev)) and np.all(np.isfinite(host))
-    assert np.max(np.abs(dev - want)) <= 0.75 * tol_idx + 4 * 2.0 ** -52 * 3.0 * n
+    assert np.max(np.abs(dev - want)) <= 0.75 * tol_idx + 4 * 2.0 ** -52 * 3.0 * n, np.max(np.abs(dev - want))
     assert np.max(np.abs(host - want)) <= 0.75 * tol_idx + 4 * 2.0 ** -52 * 3.0 * n
     # a smooth table: device against the host's reference formula, at the table's cell slopes
     smooth = 1.0 + 0.5 * np.tanh(np.sin(np.arange(n) * 0.37) * 2.0)
     dev, host = run(kat, 8, cp, smooth, pts)
     slope = np.max(np.abs(np.diff(smooth))) if n > 1 else 0.0
     assert np.max(np.abs(dev - host)) <= slope * tol_idx + 4 * 2.0 ** -52
+
+
+def test_fp64_log_coordinates_refused_bias(kat):
+    # beyond |bias| = 700 exp(bias) overflows; the device index refuses such grids at construction
+    cp = np.array([16, 0.0, 1.0, 800.0])
+    pts = np.array([0.5])
+    dev, host = np.empty(1), np.empty(1)
+    rc = kat.kat_interp(8, P(cp), P(np.zeros(16)), 16, P(pts), 1, P(dev), P(host))
+    assert rc != 0 and b"bias" in kat.kat_last_error()
 
 
 def test_dual_number_fp64_interpolator(kat):
