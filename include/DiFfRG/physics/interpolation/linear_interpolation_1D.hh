@@ -106,7 +106,7 @@ namespace DiFfRG
 #else
       const NT *d = m_data.get();
 #endif
-      return d[unsigned(fl)] * (ct(1) - idx + fl) + d[unsigned(ceil(idx))] * (idx - fl);
+      return dfrg::tld(d + unsigned(fl)) * (ct(1) - idx + fl) + dfrg::tld(d + unsigned(ceil(idx))) * (idx - fl);
     }
 
   private:

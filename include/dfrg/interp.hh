@@ -88,7 +88,7 @@ namespace dfrg
     {
       idx = fmax(C(0), fmin(idx, C(n - 1)));
       const C fl = floor(idx);
-      return d[unsigned(fl)] * (C(1) - idx + fl) + d[unsigned(ceil(idx))] * (idx - fl);
+      return tld(d + unsigned(fl)) * (C(1) - idx + fl) + tld(d + unsigned(ceil(idx))) * (idx - fl);
     }
 
     template <typename V, typename C>
@@ -96,14 +96,15 @@ namespace dfrg
     {
       const Axis<C> x(ix, nx), y(iy, ny);
       const V *r0 = d + (size_t)x.i0 * ny, *r1 = d + (size_t)x.i1 * ny;
-      return r0[y.i0] * x.w0 * y.w0 + r0[y.i1] * x.w0 * y.w1 + r1[y.i0] * x.w1 * y.w0 + r1[y.i1] * x.w1 * y.w1;
+      return tld(r0 + y.i0) * x.w0 * y.w0 + tld(r0 + y.i1) * x.w0 * y.w1 + tld(r1 + y.i0) * x.w1 * y.w0 +
+             tld(r1 + y.i1) * x.w1 * y.w1;
     }
 
     template <typename V, typename C>
     __forceinline__ __host__ __device__ V lerp3(const V *d, C ix, C iy, C iz, unsigned nx, unsigned ny, unsigned nz)
     {
       const Axis<C> x(ix, nx), y(iy, ny), z(iz, nz);
-      auto at = [&](unsigned a, unsigned b, unsigned c) { return d[((size_t)a * ny + b) * nz + c]; };
+      auto at = [&](unsigned a, unsigned b, unsigned c) { return tld(d + ((size_t)a * ny + b) * nz + c); };
       return at(x.i0, y.i0, z.i0) * x.w0 * y.w0 * z.w0 + at(x.i0, y.i0, z.i1) * x.w0 * y.w0 * z.w1 +
              at(x.i0, y.i1, z.i0) * x.w0 * y.w1 * z.w0 + at(x.i0, y.i1, z.i1) * x.w0 * y.w1 * z.w1 +
              at(x.i1, y.i0, z.i0) * x.w1 * y.w0 * z.w0 + at(x.i1, y.i0, z.i1) * x.w1 * y.w0 * z.w1 +

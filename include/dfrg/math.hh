@@ -46,6 +46,17 @@ namespace dfrg
     }
   }
 
+  // *p: a global load on the device (gld), a plain read on the host; for tables that live in HBM
+  // on the device and in host memory on the host
+  template <class T> __host__ __device__ __forceinline__ T tld(const T *p)
+  {
+#ifdef __HIP_DEVICE_COMPILE__
+    return gld(p);
+#else
+    return *p;
+#endif
+  }
+
   template <int n, typename NT> constexpr __forceinline__ __host__ __device__ NT powr(const NT x)
   {
     if constexpr (n == 0)
