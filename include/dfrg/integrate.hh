@@ -775,9 +775,9 @@ namespace dfrg
           stream = s;
           bool ok = hipHostGetDevicePointer(&dh, h, 0) == hipSuccess &&
                     hipMemcpyAsync(dd + o_args, h, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess;
+          const Args *d_args = (const Args *)(dd + o_args);
           if (ok) {
-            launch((const Args *)(dd + o_args), n, p, (double *)(dd + o_part), A.counters(),
-                   (NT *)((char *)dh + h_res), s);
+            launch(d_args, n, p, (double *)(dd + o_part), A.counters(), (NT *)((char *)dh + h_res), s);
             ok = hipGetLastError() == hipSuccess;
             if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
               ev = nullptr;
