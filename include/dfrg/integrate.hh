@@ -20,10 +20,10 @@
 //
 // get() is a batch of one, launched at once and waited for.  request() adds to the integrator's
 // open batch; a batch is launched (asynchronously: tuples to HBM, one kernel whose finishing blocks
-// store the results into pinned host memory, an event) when any future of this thread is read, when request_data() finishes its
-// loop, when the batch is full, or when a request needs another launch configuration.  Reading a future launches every open
-// batch of the calling thread first, so batches of different integrators run back to back before
-// the first wait.
+// store the results into pinned host memory, an event) when any future of this thread is read,
+// when request_data() finishes its loop, when the batch is full, or when a request needs another
+// launch configuration.  Reading a future launches every open batch of the calling thread first,
+// so batches of different integrators run back to back before the first wait.
 //
 // Scratch: every host thread owns an arena in HBM on its stream (ticket counters, partial slots,
 // argument tuples).  Counters are zeroed when a thread takes an arena (new, grown, or left
@@ -59,7 +59,7 @@ namespace dfrg
   namespace tmpl
   {
     constexpr int kMaxQ = 256; // radial nodes a flat unit may span (LDS staging)
-    
+
     // ------------------------------------------------------------------------------------------
     // number types: scalar (float, double) or two-component (complex: real/imag; dual numbers:
     // value/derivative), accumulated component-wise in FP64
@@ -254,7 +254,8 @@ namespace dfrg
       {
         const AngNode<C> a = gld(n.ang + g[1]);
         const C q0 = gld(n.mx + g[2]), ra = rf * a.a;
-        NT r = NT((ra * gld(n.mw + g[2])) * (KERNEL::kernel(q, a.c, q0, k, t...) + KERNEL::kernel(q, a.c, -q0, k, t...)));
+        NT r = NT((ra * gld(n.mw + g[2])) *
+                  (KERNEL::kernel(q, a.c, q0, k, t...) + KERNEL::kernel(q, a.c, -q0, k, t...)));
         if (n.m_T > 0 && g[2] == 0) r += NT((ra * n.m_T) * KERNEL::kernel(q, a.c, C(0), k, t...));
         return r;
       }
