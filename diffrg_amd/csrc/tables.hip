@@ -3,7 +3,7 @@
 // coordinates.hh:141-196) held in HBM, for callers that bind the library through an FFI and cannot
 // instantiate the C++ templates.  A table is n FP64 values on the coordinates
 //   y(i) = b expm1(bias i/(n-1)) + start,  b = (stop - start)/expm1(bias),
-// and evaluates as the template does: idx = log1p((y - start)/b) (n-1)/bias, clamped to [0, n-1]
+// and evaluates with the reference's index: idx = log1p((y - start)/b) (n-1)/bias, clamped to [0, n-1]
 // with fmax/fmin (a NaN index gives the last value), linear between floor and ceil.
 #include "internal.hh"
 
