@@ -50,9 +50,13 @@ SE_OPS = {0: {"pairs": (223.5, 203.0), "single": (223.5, 205.0)},
 SE_OPS_BLOCK = {}  # (mode, block, loop) whose schedule differs from the others
 
 
+BASELINE_NOTES = "BASELINE.md"
+
+
 def frozen_weights():
-    """FLOP (and VALU) per evaluation frozen in BASELINE.md (the ```json flop-weights block)."""
-    text = open(os.path.join(ROOT, "BASELINE.md")).read()
+    """FLOP (and VALU) per evaluation, measured once from the loops' assembly and frozen in the
+    repository's baseline notes (their ```json flop-weights block)."""
+    text = open(os.path.join(ROOT, BASELINE_NOTES)).read()
     start = text.index("```json flop-weights") + len("```json flop-weights")
     return json.loads(text[start:text.index("```", start)])
 
@@ -630,7 +634,7 @@ def main():
     }
     na = se["na"]
     v1 = se["variant"][1]
-    # FLOP per evaluation: the weights frozen in BASELINE.md; VALU per evaluation: the current
+    # FLOP per evaluation: the frozen weights; VALU per evaluation: the current
     # assembly of the loop that ran (for the issue fraction)
     w1, w0 = FROZEN["C1_selfenergy"]["mixed"], FROZEN["C1_selfenergy"]["fp64"]
     fl1, fl0 = w1["flop"], w0["flop"]
@@ -641,7 +645,7 @@ def main():
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                        "frac": achieved / peak, "traffic": measured_traffic(), "kernel": f"se_kernel<MIXED, {mx['shape']['block']}>",
                        "integrand_form": {1: "general", 2: "narrow"}[v1],
-                       "flop_per_eval": fl1, "flop_per_eval_source": "frozen (BASELINE.md flop-weights)",
+                       "flop_per_eval": fl1, "flop_per_eval_source": "frozen flop-weights block",
                        "ceiling": ceiling(w1), "valu_per_eval": va1,
                        "loop": se_loop(na, mx["shape"]["block"], mx["unroll"]),
                        "evals_per_launch": evals,
