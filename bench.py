@@ -464,11 +464,11 @@ def template(D, qp, args):
          "path": "Integrator4D2AngGPU<NT, SelfEnergyKernel> + request_data/update_data (physics/utils.hh)"}
     for fp32, name, tol in ((0, "fp64", 1e-12), (1, "fp32_eval_fp64_acc", 1e-5)):
         out, pg = np.empty(n), np.empty(n)
-        td, th = C.c_double(), C.c_double()
+        td, th, ph = C.c_double(), C.c_double(), np.empty(5)
         rc = kat.kat_selfenergy_request_bench(c["nq"], c["na"], C.c_double(c["x_extent"]), C.c_double(c["k"]), n,
                                               P(Z), len(Z), C.c_double(c["coord"][0]), C.c_double(c["coord"][1]),
                                               C.c_double(c["coord"][2]), fp32, 20, P(out), P(pg), C.byref(td),
-                                              C.byref(th))
+                                              C.byref(th), P(ph))
         if rc != 0:
             raise RuntimeError(kat.kat_last_error())
         ref = O.c1_selfenergy(pg, [c["Z"]] * n, [coord] * n, [xq] * n, [xa] * n, c["x_extent"], c["k"])
@@ -480,7 +480,9 @@ def template(D, qp, args):
                    "within_tolerance": err <= tol, "tolerance": tol,
                    "flop_per_eval": w["flop"], "valu_per_eval": w["valu"], "ceiling": ceiling(w),
                    "roofline_frac": w["flop"] * evals / t / PEAK_FP64_VALU,
-                   "valu_issue_frac": valu_issue_frac(w["valu"], evals, t)}
+                   "valu_issue_frac": valu_issue_frac(w["valu"], evals, t),
+                   "us_phases": {"host_requests": ph[0] * 1e6, "host_launch": ph[1] * 1e6, "host_reads": ph[2] * 1e6,
+                                 "device_launch_to_enqueued": ph[3] * 1e6, "device_enqueued_to_read": ph[4] * 1e6}}
         log(f"template {name}: {t * 1e6:.1f} us device, {th.value * 1e6:.1f} us host per call set, err {err:.2e}")
     return r
 

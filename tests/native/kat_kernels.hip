@@ -786,9 +786,13 @@ int kat_request_overlap(int nq, int na, double x_extent, double k, int n_p, cons
 // one batch -- HIP events on the thread's stream around its launch and the reads: tuples to HBM,
 // the kernel, results back -- and of the host time of the whole call set (requests, launch,
 // reads).  fp32 != 0 runs the FP32 form of the integrand (NT = float, FP64 accumulation).
+// phases (may be NULL): medians of the host request loop, the host launch (flush), the host reads,
+// the device time from before the launch to an event enqueued right after it (copy and kernel plus
+// the host's enqueue) and from that event to the one recorded after the reads (the host's wake-up
+// and reads), in seconds
 int kat_selfenergy_request_bench(int nq, int na, double x_extent, double k, int n_p, const double *Z, int n_tab,
                                  double start, double stop, double bias, int fp32, int reps, double *out,
-                                 double *p_out, double *t_dev, double *t_host)
+                                 double *p_out, double *t_dev, double *t_host, double *phases)
 {
   return guard([&] {
     LogarithmicCoordinates1D<double> pgrid((unsigned)n_p, 1e-3, 40.0, 6.0);
@@ -797,10 +801,11 @@ int kat_selfenergy_request_bench(int nq, int na, double x_extent, double k, int 
     provider(); // initialises the library on device 0
     void *sv = nullptr;
     dfrg::check(dfrg_thread_stream(&sv), "thread stream");
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    hipEvent_t e0, e1, em;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess || hipEventCreate(&em) != hipSuccess)
       throw std::runtime_error("event create failed");
-    std::vector<double> dev, host;
+    std::vector<double> dev, host, ph[5];
+    auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
     auto run = [&](auto &w, auto &grid_c, auto kk, auto &interp, auto *res) {
       using R = std::remove_pointer_t<decltype(res)>;
       const auto g = make_grid(grid_c);
@@ -809,17 +814,27 @@ int kat_selfenergy_request_bench(int nq, int na, double x_extent, double k, int 
         // the requests of request_data's loop; the batch is launched between the events
         std::vector<std::future<R>> fut;
         for (unsigned i = 0; i < g.size(); ++i) fut.push_back(w.template request<R>(kk, g[i], interp));
-        hipEventRecord(e0, (hipStream_t)sv);
-        dfrg::tmpl::flush_thread();
-        for (unsigned i = 0; i < g.size(); ++i) res[i] = fut[i].get();
-        hipEventRecord(e1, (hipStream_t)sv);
-        hipEventSynchronize(e1);
         auto t1 = std::chrono::steady_clock::now();
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventRecord(e0, (hipStream_t)sv);
+        dfrg::tmpl::flush_thread();
+        (void)hipEventRecord(em, (hipStream_t)sv);
+        auto t2 = std::chrono::steady_clock::now();
+        for (unsigned i = 0; i < g.size(); ++i) res[i] = fut[i].get();
+        (void)hipEventRecord(e1, (hipStream_t)sv);
+        (void)hipEventSynchronize(e1);
+        auto t3 = std::chrono::steady_clock::now();
+        float ms = 0.f, ms0 = 0.f, ms1 = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms0, e0, em);
+        (void)hipEventElapsedTime(&ms1, em, e1);
         if (r > 0) {
           dev.push_back(ms * 1e-3);
-          host.push_back(std::chrono::duration<double>(t1 - t0).count());
+          host.push_back(secs(t0, t3));
+          ph[0].push_back(secs(t0, t1));
+          ph[1].push_back(secs(t1, t2));
+          ph[2].push_back(secs(t2, t3));
+          ph[3].push_back(ms0 * 1e-3);
+          ph[4].push_back(ms1 * 1e-3);
         }
       }
     };
@@ -839,12 +854,17 @@ int kat_selfenergy_request_bench(int nq, int na, double x_extent, double k, int 
       run(w, pgrid, k, interp, outf.data());
       for (int i = 0; i < n_p; ++i) out[i] = outf[i];
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    std::sort(dev.begin(), dev.end());
-    std::sort(host.begin(), host.end());
-    *t_dev = dev[dev.size() / 2];
-    *t_host = host[host.size() / 2];
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(em);
+    auto median = [](std::vector<double> &v) {
+      std::sort(v.begin(), v.end());
+      return v[v.size() / 2];
+    };
+    *t_dev = median(dev);
+    *t_host = median(host);
+    if (phases)
+      for (int i = 0; i < 5; ++i) phases[i] = median(ph[i]);
   });
 }
 

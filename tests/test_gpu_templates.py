@@ -501,7 +501,7 @@ def test_selfenergy_request_data_full_size(kat, qp, fp32):
     out, pg, td, th = np.empty(64), np.empty(64), D(), D()
     print(f"[case] self-energy call set through request_data, fp32={fp32}", flush=True)
     call(kat, "kat_selfenergy_request_bench", 64, 32, D(20.0), D(1.0), 64, P(Z), len(Z), D(1e-3), D(40.0), D(6.0),
-         fp32, 3, P(out), P(pg), C.byref(td), C.byref(th))
+         fp32, 3, P(out), P(pg), C.byref(td), C.byref(th), None)
     assert np.allclose(pg, c["p"], rtol=1e-15, atol=0)
     coord = O.logcoord(len(Z), 1e-3, 40.0, 6.0)
     xq, xa = (qp.get_points(64), qp.get_weights(64)), (qp.get_points(32), qp.get_weights(32))
