@@ -520,7 +520,7 @@ def flows(D, qp, args):
     budget = 64 << 20
     # saturation: n_sat flows submitted at t = 0
     n_sat = min(args.flows, 512)
-    fs = D.FlowServer(qp, hbm_budget=budget, max_resident=64, steps_per_launch=25)
+    fs = D.FlowServer(qp, hbm_budget=budget, max_resident=128, steps_per_launch=25)
     reqs = W.flows(n_sat, rate=1.0)
     t0 = time.perf_counter()
     for q in reqs:
@@ -534,7 +534,7 @@ def flows(D, qp, args):
     n_flows = args.flows
     rate = 0.7 * sat
     reqs = W.flows(n_flows, rate=rate)
-    fs = D.FlowServer(qp, hbm_budget=budget, max_resident=64, steps_per_launch=25)
+    fs = D.FlowServer(qp, hbm_budget=budget, max_resident=128, steps_per_launch=25)
     pending = list(range(n_flows))
     ids = [None] * n_flows
     t0 = time.perf_counter()
@@ -589,7 +589,7 @@ def flows(D, qp, args):
                           "parallel_over": "flows" if args.flow_check_frac > 0 else "grid points"},
             "config": "O(4) LPA flows, n_grid ~ U{64..1024}, T ~ U[0.01,0.3], mu ~ U[0,0.4], RK4 2000 steps "
                       "k=1 -> 1e-3, two 64-node threshold integrals per point and stage; HBM budget 64 MiB, "
-                      "64 slots, 25 steps per graph launch"}
+                      "128 slots, 25 steps per graph launch"}
 
 
 def main():
