@@ -236,6 +236,12 @@ namespace
     int persistent;
     int xcd_map; // one tile per block: give each XCD a contiguous run of tiles
     double *out;
+    // shared layout: every integral has the same nodes, table and tile count, so tile t is proto
+    // with integral t / tiles_per_int, radial start (t mod tiles_per_int) nqt, slot t, and p from
+    // p_ext: a block issues its table and node loads without waiting for a tile record
+    int shared, tiles_per_int;
+    const double *p_ext;
+    SETile proto;
   };
 
   // Blocks are dispatched to the eight XCDs in turn (block b to XCD b mod 8).  With xcd_map, the
@@ -413,11 +419,14 @@ namespace
     __shared__ double s_red[BLOCK / 64];
     __shared__ int s_tile;
 
-    if (!F64)
-      for (int j = threadIdx.x; j < 64; j += BLOCK) {
+    // 2^(j/64) in the block's last wave, which stages nothing, so wave 0's loads are not delayed
+    if (!F64) {
+      const int j = (int)threadIdx.x - (BLOCK - 64);
+      if (j >= 0) {
         s_exp2[j] = exp2(j * (1.0 / 64.0));
         s_exp2k[j] = A.k * A.k * s_exp2[j];
       }
+    }
     const double k = A.k, k2 = k * k;
     int cur = -1; // integral whose table and angular terms are in LDS (block-uniform)
     for (int iter = 0;; ++iter) {
@@ -431,7 +440,16 @@ namespace
         t = iter == 0 ? (A.xcd_map ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) : A.n_tiles;
       }
       if (t >= A.n_tiles) break;
-      const SETile T = A.tiles[t];
+      SETile T;
+      if (A.shared) {
+        T = A.proto;
+        T.integral = t / A.tiles_per_int;
+        T.q0 = (t - T.integral * A.tiles_per_int) * T.nqt;
+        T.slot = t;
+        T.first_slot = T.integral * A.tiles_per_int;
+      } else {
+        T = A.tiles[t];
+      }
       const int i = T.integral;
       // one round of independent loads: table and angular terms (when the integral changes) and
       // the radial nodes of this tile
@@ -453,10 +471,11 @@ namespace
         }
       }
       const double *xq = A.nodes + T.xq_off;
+      const double pe = A.shared ? A.p_ext[i] : T.p;
       for (int j = threadIdx.x; j < T.nqt; j += BLOCK) {
         const double q = k * sqrt(xq[T.q0 + j] * A.x_extent);
-        s_qs[j] = T.p * T.p + q * q;
-        s_qc[j] = -2.0 * T.p * q;
+        s_qs[j] = pe * pe + q * q;
+        s_qc[j] = -2.0 * pe * q;
         s_qw[j] = make_double2(q, xq[T.nq + T.q0 + j] * A.x_extent);
       }
       __syncthreads();
@@ -576,7 +595,7 @@ struct dfrg_se_batch {
   std::vector<double> p, coord; // coord: c, 1/b, (n-1)/bias
   unsigned long long evals = 0;
   // device data
-  double *d_nodes = nullptr, *d_tab64 = nullptr;
+  double *d_nodes = nullptr, *d_tab64 = nullptr, *d_p = nullptr;
   SECellY *d_tabA = nullptr;
   SECellZ *d_tabB = nullptr;
   SETile *d_tiles = nullptr;
@@ -587,6 +606,9 @@ struct dfrg_se_batch {
   int block = 256, tile_evals = 8192, grid = 0, n_tiles = 0;
   std::vector<int> narrow; // per integral: the mixed path may use se_G_mixed<true>
   int uniform_tiles = 0;    // every tile has the same evaluation count
+  int shared_layout = 0;    // every tile record follows from proto and its index (SEArgs::shared)
+  int tiles_per_int = 0;
+  SETile proto{};
   std::string signature;
   // graph
   hipGraph_t graph = nullptr;
@@ -642,6 +664,29 @@ namespace
     b->uniform_tiles = 1;
     for (const SETile &t : tiles)
       if ((long long)t.nqt * t.na * t.na != (long long)tiles[0].nqt * tiles[0].na * tiles[0].na) b->uniform_tiles = 0;
+    // shared layout: the kernel's tile from proto and the index equals every record here
+    b->shared_layout = 0;
+    if (!tiles.empty()) {
+      SETile pr = tiles[0];
+      pr.p = 0.0;
+      pr.integral = pr.q0 = pr.slot = pr.first_slot = 0;
+      const int tpi = pr.n_slots;
+      bool same = tpi > 0 && (long long)tpi * n == (long long)tiles.size();
+      for (int t = 0; same && t < (int)tiles.size(); ++t) {
+        SETile c = pr;
+        c.integral = t / tpi;
+        c.q0 = (t - c.integral * tpi) * pr.nqt;
+        c.slot = t;
+        c.first_slot = c.integral * tpi;
+        c.p = b->p[c.integral];
+        same = std::memcmp(&c, &tiles[t], sizeof(SETile)) == 0;
+      }
+      if (same) {
+        b->shared_layout = 1;
+        b->tiles_per_int = tpi;
+        b->proto = pr;
+      }
+    }
     if ((size_t)b->n_tiles > b->tiles_cap) {
       hipFree(b->d_tiles);
       hipFree(b->d_partials);
@@ -674,6 +719,10 @@ namespace
     // equal tiles only: with uneven tiles the longest-first order balances the XCDs better
     a.xcd_map = !a.persistent && b->uniform_tiles;
     a.out = d_out;
+    a.shared = b->shared_layout;
+    a.tiles_per_int = b->tiles_per_int > 0 ? b->tiles_per_int : 1;
+    a.p_ext = b->d_p;
+    a.proto = b->proto;
     return a;
   }
 
@@ -923,12 +972,14 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
   DFRG_HIP_B(hipMalloc((void **)&b->d_tab64, sizeof(double) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tabA, sizeof(SECellY) * tables_len));
   DFRG_HIP_B(hipMalloc((void **)&b->d_tabB, sizeof(SECellZ) * tables_len));
+  DFRG_HIP_B(hipMalloc((void **)&b->d_p, sizeof(double) * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_counters, sizeof(unsigned) * dev::kCounterStride * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_queue, sizeof(unsigned) * 2 * dev::kCounterStride));
   DFRG_HIP_B(hipMemcpy(b->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tab64, tables, sizeof(double) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tabA, tabA.data(), sizeof(SECellY) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tabB, tabB.data(), sizeof(SECellZ) * tables_len, hipMemcpyHostToDevice));
+  DFRG_HIP_B(hipMemcpy(b->d_p, p, sizeof(double) * n_int, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemset(b->d_counters, 0, sizeof(unsigned) * dev::kCounterStride * n_int));
   DFRG_HIP_B(hipMemset(b->d_queue, 0, sizeof(unsigned) * 2 * dev::kCounterStride));
 #undef DFRG_HIP_B
@@ -959,6 +1010,7 @@ int dfrg_se_batch_destroy(dfrg_se_batch *b)
   hipFree(b->d_tab64);
   hipFree(b->d_tabA);
   hipFree(b->d_tabB);
+  hipFree(b->d_p);
   hipFree(b->d_tiles);
   hipFree(b->d_partials);
   hipFree(b->d_counters);
@@ -1134,33 +1186,78 @@ int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cac
   float best = 1e30f;
   bool all_ok = true;
   int bb = b->block, bt = b->tile_evals, bg = b->grid;
+  // settle the clocks first: launches of the current shape until 4 ms of device time have passed
+  // (at most 2048), so the first candidates are not timed on a GPU still ramping up
+  {
+    bool ok = hipEventRecord(e0, s) == hipSuccess;
+    for (int n = 0; ok && n < 2048; n += 16) {
+      for (int r = 0; r < 16 && ok; ++r) ok = se_launch(b, mode, d_out, s) == DFRG_OK;
+      float t = 0.f;
+      ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+           hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+      if (t >= 4.f) break;
+    }
+    if (!ok) hipGetLastError();
+  }
+  // time per launch of `n` launches at the current shape after one untimed launch; < 0 when a
+  // launch or event call failed
+  auto time_shape = [&](int n) -> float {
+    bool ok = se_launch(b, mode, d_out, s) == DFRG_OK; // warm-up
+    ok = ok && hipEventRecord(e0, s) == hipSuccess;
+    for (int r = 0; r < n && ok; ++r) ok = se_launch(b, mode, d_out, s) == DFRG_OK;
+    ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float t = 0.f;
+    ok = ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess && t > 0.f;
+    if (!ok) {
+      hipGetLastError(); // clear a launch error so it is not reported by a later call
+      return -1.f;
+    }
+    return t / n;
+  };
+  struct Cand {
+    float t;
+    int bl, te, gr;
+  };
+  std::vector<Cand> cands;
+  // round 1: every shape, `reps` launches each; a shape counts only when every launch and event
+  // call of its measurement succeeded
   for (int bl : blocks) {
     for (int te : tiles) {
       for (int gmul : {0, 2, 4, 8}) {
         const int gr = gmul == 0 ? 0 : gmul * cus * 512 / bl;
         if (dfrg_se_batch_set_shape(b, bl, te, gr) != DFRG_OK) continue;
-        // a shape counts only when every launch and event call of its measurement succeeded
-        bool ok = se_launch(b, mode, d_out, s) == DFRG_OK; // warm-up
-        ok = ok && hipEventRecord(e0, s) == hipSuccess;
-        for (int r = 0; r < reps && ok; ++r) ok = se_launch(b, mode, d_out, s) == DFRG_OK;
-        ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-        float t = 0.f;
-        ok = ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess && t > 0.f;
-        if (!ok) {
+        const float t = time_shape(reps);
+        if (t < 0.f) {
           all_ok = false;
-          hipGetLastError(); // clear a launch error so it is not reported by a later call
           continue;
         }
-        t /= reps;
-        if (t < best) {
-          best = t;
-          bb = bl;
-          bt = te;
-          bg = gr;
-        }
+        cands.push_back({t, bl, te, gr});
       }
     }
   }
+  // round 2: the four fastest again, 4 x reps launches each, in three interleaved passes (the
+  // best pass counts), so one noisy measurement does not decide between shapes a few % apart
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.t < y.t; });
+  if (cands.size() > 4) cands.resize(4);
+  std::vector<float> t2(cands.size(), 1e30f);
+  for (int pass = 0; pass < 3; ++pass)
+    for (size_t c = 0; c < cands.size(); ++c) {
+      const float t = dfrg_se_batch_set_shape(b, cands[c].bl, cands[c].te, cands[c].gr) == DFRG_OK
+                          ? time_shape(4 * reps)
+                          : -1.f;
+      if (t < 0.f) {
+        all_ok = false;
+        continue;
+      }
+      t2[c] = std::min(t2[c], t);
+    }
+  for (size_t c = 0; c < cands.size(); ++c)
+    if (t2[c] < best) {
+      best = t2[c];
+      bb = cands[c].bl;
+      bt = cands[c].te;
+      bg = cands[c].gr;
+    }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   hipStreamDestroy(s);

@@ -168,7 +168,9 @@ int dfrg_se_batch_set_shape(dfrg_se_batch *b, int block, int tile_evals, int gri
 int dfrg_se_batch_get_shape(dfrg_se_batch *b, int *block, int *tile_evals, int *grid, int *n_tiles);
 /* Enqueue one evaluation of every integral: d_out[i] = I_i (n_int doubles in HBM). */
 int dfrg_se_batch_run(dfrg_se_batch *b, int mode, double *d_out, void *stream);
-/* Time shapes on this batch and keep the fastest (per mode).  cache_path (may be NULL) is a
+/* Time shapes on this batch and keep the fastest (per mode): after ~4 ms of launches that let the
+ * clocks settle, every shape with `reps` launches, then the four fastest again in three
+ * interleaved passes of 4 reps launches (the best pass counts).  cache_path (may be NULL) is a
  * text file of earlier results keyed by device name and batch signature: a hit skips the
  * search; a miss appends the winner. */
 int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cache_path, int *block,
