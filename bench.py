@@ -187,6 +187,29 @@ N_STREAMS = 4
 BATCH = 8  # call sets per launch in the batched measurements of the small workloads
 
 
+def eager_and_graph(D, fn, args):
+    """ms per call of fn() launched eagerly from Python, and replayed from a hipGraph of GRAPH_SETS
+    captured calls (no host cost per call: a kernel of a few microseconds launched from Python is
+    otherwise timed at the host's launch rate)."""
+    ms_e, _ = time_steps(D, fn, args.steps, args.warmup)
+    g = D.Graph(fn, times=GRAPH_SETS)
+    ms_g, _ = time_steps(D, g.launch, max(1, args.steps // GRAPH_SETS), max(1, args.warmup // GRAPH_SETS))
+    return ms_e, ms_g / GRAPH_SETS
+
+
+def ft_best_block(D, run):
+    """Threads per integral of ft_batch with the shortest device time on this batch (hipGraph of
+    GRAPH_SETS calls per candidate, so the host's launch rate does not decide)."""
+    best = None
+    for block in (64, 128, 256, 512):
+        D.ft_set_block(block)
+        g = D.Graph(run, times=GRAPH_SETS)
+        t, _ = time_steps(D, g.launch, 20, 5)
+        if best is None or t < best[0]:
+            best = (t, block)
+    return best[1]
+
+
 def finite(v):
     return v if math.isfinite(v) and v != 0.0 else None
 
@@ -305,10 +328,12 @@ def threshold(D, qp, args):
     out = D.DeviceArray(len(c["m2"]))
     r = {}
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
-        ms, _ = time_steps(D, lambda: D.threshold_batch(d_m2, d_x, d_w, out, c["x_extent"], c["k"], mode, c["d"]),
-                           args.steps, args.warmup)
+        run = lambda: D.threshold_batch(d_m2, d_x, d_w, out, c["x_extent"], c["k"], mode, c["d"])
+        ms_e, ms_g = eager_and_graph(D, run, args)
+        ms = min(ms_e, ms_g)
         wt = FROZEN["C0_threshold"][name]
         r[name] = {"integrals_per_s": len(c["m2"]) / (ms * 1e-3), "us_per_step": ms * 1e3,
+                   "us_eager": ms_e * 1e3, "us_graph": ms_g * 1e3, "launch": "hipGraph" if ms_g < ms_e else "eager",
                    "max_rel_err": rel_err(out.to_host(), ref),
                    "roofline_frac": wt["flop"] * len(c["m2"]) * 64 / (ms * 1e-3) / PEAK_FP64_VALU,
                    "ceiling": ceiling(wt)}
@@ -317,8 +342,8 @@ def threshold(D, qp, args):
     d_m2b = D.DeviceArray.from_host(np.tile(c["m2"], BATCH))
     outb = D.DeviceArray(BATCH * len(c["m2"]))
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
-        ms, _ = time_steps(D, lambda: D.threshold_batch(d_m2b, d_x, d_w, outb, c["x_extent"], c["k"], mode, c["d"]),
-                           args.steps, args.warmup)
+        ms = min(eager_and_graph(D, lambda: D.threshold_batch(d_m2b, d_x, d_w, outb, c["x_extent"], c["k"], mode,
+                                                              c["d"]), args))
         r[name][f"integrals_per_s_batched_x{BATCH}"] = BATCH * len(c["m2"]) / (ms * 1e-3)
     t_cpu, runs = cpu_timed(ref_fn, 2.0)
     r["cpu_integrals_per_s"] = len(c["m2"]) / t_cpu
@@ -349,20 +374,17 @@ def finite_T(D, qp, args):
     r = {"evals_per_step": evals}
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
         run = lambda: D.ft_batch(dev, out, c["q_max"], c["k"], mode, c["d"], c["radial_mode"])
-        best = None
-        for block in (64, 128, 256, 512):  # threads per integral, timed on this batch
-            D.ft_set_block(block)
-            t, _ = time_steps(D, run, 50, 5)
-            if best is None or t < best[0]:
-                best = (t, block)
-        D.ft_set_block(best[1])
-        ms, _ = time_steps(D, run, args.steps, args.warmup)
+        best = ft_best_block(D, run)
+        D.ft_set_block(best)
+        ms_e, ms_g = eager_and_graph(D, run, args)
+        ms = min(ms_e, ms_g)
         got = out.to_host()
         err = np.abs(got - ref) / np.abs(ref)
         wt = FROZEN["C2_finite_T"][name]
         pairs = int(sum(c["nx"] * len(c["rules"][t][0]) for t in c["T"]))
         r[name] = {"integrals_per_s": len(c["T"]) / (ms * 1e-3), "evals_per_s": evals / (ms * 1e-3),
-                   "us_per_step": ms * 1e3, "max_rel_err": float(err.max()), "block": best[1],
+                   "us_per_step": ms * 1e3, "us_eager": ms_e * 1e3, "us_graph": ms_g * 1e3,
+                   "launch": "hipGraph" if ms_g < ms_e else "eager", "max_rel_err": float(err.max()), "block": best,
                    "roofline_frac": wt["flop"] * pairs / (ms * 1e-3) / PEAK_FP64_VALU, "ceiling": ceiling(wt),
                    "valu_issue_frac": valu_issue_frac(wt["valu"], pairs, ms * 1e-3),
                    "max_rel_err_per_T": {str(t): float(err[c["T"] == t].max()) for t in W.TEMPS}}
@@ -375,16 +397,11 @@ def finite_T(D, qp, args):
     outb = D.DeviceArray(BATCH * len(c["T"]))
     for mode, name in ((D.FP64, "fp64"), (D.MIXED, "mixed")):
         run = lambda: D.ft_batch(devb, outb, c["q_max"], c["k"], mode, c["d"], c["radial_mode"])
-        best = None
-        for block in (64, 128, 256, 512):
-            D.ft_set_block(block)
-            t, _ = time_steps(D, run, 20, 2)
-            if best is None or t < best[0]:
-                best = (t, block)
-        D.ft_set_block(best[1])
-        ms, _ = time_steps(D, run, args.steps, args.warmup)
+        best = ft_best_block(D, run)
+        D.ft_set_block(best)
+        ms = min(eager_and_graph(D, run, args))
         r[name][f"integrals_per_s_batched_x{BATCH}"] = BATCH * len(c["T"]) / (ms * 1e-3)
-        r[name][f"block_batched_x{BATCH}"] = best[1]
+        r[name][f"block_batched_x{BATCH}"] = best
         if name == "mixed":
             got = outb.to_host()
             r[name][f"max_rel_err_batched_x{BATCH}"] = float(np.max(np.abs(got - np.tile(ref, BATCH)) /

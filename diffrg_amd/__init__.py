@@ -65,6 +65,10 @@ SIGNATURES = {
     "dfrg_event_record": [_vp, _vp],
     "dfrg_event_sync": [_vp],
     "dfrg_event_elapsed_ms": [_vp, _vp, C.POINTER(C.c_float)],
+    "dfrg_graph_begin": [_vp],
+    "dfrg_graph_end": [_vp, C.POINTER(_vp)],
+    "dfrg_graph_launch": [_vp, _vp],
+    "dfrg_graph_destroy": [_vp],
     "dfrg_pool_alloc": [C.POINTER(_vp), C.c_size_t],
     "dfrg_pool_free": [_vp],
     "dfrg_pool_host_alloc": [C.POINTER(_vp), C.c_size_t],
@@ -263,6 +267,40 @@ class Event:
     def __del__(self):
         try:
             lib().dfrg_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Graph:
+    """Work enqueued by `fn()` (called `times` times on `stream`, NULL: the thread's stream)
+    captured into a hipGraph; launch() replays all of it with one launch.  fn must only enqueue
+    (the batched entry points do); a call that synchronises fails the capture."""
+
+    def __init__(self, fn, times=1, stream=None):
+        self.ptr = None
+        check(lib().dfrg_graph_begin(stream))
+        err = None
+        try:
+            for _ in range(times):
+                fn()
+        except Exception as e:  # the capture must end either way
+            err = e
+        g = _vp()
+        rc = lib().dfrg_graph_end(stream, C.byref(g))
+        if err is not None:
+            if g.value:
+                lib().dfrg_graph_destroy(g)
+            raise err
+        check(rc)
+        self.ptr = g.value
+
+    def launch(self, stream=None):
+        check(lib().dfrg_graph_launch(self.ptr, stream))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().dfrg_graph_destroy(self.ptr)
         except Exception:
             pass
 

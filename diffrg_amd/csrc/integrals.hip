@@ -203,7 +203,9 @@ namespace
   // ============================================================================================
   // A tile is (integral, range of radial nodes).  Everything a block needs about its tile sits in
   // one record, so the block starts with a single load; the table, angular terms and the tile's
-  // radial nodes then come in one round of independent loads into LDS.
+  // radial nodes then come in one round of independent loads into LDS.  When every integral of a
+  // batch shares nodes, table and tile count, the record follows from the tile index and one
+  // prototype in the kernel arguments (SEArgs::shared), and that round is the block's first.
   struct SETile {
     double p, c, inv_b, scale; // external momentum; log coordinates: start c, 1/b, (n-1)/bias
     int integral, q0, nqt, slot;

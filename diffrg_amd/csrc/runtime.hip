@@ -311,6 +311,64 @@ int dfrg_event_elapsed_ms(void *start, void *stop, float *ms)
   return DFRG_OK;
 }
 
+struct dfrg_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int dfrg_graph_begin(void *stream)
+{
+  DFRG_TRY(require_init());
+  DFRG_HIP(hipStreamBeginCapture(resolve_stream(stream), hipStreamCaptureModeThreadLocal));
+  return DFRG_OK;
+}
+
+int dfrg_graph_end(void *stream, dfrg_graph **graph)
+{
+  DFRG_TRY(require_init());
+  hipGraph_t g = nullptr;
+  // the capture ends in every case, so a failed argument check does not leave the stream capturing
+  const hipError_t e = hipStreamEndCapture(resolve_stream(stream), &g);
+  if (graph == nullptr) {
+    if (g) hipGraphDestroy(g);
+    DFRG_CHECK_ARG(false, "graph is NULL");
+  }
+  *graph = nullptr;
+  if (e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    return DFRG_ERR_HIP;
+  }
+  auto *out = new dfrg_graph;
+  out->graph = g;
+  const hipError_t ei = hipGraphInstantiate(&out->exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    hipGraphDestroy(g);
+    delete out;
+    set_error("hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    return DFRG_ERR_HIP;
+  }
+  *graph = out;
+  return DFRG_OK;
+}
+
+int dfrg_graph_launch(dfrg_graph *graph, void *stream)
+{
+  DFRG_TRY(require_init());
+  DFRG_CHECK_ARG(graph != nullptr && graph->exec != nullptr, "NULL graph");
+  DFRG_HIP(hipGraphLaunch(graph->exec, resolve_stream(stream)));
+  return DFRG_OK;
+}
+
+int dfrg_graph_destroy(dfrg_graph *graph)
+{
+  if (!graph) return DFRG_OK;
+  if (graph->exec) hipGraphExecDestroy(graph->exec);
+  if (graph->graph) hipGraphDestroy(graph->graph);
+  delete graph;
+  return DFRG_OK;
+}
+
 int dfrg_pool_alloc(void **d_ptr, size_t bytes)
 {
   DFRG_TRY(require_init());

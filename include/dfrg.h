@@ -80,6 +80,18 @@ int dfrg_event_destroy(void *event);
 int dfrg_event_record(void *event, void *stream);
 int dfrg_event_sync(void *event);
 int dfrg_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Capture of enqueued work into a hipGraph, replayed with one launch.  No counterpart in the
+ * reference: it takes the host's per-launch cost out of repeated small launches.  Between
+ * dfrg_graph_begin and dfrg_graph_end the calling thread's work on `stream` (NULL: the thread's
+ * stream) is recorded instead of run.  dfrg_threshold_batch, dfrg_ft_batch, dfrg_se_batch_run and
+ * dfrg_se_batch_run_noalloc enqueue kernels only (no memsets, no synchronisation), so any
+ * sequence of them can be captured; calls that synchronise (the flow server, table updates)
+ * cannot. */
+typedef struct dfrg_graph dfrg_graph;
+int dfrg_graph_begin(void *stream);
+int dfrg_graph_end(void *stream, dfrg_graph **graph);
+int dfrg_graph_launch(dfrg_graph *graph, void *stream);
+int dfrg_graph_destroy(dfrg_graph *graph);
 /* Stream-ordered caching pool for per-call scratch (partials, counters, result slots) used by
  * the C++ integrator templates; replaces the per-integrator rmm pools
  * (integrator_4D_2ang_gpu.hh:86-90).  Memory returned by dfrg_pool_alloc is zeroed once at

@@ -84,6 +84,31 @@ def test_selfenergy_fp64_and_mixed(gpu, qp):
         assert err.max() <= tol
 
 
+@pytest.mark.parametrize("block,tile,grid", [(512, 16384, 0), (256, 4096, 0), (256, 8192, 512)])
+def test_shared_layout_equals_tile_records(gpu, qp, block, tile, grid):
+    # Integrals that share nodes and table (C1) take their tiles from the kernel arguments and the
+    # block index; one more integral with another radial order makes the batch read tile records
+    # instead.  Each of the 64 shared integrals gives the same bits either way.
+    c = W.selfenergy()
+    n = len(c["p"])
+    b = selfenergy_batch(gpu, qp, c)
+    b.set_shape(block, tile, grid)
+    nq = np.full(n + 1, c["nq"])
+    nq[-1] = c["nq"] // 2
+    rec = gpu.SEBatch(qp, list(c["p"]) + [1.0], nq, [c["na"]] * (n + 1), [c["Z"]] * (n + 1),
+                      *[[v] * (n + 1) for v in c["coord"]], x_extent=c["x_extent"], k=c["k"])
+    rec.set_shape(block, tile, grid)
+    out, out_rec = gpu.DeviceArray(n), gpu.DeviceArray(n + 1)
+    ref = selfenergy_oracle(qp, c)
+    for mode, tol in ((gpu.FP64, TOL_FP64), (gpu.MIXED, TOL_MIXED)):
+        print(f"[case] shared layout vs records block={block} tile={tile} grid={grid} mode={mode}", flush=True)
+        b.run(out, mode)
+        rec.run(out_rec, mode)
+        shared = out.to_host()
+        assert np.array_equal(shared, out_rec.to_host()[:n])
+        assert rel_err(shared, ref).max() <= tol
+
+
 def test_selfenergy_partials_never_stale(gpu, qp):
     # Cross-block partials are published with device-scope atomics and no cache-wide fence.
     # Alternating the modes makes every published partial differ from the one a slot held
@@ -406,3 +431,59 @@ def test_empty_and_negative_batch_sizes(gpu, qp):
         gpu.check(L.dfrg_ft_batch(4, 1, 2, *([None] * 11), 16, 20.0, 1.0, gpu.FP64, None, None))
     with pytest.raises(gpu.DfrgInvalidArgument, match="n_int"):
         gpu.SEBatch(qp, [], [], [], [], [], [], [])
+
+
+def test_graph_capture_replays_batches(gpu, qp):
+    # Threshold, finite-T and self-energy launches captured into one hipGraph (two call sets of
+    # each).  Between replays the same batches run eagerly in the other mode, which overwrites the
+    # outputs with other values; every replay must restore the eager results of its own mode bitwise.
+    ct = W.threshold()
+    x, w = qp.get_points(64), qp.get_weights(64)
+    d_m2 = gpu.DeviceArray.from_host(ct["m2"])
+    d_x, d_w = gpu.DeviceArray.from_host(x), gpu.DeviceArray.from_host(w)
+    out_t = gpu.DeviceArray(len(ct["m2"]))
+    cf = W.finite_T(n=64, nx=37, n_max=100)
+    dev, _, _ = finite_T_device(gpu, cf, qp)
+    out_f = gpu.DeviceArray(len(cf["T"]))
+    cs = W.selfenergy(n_p=16)
+    b = selfenergy_batch(gpu, qp, cs)
+    out_s = gpu.DeviceArray(len(cs["p"]))
+    outs = (out_t, out_f, out_s)
+
+    def enqueue(mode):
+        gpu.threshold_batch(d_m2, d_x, d_w, out_t, ct["x_extent"], ct["k"], mode, ct["d"])
+        gpu.ft_batch(dev, out_f, cf["q_max"], cf["k"], mode, cf["d"], cf["radial_mode"])
+        b.run(out_s, mode)
+
+    for mode, other in ((gpu.FP64, gpu.MIXED), (gpu.MIXED, gpu.FP64)):
+        print(f"[case] graph of threshold + finite-T + self-energy, mode={mode}", flush=True)
+        enqueue(mode)
+        eager = [o.to_host() for o in outs]
+        g = gpu.Graph(lambda: enqueue(mode), times=2)
+        for rep in range(4):
+            enqueue(other)
+            assert not np.array_equal(out_s.to_host(), eager[2])
+            g.launch()
+            for o, e in zip(outs, eager):
+                assert np.array_equal(o.to_host(), e)
+
+
+def test_graph_capture_ends_when_the_enqueuing_code_raises(gpu, qp):
+    # an exception in the enqueuing code (after one captured launch) ends the capture, discards the
+    # graph and propagates; the stream then runs eager launches again
+    c = W.threshold()
+    x, w = qp.get_points(64), qp.get_weights(64)
+    d_m2 = gpu.DeviceArray.from_host(c["m2"])
+    d_x, d_w = gpu.DeviceArray.from_host(x), gpu.DeviceArray.from_host(w)
+    out = gpu.DeviceArray(len(c["m2"]))
+
+    def enqueue_then_raise():
+        gpu.threshold_batch(d_m2, d_x, d_w, out, c["x_extent"], c["k"], gpu.FP64, c["d"])
+        raise ValueError("caller error inside the capture")
+
+    print("[case] capture ended by a Python exception", flush=True)
+    with pytest.raises(ValueError):
+        gpu.Graph(enqueue_then_raise)
+    gpu.threshold_batch(d_m2, d_x, d_w, out, c["x_extent"], c["k"], gpu.FP64, c["d"])
+    ref = O.c0_threshold(c["m2"], x, w, c["x_extent"], c["k"], 1.0, c["d"])
+    assert rel_err(out.to_host(), ref).max() <= TOL_FP64
