@@ -465,6 +465,9 @@ namespace dfrg
                 args);
           };
           NodeWalk<S::ND - 1, int> w(m0 + tid, B, rad + 1);
+          // two nodes per iteration: their evaluations are independent chains the scheduler can
+          // interleave (the sum keeps node order, so results do not change)
+#pragma unroll 2
           for (int m = m0 + tid; m < m1; m += B, w.next()) acc.add(eval(w));
         }
       }
