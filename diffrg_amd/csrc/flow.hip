@@ -13,8 +13,8 @@
 // steps) and a slot in SoA descriptor arrays.  The host cuts the resident flows into runs of
 // consecutive grid points; one launch advances every run by a whole RK4 step, one workgroup per
 // run, evaluating the stages outside the run as far as the next stages' stencils reach
-// (flow_step_kernel), each wave four grid points at a time (one Gauss-Legendre node per lane, the
-// eight node sums reduced together across lanes).  steps_per_launch steps are captured into one hipGraph.  A flow's step
+// (flow_step_kernel), each wave two grid points at a time (one Gauss-Legendre node per lane, the
+// four node sums reduced together across lanes).  steps_per_launch steps are captured into one hipGraph.  A flow's step
 // index is the launch's base step plus the step within the graph minus the base step at its
 // admission, so flows admitted at different times share the graph and a finished flow simply
 // stops updating.  The graph's last node advances the device copy
@@ -55,6 +55,9 @@ namespace
   constexpr int kHalo = 3;                        // stages 1-3 are also evaluated this far outside the run
   constexpr int kMaxRun = 256;                    // longest run
   constexpr int kSpan = kMaxRun + 2 * (kHalo + 1); // the run and the stencil reach of its stage 1
+  // grid points a wave evaluates together: 2 (rather than 4) balances a stage's points over the 8
+  // waves at short runs; each node sum gets the same add tree either way (dev::wave_sum4)
+  constexpr int kGroup = 2;
 
   struct FlowDev {
     int *status, *bad;    // per slot; status 0 idle, 1 running (host-owned)
@@ -94,7 +97,7 @@ namespace
   // point wider: points outside the run are evaluated again by the workgroup that owns them, with
   // the same inputs and arithmetic, so every point's result is what a stage-by-stage sweep gives
   // and does not depend on how flows are cut into runs.  Within a stage the waves share the points
-  // four at a time (lanes over Gauss-Legendre nodes).  u is read from the buffer of this step's
+  // two at a time (lanes over Gauss-Legendre nodes).  u is read from the buffer of this step's
   // parity and written to the other, so neighbouring runs read the old u.
   __global__ __launch_bounds__(kRunBlock) void flow_step_kernel(FlowDev F, int local)
   {
@@ -159,13 +162,13 @@ namespace
           s_m2sig[p] = yv + 2.0 * rho * up;
         }
         __syncthreads();
-        // right-hand sides, four points per wave at a time: their eight node sums (two threshold
-        // integrals each) are reduced together by dev::wave_sum8 (kind of point pp: vector 2 pp + kind)
+        // right-hand sides, two points per wave at a time: their four node sums (two threshold
+        // integrals each) are reduced together by dev::wave_sum4 (kind of point pp: vector 2 pp + kind)
         const double k2 = k * k, k5 = k * k * (k * k) * k;
-        for (int g0 = p0 + 4 * wave; g0 < p1; g0 += 4 * n_wv) {
-          double v[8];
+        for (int g0 = p0 + kGroup * wave; g0 < p1; g0 += kGroup * n_wv) {
+          double v[2 * kGroup];
 #pragma unroll
-          for (int pp = 0; pp < 4; ++pp) {
+          for (int pp = 0; pp < kGroup; ++pp) {
             const int p = g0 + pp;
             double ipi = 0.0, isg = 0.0;
             if (p < p1) { // wave-uniform
@@ -187,9 +190,9 @@ namespace
             v[2 * pp] = ipi;
             v[2 * pp + 1] = isg;
           }
-          const double sum = dev::wave_sum8(v);
+          const double sum = dev::wave_sum4(v);
 #pragma unroll
-          for (int pp = 0; pp < 4; ++pp) {
+          for (int pp = 0; pp < kGroup; ++pp) {
             const int p = g0 + pp;
             if (p >= p1) continue; // wave-uniform
             const double ipi = k5 * dev::lane_value(sum, dev::sum8_lane(2 * pp));

@@ -84,6 +84,20 @@ namespace dfrg
       return y;
     }
 
+    // Sums of four wave-wide vectors: wave_sum8's first two levels leave each vector in one 16-lane
+    // row, then the same DPP steps finish inside the rows.  Every vector goes through wave_sum8's
+    // add tree, so its total is bitwise the one wave_sum8 gives it; vector v's total is in the 16
+    // lanes from sum8_lane(v) (v < 4).  21 VALU instructions for the four sums.
+    __device__ __forceinline__ double wave_sum4(const double (&v)[4])
+    {
+      const double x = pair_sum16(pair_sum32(v[0], v[1]), pair_sum32(v[2], v[3]));
+      double y = x + dpp_f64<0x128>(x); // row_ror:8 pairs lane l with l ^ 8
+      y += dpp_f64<0xB1>(y);            // lane ^ 1
+      y += dpp_f64<0x4E>(y);            // lane ^ 2
+      y += dpp_f64<0x141>(y);           // the two quads of each 8 lanes
+      return y;
+    }
+
     // Block sum in a fixed order: wave sums to LDS, then wave 0 adds them in index order.
     // `smem` needs blockDim.x / 64 doubles.  Returns the total to every thread of the block.
     __device__ __forceinline__ double block_sum(double v, double *smem)

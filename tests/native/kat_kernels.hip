@@ -418,6 +418,13 @@ namespace
     for (int i = 0; i < 8; ++i) v[i] = in[i * 64 + l];
     out[l] = dfrg::dev::wave_sum8(v);
   }
+  __global__ void wave_sum4_kernel(const double *in, double *out)
+  {
+    const int l = threadIdx.x; // one wave
+    double v[4];
+    for (int i = 0; i < 4; ++i) v[i] = in[i * 64 + l];
+    out[l] = dfrg::dev::wave_sum4(v);
+  }
 } // namespace
 
 extern "C" {
@@ -435,6 +442,26 @@ int kat_wave_sum8(const double *in, double *out)
     dfrg::check(dfrg_memcpy_h2d(d_in, in, sizeof(double) * 512, s), "in");
     hipLaunchKernelGGL(wave_sum8_kernel, dim3(1), dim3(64), 0, (hipStream_t)s, (const double *)d_in, (double *)d_out);
     dfrg::check_hip("wave_sum8_kernel");
+    dfrg::check(dfrg_memcpy_d2h(out, d_out, sizeof(double) * 64, s), "out");
+    dfrg::check(dfrg_stream_sync(s), "out");
+    dfrg_free(d_in);
+    dfrg_free(d_out);
+  });
+}
+
+// dfrg::dev::wave_sum4 on one wave: in holds 4 vectors of 64 lanes (vector-major), out the 64
+// lane results
+int kat_wave_sum4(const double *in, double *out)
+{
+  return guard([&] {
+    void *d_in = nullptr, *d_out = nullptr;
+    dfrg::check(dfrg_malloc(&d_in, sizeof(double) * 256), "in");
+    dfrg::check(dfrg_malloc(&d_out, sizeof(double) * 64), "out");
+    void *s = nullptr;
+    dfrg::check(dfrg_thread_stream(&s), "thread stream");
+    dfrg::check(dfrg_memcpy_h2d(d_in, in, sizeof(double) * 256, s), "in");
+    hipLaunchKernelGGL(wave_sum4_kernel, dim3(1), dim3(64), 0, (hipStream_t)s, (const double *)d_in, (double *)d_out);
+    dfrg::check_hip("wave_sum4_kernel");
     dfrg::check(dfrg_memcpy_d2h(out, d_out, sizeof(double) * 64, s), "out");
     dfrg::check(dfrg_stream_sync(s), "out");
     dfrg_free(d_in);
