@@ -10,7 +10,8 @@
 // division per node), read the inner nodes' coordinates and weight factors from a per-integrator
 // table, and add KERNEL::kernel(...) x (radial factor x inner factors) to an FP64 accumulator.
 // Blocks reduce with a wave64 DPP tree and LDS; the units of one integral meet in per-unit partial
-// slots whose last-arriving block adds them in unit order.  Units, their lane assignment and the
+// slots whose last-arriving block adds them with a fixed tree over the units (wave64 DPP, 64
+// units per round, rounds in unit order).  Units, their lane assignment and the
 // reduction order depend on the node counts only: results are bitwise reproducible, and a request
 // in any batch gives bitwise the value of get().  Launches keep as many blocks as fit on the device
 // resident and let them stride over the units.
@@ -341,8 +342,8 @@ namespace dfrg
     //   rows of fewer inner nodes: units of `rows` whole rows (about kRowMin nodes), their radial
     //     factors staged in LDS, walked flat.
     // A unit has a fixed block size and lane assignment and its own partial slot, and the integral
-    // is the sum of its unit partials in unit order, so how units are grouped into launches and
-    // blocks does not enter the result.
+    // is the sum of its unit partials by a tree fixed by the unit count, so how units are grouped
+    // into launches and blocks does not enter the result.
     // ------------------------------------------------------------------------------------------
     constexpr int kRowMin = 256, kMaxUnit = 65536;
     constexpr int kRowNodesPerLane = 16;
@@ -508,7 +509,7 @@ namespace dfrg
       if (p.upr == 1) {
         if (tid == 0) results[r] = Acc<NT>::value(tot);
       } else {
-        // units of the integral: publish, ticket; the last block adds the partials in unit order
+        // units of the integral: publish, ticket; the last block adds the partials (fixed tree)
         // and resets the counter
         double *slots = partials + (long long)r * p.upr * dev::kSlotStride;
         unsigned *counter = counters + (long long)r * dev::kCounterStride;
@@ -520,16 +521,18 @@ namespace dfrg
         }
         __syncthreads();
         if (s_last && tid < 64) {
+          // 64 partials per round, one per lane, added by the fixed wave tree (the rounds in unit
+          // order): a few dependent steps instead of a serial chain over the lanes, which had cost
+          // the launch microseconds at the 64 units of a row-mode integral
           double sum[NC];
 #pragma unroll
           for (int i = 0; i < NC; ++i) sum[i] = 0.0;
           for (int c0 = 0; c0 < p.upr; c0 += 64) {
             const int jj = c0 + tid;
-            const int mm = p.upr - c0 < 64 ? p.upr - c0 : 64;
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
               const double v = jj < p.upr ? dev::fetch(&slots[(long long)jj * dev::kSlotStride + i]) : 0.0;
-              for (int l = 0; l < mm; ++l) sum[i] += dev::lane_value(v, l);
+              sum[i] += dev::wave_sum(v);
             }
           }
           if (tid == 0) {
