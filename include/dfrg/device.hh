@@ -115,18 +115,6 @@ namespace dfrg
       return t;
     }
 
-    // Compensated (Neumaier) running sum in FP64.
-    struct KahanSum {
-      double s = 0.0, c = 0.0;
-      __device__ __host__ __forceinline__ void add(double v)
-      {
-        const double t = s + v;
-        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
-        s = t;
-      }
-      __device__ __host__ __forceinline__ double value() const { return s + c; }
-    };
-
     // ------------------------------------------------------------------------------------------
     // Cross-block publication without cache-wide fences.  Each XCD has its own L2, so a release
     // fence at device scope writes back the whole L2 and an acquire fence invalidates it, which
@@ -187,13 +175,13 @@ namespace dfrg
 
     // ------------------------------------------------------------------------------------------
     // Deterministic cross-block reduction.  Each block publishes its partial and takes a ticket;
-    // the block that takes the last ticket of a group sums all partials of that group in index
-    // order (compensated) and resets the group's counter so the next launch (or graph replay)
-    // starts from zero.  Slot s lives at partials[s * kSlotStride].  The finishing block's first
-    // wave fetches the partials 64 at a time, one
-    // per lane, so the fetches of a group cost one memory round trip per 64 slots instead of one
-    // per slot; the sum then runs over the lanes in slot order.  Returns true in the finishing
-    // block, with the group total in *total (valid in thread 0).
+    // the block that takes the last ticket of a group sums all partials of that group with a tree
+    // fixed by the group size and resets the group's counter so the next launch (or graph replay) starts from zero.
+    // Slot s lives at partials[s * kSlotStride].  The finishing block's first wave fetches the
+    // partials 64 at a time, one per lane, so the fetches of a group cost one memory round trip
+    // per 64 slots instead of one per slot, and adds each round with the wave's fixed DPP tree
+    // (rounds in slot order) rather than a serial chain over the lanes.  Returns true in the
+    // finishing block, with the group total in *total (valid in thread 0).
     // ------------------------------------------------------------------------------------------
     __device__ __forceinline__ bool finish_group(double partial, double *partials, int slot, int first_slot,
                                                  int n_slots, unsigned *counter, double *total)
@@ -207,15 +195,14 @@ namespace dfrg
       __syncthreads();
       if (!last) return false;
       if (threadIdx.x < 64) {
-        KahanSum s;
+        double t = 0.0;
         for (int c = 0; c < n_slots; c += 64) {
           const int j = c + (int)threadIdx.x;
           const double v = j < n_slots ? fetch(&partials[(long long)(first_slot + j) * kSlotStride]) : 0.0;
-          const int m = n_slots - c < 64 ? n_slots - c : 64;
-          for (int l = 0; l < m; ++l) s.add(lane_value(v, l));
+          t += wave_sum(v);
         }
         if (threadIdx.x == 0) {
-          *total = s.value();
+          *total = t;
           reset_counter(counter);
         }
       }
