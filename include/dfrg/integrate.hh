@@ -20,8 +20,9 @@
 // the KERNEL in FP32 on FP32 nodes and accumulate in FP64, the mixed form of the template path.
 //
 // get() is a batch of one, launched at once and waited for.  request() adds to the integrator's
-// open batch; a batch is launched (asynchronously: tuples to HBM, one kernel whose finishing blocks
-// store the results into pinned host memory, an event) when any future of this thread is read,
+// open batch; a batch is launched (asynchronously: one kernel that reads the tuples from pinned
+// host memory and whose finishing blocks store the results there, an event) when any future of
+// this thread is read,
 // when request_data() finishes its loop, when the batch is full, or when a request needs another
 // launch configuration.  Reading a future launches every open batch of the calling thread first,
 // so batches of different integrators run back to back before the first wait.
@@ -761,30 +762,29 @@ namespace dfrg
       }
 
     private:
-      // Enqueue on this thread's stream: tuples to HBM, the kernel (results into the pinned
-      // staging), the event.
+      // Enqueue on this thread's stream: the kernel, then the event.  The tuples and the results
+      // share the pinned staging: the kernel reads each unit's tuple over the host link instead of
+      // waiting for a copy to HBM (a copy costs the host ~3.5 us to enqueue and the device a blit
+      // launch, more than the units' reads add to the kernel), and its finishing blocks store the
+      // results there.
       void start()
       {
         launched = true;
         const int n = (int)args.size();
         try {
           const Plan &p = plan;
-          const size_t o_args = 0, o_part = up256(sizeof(Args) * n);
-          const size_t bytes = o_part + up256(sizeof(double) * dev::kSlotStride * (size_t)n * (p.upr > 1 ? p.upr : 0));
+          const size_t bytes = up256(sizeof(double) * dev::kSlotStride * (size_t)n * (p.upr > 1 ? p.upr : 0));
           Arena &A = thread_arena();
           A.ensure((size_t)n, bytes);
           const hipStream_t s = A.stream();
           const size_t h_res = up256(sizeof(Args) * n);
           check(dfrg_pool_host_alloc(&h, h_res + sizeof(NT) * n), "batch staging");
           std::memcpy(h, (const void *)args.data(), sizeof(Args) * n);
-          char *dd = A.data();
-          void *dh = nullptr; // the staging's device address: the finishing blocks store the results there
+          void *dh = nullptr; // the staging's device address
           stream = s;
-          bool ok = hipHostGetDevicePointer(&dh, h, 0) == hipSuccess &&
-                    hipMemcpyAsync(dd + o_args, h, sizeof(Args) * n, hipMemcpyHostToDevice, s) == hipSuccess;
-          const Args *d_args = (const Args *)(dd + o_args);
+          bool ok = hipHostGetDevicePointer(&dh, h, 0) == hipSuccess;
           if (ok) {
-            launch(d_args, n, p, (double *)(dd + o_part), A.counters(), (NT *)((char *)dh + h_res), s);
+            launch((const Args *)dh, n, p, (double *)A.data(), A.counters(), (NT *)((char *)dh + h_res), s);
             ok = hipGetLastError() == hipSuccess;
             if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
               ev = nullptr;
