@@ -691,7 +691,14 @@ namespace dfrg
     struct BatchBase {
       virtual ~BatchBase() = default;
       virtual void flush() = 0;
+      const void *type = nullptr; // identifies the derived batch type (see batch_type)
     };
+    // A distinct address per batch type, so the open batch's type is checked without RTTI.
+    template <class B> inline const void *batch_type()
+    {
+      static const char tag = 0;
+      return &tag;
+    }
 
     // Batches opened on this thread that are not launched yet.
     inline std::vector<std::weak_ptr<BatchBase>> &thread_open()
@@ -821,10 +828,16 @@ namespace dfrg
         bool full = false;
         {
           std::lock_guard<std::mutex> lock(m);
-          b = std::dynamic_pointer_cast<B>(open.lock());
+          {
+            std::shared_ptr<BatchBase> o = open.lock();
+            if (o && o->type == batch_type<B>()) b = std::static_pointer_cast<B>(o);
+          }
           for (int attempt = 0;; ++attempt) {
             if (!b || state != open_state || attempt > 0) {
               b = std::make_shared<B>();
+              b->type = batch_type<B>();
+              b->args.reserve(64); // a request_data grid without reallocations
+              b->constants.reserve(64);
               b->launch = make_launch();
               b->plan = plan;
               open = b;
