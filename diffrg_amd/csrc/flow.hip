@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <cmath>
 #include <deque>
 #include <map>
@@ -62,6 +63,7 @@ namespace
   struct FlowDev {
     int *status, *bad;    // per slot; status 0 idle, 1 running (host-owned)
     double *h, *T, *N;    // per slot: grid spacing, temperature, O(N)
+    double *m2_0, *lam;   // per slot: initial condition u(rho) = m2_0 + lam/2 rho (step 0 reads it)
     FlowRun *runs;        // every resident flow cut into runs
     int *n_runs;          // device copy of the run count
     int *base;            // global step index of the first step of this launch
@@ -129,9 +131,12 @@ namespace
       const double *u_old = F.pool + run.off + (step & 1) * n;
       double *u_new = F.pool + run.off + ((step + 1) & 1) * n;
       const int x0 = a - kHalo - 1; // grid point of LDS entry 0
+      // step 0 starts from the initial condition itself, so admission copies nothing to the device
+      const bool initial = step == 0;
+      const double m2_0 = F.m2_0[run.slot], half_lam = 0.5 * F.lam[run.slot];
       for (int i = tid; i < b - a + 2 * (kHalo + 1); i += kRunBlock) {
         const int x = x0 + i;
-        const double v = x >= 0 && x < n ? u_old[x] : 0.0;
+        const double v = x >= 0 && x < n ? (initial ? m2_0 + half_lam * (x * h) : u_old[x]) : 0.0;
         s_u[i] = v;
         s_y[i] = v;
       }
@@ -247,6 +252,13 @@ struct FlowRecord {
   std::vector<double> u;
 };
 
+// The per-slot parameters and the run count live in one device block with a pinned host mirror:
+// the host edits the mirror and uploads it with one asynchronous copy before a launch, and the
+// launch's non-finite flags come back into the mirror behind the graph.
+struct SlotLayout {
+  size_t n_runs = 0, status = 0, bad = 0, h = 0, T = 0, N = 0, m2_0 = 0, lam = 0, bytes = 0;
+};
+
 struct dfrg_fs {
   size_t budget = 0;
   int max_resident = 0, nx = 64, n_steps = 2000, steps_per_launch = 10;
@@ -273,6 +285,18 @@ struct dfrg_fs {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   long long launches = 0, point_steps = 0;
   double device_s = 0;
+  // slot block (device) and its pinned mirror; pinned staging of the run table and of the final u
+  // of the flows that end in a launch (copied back behind the graph)
+  SlotLayout sl;
+  char *d_slots = nullptr, *h_slots = nullptr;
+  bool slots_dirty = true;
+  FlowRun *h_runs = nullptr;
+  size_t h_runs_cap = 0;
+  int runs_upload = 0;
+  double *h_u = nullptr;
+  size_t h_u_cap = 0;
+  std::vector<std::pair<int, size_t>> u_staged; // (slot, offset in h_u) of this launch's finishers
+  template <class T> T *mirror(size_t off) { return (T *)(h_slots + off); }
 };
 
 namespace
@@ -330,8 +354,7 @@ namespace
   // retire finished / failed flows: copy u back, free their slot and HBM range
   int fs_retire(dfrg_fs *fs)
   {
-    std::vector<int> bad(fs->max_resident);
-    DFRG_HIP(hipMemcpy(bad.data(), fs->dev.bad, sizeof(int) * fs->max_resident, hipMemcpyDeviceToHost));
+    const int *bad = fs->mirror<int>(fs->sl.bad); // copied back behind the launch
     const double t = now_s();
     for (int s = 0; s < fs->max_resident; ++s) {
       if (fs->slot_flow[s] < 0) continue;
@@ -342,8 +365,15 @@ namespace
       r.u.resize(n);
       // u after the last step taken sits in the buffer of that step count's parity
       const int taken = std::min(fs->base - r.admit, fs->n_steps);
-      DFRG_HIP(hipMemcpy(r.u.data(), fs->dev.pool + r.off + (taken & 1) * n, sizeof(double) * n,
-                         hipMemcpyDeviceToHost));
+      const double *staged = nullptr;
+      if (!bad[s])
+        for (const auto &st : fs->u_staged)
+          if (st.first == s) staged = fs->h_u + st.second;
+      if (staged)
+        std::copy(staged, staged + n, r.u.begin());
+      else // a flow evicted as non-finite: its u after the last step taken
+        DFRG_HIP(hipMemcpy(r.u.data(), fs->dev.pool + r.off + (taken & 1) * n, sizeof(double) * n,
+                           hipMemcpyDeviceToHost));
       r.state = bad[s] ? 3 : 2;
       r.t_done = t;
       if (r.state == 2) {
@@ -351,8 +381,8 @@ namespace
         ++fs->finished;
       }
       fs_free_range(fs, r.off, 2 * n);
-      const int zero = 0;
-      DFRG_HIP(hipMemcpy(fs->dev.status + s, &zero, sizeof(int), hipMemcpyHostToDevice));
+      fs->mirror<int>(fs->sl.status)[s] = 0;
+      fs->slots_dirty = true;
       fs->slot_flow[s] = -1;
       r.slot = -1;
       fs->runs_dirty = true;
@@ -376,16 +406,17 @@ namespace
       const int off = fs_alloc_range(fs, 2 * n);
       if (off < 0) break; // does not fit the HBM budget now: wait for retirements (FIFO)
       fs->queue.pop_front();
-      std::vector<double> init(2 * n, 0.0); // u at step 0 (even buffer); odd buffer written by step 0
-      const double h = r.desc.rho_max / (n - 1);
-      for (int i = 0; i < n; ++i) init[i] = r.desc.m2_0 + r.desc.lambda / 2.0 * (i * h);
-      DFRG_HIP(hipMemcpy(fs->dev.pool + off, init.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
-      const int one = 1, zero = 0;
-      DFRG_HIP(hipMemcpy(fs->dev.bad + slot, &zero, sizeof(int), hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.h + slot, &h, sizeof(double), hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.T + slot, &r.desc.T, sizeof(double), hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.N + slot, &r.desc.N, sizeof(double), hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.status + slot, &one, sizeof(int), hipMemcpyHostToDevice));
+      // the slot's parameters go to the mirror; step 0 reads the initial condition from them, and
+      // writes the odd buffer, so the flow's HBM range needs no initialisation
+      const SlotLayout &L = fs->sl;
+      fs->mirror<int>(L.status)[slot] = 1;
+      fs->mirror<int>(L.bad)[slot] = 0;
+      fs->mirror<double>(L.h)[slot] = r.desc.rho_max / (n - 1);
+      fs->mirror<double>(L.T)[slot] = r.desc.T;
+      fs->mirror<double>(L.N)[slot] = r.desc.N;
+      fs->mirror<double>(L.m2_0)[slot] = r.desc.m2_0;
+      fs->mirror<double>(L.lam)[slot] = r.desc.lambda;
+      fs->slots_dirty = true;
       fs->slot_flow[slot] = id;
       r.slot = slot;
       r.off = off;
@@ -428,8 +459,20 @@ namespace
         set_error("internal: %d flow runs exceed the %zu allocated", R, fs->runs_cap);
         return DFRG_ERR_RUNTIME;
       }
-      if (R) DFRG_HIP(hipMemcpy(fs->dev.runs, runs.data(), sizeof(FlowRun) * R, hipMemcpyHostToDevice));
-      DFRG_HIP(hipMemcpy(fs->dev.n_runs, &R, sizeof(int), hipMemcpyHostToDevice));
+      // staged in pinned memory and uploaded before the next launch (no copy is in flight here:
+      // every launch ends with a synchronisation)
+      if ((size_t)R > fs->h_runs_cap) {
+        const size_t cap = std::max<size_t>((size_t)R, 2 * fs->h_runs_cap);
+        if (fs->h_runs) (void)hipHostFree(fs->h_runs);
+        fs->h_runs = nullptr;
+        fs->h_runs_cap = 0;
+        DFRG_HIP(hipHostMalloc((void **)&fs->h_runs, sizeof(FlowRun) * cap, hipHostMallocDefault));
+        fs->h_runs_cap = cap;
+      }
+      if (R) std::copy(runs.begin(), runs.end(), fs->h_runs);
+      fs->runs_upload = R;
+      *fs->mirror<int>(fs->sl.n_runs) = R;
+      fs->slots_dirty = true;
       fs->runs_dirty = false;
     }
     return DFRG_OK;
@@ -514,13 +557,38 @@ int dfrg_fs_create(dfrg_fs **out, dfrg_qp *qp, size_t hbm_budget, int max_reside
       return DFRG_ERR_HIP;                                                                                     \
     }                                                                                                          \
   } while (0)
-  FS_ALLOC(d.status, sizeof(int) * max_resident);
-  FS_ALLOC(d.bad, sizeof(int) * max_resident);
-  FS_ALLOC(d.h, sizeof(double) * max_resident);
-  FS_ALLOC(d.T, sizeof(double) * max_resident);
-  FS_ALLOC(d.N, sizeof(double) * max_resident);
+  {
+    SlotLayout &L = fs->sl;
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t M = (size_t)max_resident;
+    size_t o = 0;
+    L.n_runs = o, o += up(sizeof(int));
+    L.status = o, o += up(sizeof(int) * M);
+    L.bad = o, o += up(sizeof(int) * M);
+    L.h = o, o += up(sizeof(double) * M);
+    L.T = o, o += up(sizeof(double) * M);
+    L.N = o, o += up(sizeof(double) * M);
+    L.m2_0 = o, o += up(sizeof(double) * M);
+    L.lam = o, o += up(sizeof(double) * M);
+    L.bytes = o;
+  }
+  FS_ALLOC(fs->d_slots, fs->sl.bytes);
+  if (hipHostMalloc((void **)&fs->h_slots, fs->sl.bytes, hipHostMallocDefault) != hipSuccess) {
+    fs->h_slots = nullptr;
+    set_error("flow server: pinned allocation of %zu bytes failed", fs->sl.bytes);
+    dfrg_fs_destroy(fs);
+    return DFRG_ERR_HIP;
+  }
+  std::memset(fs->h_slots, 0, fs->sl.bytes);
+  d.n_runs = (int *)(fs->d_slots + fs->sl.n_runs);
+  d.status = (int *)(fs->d_slots + fs->sl.status);
+  d.bad = (int *)(fs->d_slots + fs->sl.bad);
+  d.h = (double *)(fs->d_slots + fs->sl.h);
+  d.T = (double *)(fs->d_slots + fs->sl.T);
+  d.N = (double *)(fs->d_slots + fs->sl.N);
+  d.m2_0 = (double *)(fs->d_slots + fs->sl.m2_0);
+  d.lam = (double *)(fs->d_slots + fs->sl.lam);
   FS_ALLOC(d.runs, runs_bytes);
-  FS_ALLOC(d.n_runs, sizeof(int));
   FS_ALLOC(d.base, sizeof(int));
   FS_ALLOC(fs->d_nodes, sizeof(double) * nodes.size());
   FS_ALLOC(d.pool, sizeof(double) * fs->pool_doubles);
@@ -554,10 +622,10 @@ int dfrg_fs_destroy(dfrg_fs *fs)
   if (fs->ev1) hipEventDestroy(fs->ev1);
   if (fs->stream) hipStreamDestroy(fs->stream);
   FlowDev &d = fs->dev;
-  for (void *p : {(void *)d.status, (void *)d.bad, (void *)d.h,
-                  (void *)d.T, (void *)d.N, (void *)d.runs, (void *)d.n_runs, (void *)d.base, (void *)d.pool,
-                  (void *)fs->d_nodes})
+  for (void *p : {(void *)fs->d_slots, (void *)d.runs, (void *)d.base, (void *)d.pool, (void *)fs->d_nodes})
     if (p) hipFree(p);
+  for (void *p : {(void *)fs->h_slots, (void *)fs->h_runs, (void *)fs->h_u})
+    if (p) hipHostFree(p);
   delete fs;
   return DFRG_OK;
 }
@@ -596,10 +664,51 @@ int dfrg_fs_progress(dfrg_fs *fs, int *resident, int *queued, int *done)
         const int left = fs->n_steps - (fs->base - r.admit);
         fs->point_steps += (long long)r.desc.n_grid * std::max(0, std::min(left, fs->steps_per_launch));
       }
+    // pinned staging for the final u of the flows that take their last step in this launch (no
+    // copy is in flight here: the previous launch ended with a synchronisation)
+    size_t need = 0;
+    for (long long id : fs->slot_flow)
+      if (id >= 0) {
+        const FlowRecord &r = fs->flows[id];
+        if (fs->base + fs->steps_per_launch - r.admit >= fs->n_steps) need += r.desc.n_grid;
+      }
+    if (need > fs->h_u_cap) {
+      const size_t cap = std::max(need, 2 * fs->h_u_cap);
+      if (fs->h_u) (void)hipHostFree(fs->h_u);
+      fs->h_u = nullptr;
+      fs->h_u_cap = 0;
+      DFRG_HIP(hipHostMalloc((void **)&fs->h_u, sizeof(double) * cap, hipHostMallocDefault));
+      fs->h_u_cap = cap;
+    }
     const double t0 = now_s();
+    // the slot block and run table go up asynchronously from their pinned staging
+    if (fs->slots_dirty) {
+      DFRG_HIP(hipMemcpyAsync(fs->d_slots, fs->h_slots, fs->sl.bytes, hipMemcpyHostToDevice, fs->stream));
+      fs->slots_dirty = false;
+    }
+    if (fs->runs_upload > 0)
+      DFRG_HIP(hipMemcpyAsync(fs->dev.runs, fs->h_runs, sizeof(FlowRun) * fs->runs_upload, hipMemcpyHostToDevice,
+                              fs->stream));
+    fs->runs_upload = 0;
     DFRG_HIP(hipEventRecord(fs->ev0, fs->stream));
     DFRG_HIP(hipGraphLaunch(fs->graph_exec, fs->stream));
     DFRG_HIP(hipEventRecord(fs->ev1, fs->stream));
+    // behind the graph: the non-finite flags into the mirror, and the finishers' final u
+    DFRG_HIP(hipMemcpyAsync(fs->h_slots + fs->sl.bad, fs->dev.bad, sizeof(int) * fs->max_resident,
+                            hipMemcpyDeviceToHost, fs->stream));
+    fs->u_staged.clear();
+    size_t o = 0;
+    for (int sl = 0; sl < fs->max_resident; ++sl) {
+      const long long id = fs->slot_flow[sl];
+      if (id < 0) continue;
+      const FlowRecord &r = fs->flows[id];
+      if (fs->base + fs->steps_per_launch - r.admit < fs->n_steps) continue;
+      const int n = r.desc.n_grid; // u after the last step sits in the buffer of n_steps' parity
+      DFRG_HIP(hipMemcpyAsync(fs->h_u + o, fs->dev.pool + r.off + (fs->n_steps & 1) * n, sizeof(double) * n,
+                              hipMemcpyDeviceToHost, fs->stream));
+      fs->u_staged.push_back({sl, o});
+      o += n;
+    }
     DFRG_HIP(hipStreamSynchronize(fs->stream));
     fs->base += fs->steps_per_launch;
     fs->busy_s += now_s() - t0;
