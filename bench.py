@@ -463,8 +463,10 @@ def template(D, qp, args):
     user KERNEL (LinearInterpolator1D dressing), its 64 requests made as request_data makes them and
     read back.  The C++ side is the template test library (tests/native/libdfrg_kat.so, a user of
     the drop-in headers).  Device time: HIP events on the request thread's stream around the batch's
-    launch and the reads (tuples to HBM, the kernel, results back); host time: the whole call set
-    (64 requests, launch, reads)."""
+    launch and the reads (the kernel reads the tuples from pinned host memory and stores the results
+    there); host time: the whole call set (64 requests, launch, reads).  Also timed: the same 64
+    integrals as 64 get() calls, one launch and one synchronisation each, the reference's per-call
+    pattern."""
     import ctypes as C
     import diffrg_amd.workloads as W
     from oracle import oracle as O
@@ -502,6 +504,19 @@ def template(D, qp, args):
                    "us_phases": {"host_requests": ph[0] * 1e6, "host_launch": ph[1] * 1e6, "host_reads": ph[2] * 1e6,
                                  "device_launch_to_enqueued": ph[3] * 1e6, "device_enqueued_to_read": ph[4] * 1e6}}
         log(f"template {name}: {t * 1e6:.1f} us device, {th.value * 1e6:.1f} us host per call set, err {err:.2e}")
+    # the same integrals one get() at a time (FP64)
+    out, tg = np.empty(n), C.c_double()
+    pv = np.ascontiguousarray(pg, dtype=np.float64)
+    rc = kat.kat_selfenergy_get_bench(c["nq"], c["na"], C.c_double(c["x_extent"]), C.c_double(c["k"]), P(pv), n,
+                                      P(Z), len(Z), C.c_double(c["coord"][0]), C.c_double(c["coord"][1]),
+                                      C.c_double(c["coord"][2]), 5, P(out), C.byref(tg))
+    if rc != 0:
+        raise RuntimeError(kat.kat_last_error())
+    ref = O.c1_selfenergy(pv, [c["Z"]] * n, [coord] * n, [xq] * n, [xa] * n, c["x_extent"], c["k"])
+    r["fp64_get_per_integral"] = {"us_per_get": tg.value * 1e6, "integrals_per_s": 1.0 / tg.value,
+                                  "max_rel_err_vs_fp64_oracle": rel_err(out, ref),
+                                  "pattern": "64 get() calls, each one launch and one synchronisation"}
+    log(f"template fp64 get(): {tg.value * 1e6:.1f} us per integral")
     return r
 
 

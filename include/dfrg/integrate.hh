@@ -876,11 +876,10 @@ namespace dfrg
       const Plan p = plan_of<S>(n);
       const NT constant = KERNEL::constant(k, t...);
       Arena &A = thread_arena();
-      const size_t o_part = up256(sizeof(Args));
-      A.ensure(1, o_part + up256(sizeof(double) * dev::kSlotStride * (p.upr > 1 ? p.upr : 0)));
+      A.ensure(1, up256(sizeof(double) * dev::kSlotStride * (p.upr > 1 ? p.upr : 0)));
       const hipStream_t s = A.stream();
-      // pinned staging of this thread: the tuple, then the result, which the finishing block
-      // stores through the staging's device address
+      // pinned staging of this thread: the tuple, which the kernel reads from there, then the
+      // result, which the finishing block stores through the staging's device address
       thread_local void *h = nullptr, *dh = nullptr;
       thread_local size_t h_size = 0;
       const size_t h_res = up256(sizeof(Args)), need = h_res + sizeof(NT);
@@ -897,11 +896,8 @@ namespace dfrg
         h_size = need;
       }
       const Args args(k, t...);
-      std::memcpy(h, (const void *)&args, sizeof(Args));
-      char *dd = A.data();
-      if (hipMemcpyAsync(dd, h, sizeof(Args), hipMemcpyHostToDevice, s) != hipSuccess)
-        throw std::runtime_error("integrator: argument copy failed");
-      launch_batch<S, NT, KERNEL>(n, p, (const Args *)dd, 1, (double *)(dd + o_part), A.counters(),
+      std::memcpy(h, (const void *)&args, sizeof(Args)); // the previous get() has finished with h
+      launch_batch<S, NT, KERNEL>(n, p, (const Args *)dh, 1, (double *)A.data(), A.counters(),
                                   (NT *)((char *)dh + h_res), s);
       check_hip("integrator kernel launch");
       if (hipStreamSynchronize(s) != hipSuccess) throw std::runtime_error("integrator: stream error");

@@ -690,6 +690,28 @@ int kat_selfenergy_get(int nq, int na, double x_extent, double k, const double *
   });
 }
 
+// the same call set, timed: reps rounds of n_p get() calls after one untimed round; *t_call is the
+// median over the rounds of the wall time per get() (each is a launch and a synchronisation)
+int kat_selfenergy_get_bench(int nq, int na, double x_extent, double k, const double *p, int n_p, const double *Z,
+                             int n_tab, double start, double stop, double bias, int reps, double *out,
+                             double *t_call)
+{
+  return guard([&] {
+    LogarithmicCoordinates1D<double> coords((uint)n_tab, start, stop, bias);
+    ZInterp interp(Z, coords);
+    Integrator4D2AngGPU<double, SelfEnergyKernel> integ(provider(), {{(uint)nq, (uint)na, (uint)na}}, x_extent);
+    std::vector<double> per;
+    for (int r = 0; r <= reps; ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int i = 0; i < n_p; ++i) out[i] = integ.get(k, p[i], interp);
+      const auto t1 = std::chrono::steady_clock::now();
+      if (r > 0) per.push_back(std::chrono::duration<double>(t1 - t0).count() / n_p);
+    }
+    std::sort(per.begin(), per.end());
+    *t_call = per[per.size() / 2];
+  });
+}
+
 // request_data / update_data and get_data over the external momenta of a log grid
 int kat_request_data(int nq, int na, double x_extent, double k, int n_p, double p_start, double p_stop,
                      double p_bias, const double *Z, int n_tab, double start, double stop, double bias,
