@@ -9,9 +9,11 @@ accumulation), accepted only when its relative error against the FP64 oracle is 
 is one evaluation of the whole call set with inputs resident in HBM; K steps are timed between
 two HIP events on the stream the kernels run on.  The plain FP64 path is timed in the same run.
 
-The same run also measures the other workloads (reported under "workloads"): LPA threshold
-integrals, finite-T integrals, the uneven 4096-integral persistent batch (hipGraph-captured) and
-the flow server under Poisson arrivals (p50/p99 latency), each checked against the oracle.
+The same run also measures the other workloads (reported under "workloads"): the same call set
+through the drop-in template API (Integrator4D2AngGPU + request_data, and as 64 get() calls), LPA
+threshold integrals and finite-T integrals (launched one by one and from a hipGraph of 10 call
+sets), the uneven 4096-integral persistent batch (hipGraph-captured) and the flow server under
+Poisson arrivals (p50/p99 latency), each checked against the oracle.
 """
 from __future__ import annotations
 
