@@ -133,7 +133,9 @@ namespace dfrg
       const AngNode<ctype> *ang = nullptr;          // angular tables, one per angular digit
       const ctype *mx = nullptr, *mw = nullptr;     // Matsubara rule (positive nodes)
       int nx = 0, na = 0, nm = 0;
+      int nb = 0;                                   // Cartesian: nodes of the third axis
       ctype x_extent = 0, m_T = 0;
+      ctype q_min = 0, q_ext = 0;                   // Cartesian: range of the first axis
     };
 
     template <class ctype> constexpr ctype two_pi = ctype(2) * ctype(3.14159265358979323846);
@@ -287,6 +289,89 @@ namespace dfrg
         return r;
       }
     };
+
+    // Cartesian momentum integrals over a box, int d^Dq/(2 pi)^D f(qx[, qy[, qz]]): every axis is a
+    // Gauss-Legendre rule on [0,1] mapped linearly, q = q_min + q_ext x with weight q_ext w.  The
+    // first axis takes the radial digit's place (Nodes::q_min, q_ext); qy and qz come from tables
+    // (coordinate, weight) of ny = na and nz = nb entries, qz's after qy's.
+    // Integrator1DCartesianGPU (integrator_1D_cartesian_gpu.hh:36-60): (qx)
+    struct Cartesian1 {
+      static constexpr int ND = 1, NA = 0;
+      static constexpr bool cartesian = true;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND]) { r[0] = n.nx; }
+      template <class C> static __host__ __device__ __forceinline__ C element(C, C) { return C(1) / two_pi<C>; }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &, C q, C rf, const int (&)[ND], C k, const T &...t)
+      {
+        return NT(rf * KERNEL::kernel(q, k, t...));
+      }
+    };
+    // Integrator2DCartesianGPU (integrator_2D_cartesian_gpu.hh:21-45): (qx, qy)
+    struct Cartesian2 {
+      static constexpr int ND = 2, NA = 1;
+      static constexpr bool cartesian = true;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = n.na;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C, C)
+      {
+        return C(1) / powr<2>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> y = gld(n.ang + g[1]);
+        return NT((rf * y.a) * KERNEL::kernel(q, y.c, k, t...));
+      }
+    };
+    // Integrator3DCartesianGPU (integrator_3D_cartesian_gpu.hh:20-46): (qx, qy, qz)
+    struct Cartesian3 {
+      static constexpr int ND = 3, NA = 2;
+      static constexpr bool cartesian = true;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = n.na;
+        r[2] = n.nb;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C, C)
+      {
+        return C(1) / powr<3>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> y = gld(n.ang + g[1]), z = gld(n.ang + n.na + g[2]);
+        return NT((rf * (y.a * z.a)) * KERNEL::kernel(q, y.c, z.c, k, t...));
+      }
+    };
+
+    // Radial node ix of shape S: its momentum q and factor rf (element times weight).  Cartesian
+    // shapes map their first axis linearly; the others take q = k sqrt(x x_ext) and the factor
+    // element(q, k) w x_ext.
+    template <class S, class C>
+    __device__ __forceinline__ void radial_node(const Nodes<C> &n, int ix, C k, C &q, C &rf)
+    {
+      if constexpr (requires { S::cartesian; }) {
+        q = n.q_min + n.q_ext * n.xq[ix];
+        rf = S::element(q, k) * (n.q_ext * n.wq[ix]);
+      } else {
+        q = k * sqrt(n.xq[ix] * n.x_extent);
+        rf = S::element(q, k) * (n.wq[ix] * n.x_extent);
+      }
+    }
+
+    // A Cartesian axis table {q_min + q_ext x_i, q_ext w_i} from the rule on [0,1].
+    template <class ctype>
+    std::vector<AngNode<ctype>> cartesian_table(const std::vector<ctype> &x, const std::vector<ctype> &w, ctype q_min,
+                                                ctype q_ext)
+    {
+      std::vector<AngNode<ctype>> t(x.size());
+      for (size_t i = 0; i < x.size(); ++i) t[i] = {q_min + q_ext * x[i], q_ext * w[i]};
+      return t;
+    }
 
     // Angular tables of a shape from the Gauss-Legendre rule on [0,1] of order na (host nodes):
     //   cos = 2(x - 1/2), weight 2w; cos1 of the 4D forms also carries the Jacobian sqrt(1 - cos1^2);
@@ -455,8 +540,8 @@ namespace dfrg
         {
           const int ix = u / p.parts, part = u - ix * p.parts;
           const int m0 = part * p.unit, m1 = m0 + p.unit < p.ni ? m0 + p.unit : p.ni;
-          const C q = k * sqrt(n.xq[ix] * n.x_extent);
-          const C rf = S::element(q, k) * (n.wq[ix] * n.x_extent);
+          C q, rf;
+          radial_node<S>(n, ix, k, q, rf);
           auto eval = [&](const NodeWalk<S::ND - 1, int> &w) {
             int g[S::ND];
             g[0] = ix;
@@ -476,9 +561,10 @@ namespace dfrg
       if constexpr (!ROW || S::ND == 1) {
         const int ix0 = u * p.rows, ix1 = ix0 + p.rows < p.nx ? ix0 + p.rows : p.nx;
         for (int j = tid; j < ix1 - ix0; j += B) {
-          const C q = k * sqrt(n.xq[ix0 + j] * n.x_extent);
+          C q, rf;
+          radial_node<S>(n, ix0 + j, k, q, rf);
           s_q[j] = q;
-          s_rf[j] = S::element(q, k) * (n.wq[ix0 + j] * n.x_extent);
+          s_rf[j] = rf;
         }
         __syncthreads();
         const long long f1 = (long long)ix1 * p.ni;
@@ -906,16 +992,24 @@ namespace dfrg
       return constant + v;
     }
 
+    // hold: device memory the nodes point into that the integrator may replace (the Cartesian axis
+    // tables after a range change), kept alive by the batch until it is released
+    template <class S, class NT, class KERNEL, class C, class... T>
+    std::future<NT> request_holding(RequestBatcher<NT> &batcher, unsigned long long state,
+                                    std::shared_ptr<const void> hold, const Nodes<C> &n, C k, const T &...t)
+    {
+      using Args = std::tuple<C, T...>;
+      return batcher.enqueue(state, plan_of<S>(n), KERNEL::constant(k, t...), Args(k, t...), [n, hold]() {
+        return [n, hold](const Args *a, int r, const Plan &p, double *part, unsigned *cnt, NT *res, hipStream_t s) {
+          launch_batch<S, NT, KERNEL>(n, p, a, r, part, cnt, res, s);
+        };
+      });
+    }
     template <class S, class NT, class KERNEL, class C, class... T>
     std::future<NT> request(RequestBatcher<NT> &batcher, unsigned long long state, const Nodes<C> &n, C k,
                             const T &...t)
     {
-      using Args = std::tuple<C, T...>;
-      return batcher.enqueue(state, plan_of<S>(n), KERNEL::constant(k, t...), Args(k, t...), [n]() {
-        return [n](const Args *a, int r, const Plan &p, double *part, unsigned *cnt, NT *res, hipStream_t s) {
-          launch_batch<S, NT, KERNEL>(n, p, a, r, part, cnt, res, s);
-        };
-      });
+      return request_holding<S, NT, KERNEL>(batcher, state, nullptr, n, k, t...);
     }
   } // namespace tmpl
 } // namespace dfrg

@@ -583,6 +583,44 @@ double orc_kat_angle(int d, const double *xq, const double *wq, int nq, const do
   return constant + value;
 }
 
+static inline double poly4c(const double *c, double v) { return c[0] + v * (c[1] + v * (c[2] + v * c[3])); }
+
+double orc_kat_cartesian(int D, const double *const *x, const double *const *w, const int *n, const double *lo,
+                         const double *hi, const double *c, double constant)
+{
+  /* integrator_{1,2,3}D_cartesian_cpu.hh (e.g. 3D: lines 98-121): q_i = q_min_i + q_ext_i x_i,
+   * weight prod_i q_ext_i w_i, factor (2pi)^-D; integrand prod_i cubic_i(q_i), 4 coefficients per axis */
+  const int ny = D > 1 ? n[1] : 1, nz = D > 2 ? n[2] : 1;
+  const double el = 1.0 / ipow(2.0 * M_PI, D);
+  double value = 0.0;
+  for (int ix = 0; ix < n[0]; ++ix) {
+    const double ex = hi[0] - lo[0], qx = lo[0] + ex * x[0][ix], wx = ex * w[0][ix];
+    double part = 0.0;
+    for (int iy = 0; iy < ny; ++iy) {
+      double qy = 0.0, wy = 1.0;
+      if (D > 1) {
+        const double ey = hi[1] - lo[1];
+        qy = lo[1] + ey * x[1][iy];
+        wy = ey * w[1][iy];
+      }
+      for (int iz = 0; iz < nz; ++iz) {
+        double qz = 0.0, wz = 1.0;
+        if (D > 2) {
+          const double ez = hi[2] - lo[2];
+          qz = lo[2] + ez * x[2][iz];
+          wz = ez * w[2][iz];
+        }
+        double f = poly4c(c, qx);
+        if (D > 1) f *= poly4c(c + 4, qy);
+        if (D > 2) f *= poly4c(c + 8, qz);
+        part += el * (wz * wy * wx) * f;
+      }
+    }
+    value += part;
+  }
+  return constant + value;
+}
+
 static inline double poly5(const double *c, double v)
 {
   return c[0] + c[1] * v + c[2] * v * v + c[3] * v * v * v + c[4] * v * v * v * v;

@@ -113,6 +113,11 @@ double orc_kat_finiteT(int d, const double *x, const double *w, int nx, double x
 /* angle: poly(q) poly1(cos) in d dims (integrator_angle_cpu.cc, integrator_angle_cpu.hh:60-80) */
 double orc_kat_angle(int d, const double *xq, const double *wq, int nq, const double *xa, const double *wa, int na,
                      double x_extent, double k, const double *c6, const double *a4, double constant);
+/* Cartesian box in D = 1..3 dims: prod_i cubic_i(q_i) over [lo_i, hi_i], factor (2pi)^-D
+ * (integrator_{1,2,3}D_cartesian_cpu.cc, integrator_3D_cartesian_cpu.hh:98-121); x[i], w[i]: the
+ * rule of axis i on [0,1] with n[i] nodes; c: 4 coefficients per axis */
+double orc_kat_cartesian(int D, const double *const *x, const double *const *w, const int *n, const double *lo,
+                         const double *hi, const double *c, double constant);
 /* 4D finite-T q0: poly(q) poly1(cos) poly2(phi) / poly5(q0) (integrator_4D_finiteTq0_cpu.cc:14-40,
  * integrator_4D_finiteTq0_cpu.hh:90-120); equal cos and phi orders */
 double orc_kat_4d_finiteT(const double *xq, const double *wq, int nq, const double *xa, const double *wa, int na,

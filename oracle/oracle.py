@@ -92,6 +92,9 @@ def lib():
         L.orc_kat_angle.restype = C.c_double
         L.orc_kat_angle.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp,
                                     C.c_double]
+        L.orc_kat_cartesian.restype = C.c_double
+        L.orc_kat_cartesian.argtypes = [C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(C.c_int), _dp, _dp, _dp,
+                                        C.c_double]
         L.orc_kat_4d_finiteT.restype = C.c_double
         L.orc_kat_4d_finiteT.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp,
                                          C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_double]
@@ -287,6 +290,19 @@ def kat_angle(d, xq, wq, xa, wa, x_extent, k, c6, a4, constant):
     xq, wq, xa, wa, c6, a4 = (_f64(v) for v in (xq, wq, xa, wa, c6, a4))
     return lib().orc_kat_angle(d, _p(xq), _p(wq), len(xq), _p(xa), _p(wa), len(xa), x_extent, k, _p(c6), _p(a4),
                                constant)
+
+
+def kat_cartesian(rules, lo, hi, c, constant):
+    """rules: [(x, w)] per axis (1 to 3); c: 4 cubic coefficients per axis."""
+    D = len(rules)
+    xs = [_f64(r[0]) for r in rules]
+    ws = [_f64(r[1]) for r in rules]
+    xp = (_dp * D)(*[_p(v) for v in xs])
+    wp = (_dp * D)(*[_p(v) for v in ws])
+    n = (C.c_int * D)(*[len(v) for v in xs])
+    lo, hi, c = _f64(lo), _f64(hi), _f64(c)
+    assert len(lo) == D and len(hi) == D and len(c) == 4 * D
+    return lib().orc_kat_cartesian(D, xp, wp, n, _p(lo), _p(hi), _p(c), constant)
 
 
 def kat_4d_finiteT(xq, wq, xa, wa, x_extent, k, mx, mw, T, c6, a4, p4, d5, constant):

@@ -17,6 +17,12 @@
 #include <DiFfRG/physics/thermodynamics.hh>
 
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
+#include <DiFfRG/physics/integration/integrator_1D_cartesian_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_1D_cartesian_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_2D_cartesian_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_2D_cartesian_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_3D_cartesian_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_3D_cartesian_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_2ang_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_2ang_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_cpu.hh>
@@ -100,6 +106,22 @@ namespace
     static __host__ __device__ double constant(double, double) { return 0.; }
   };
 
+  struct KC1 {
+    static __host__ __device__ double kernel(double qx, double k, double a) { return a * qx * k; }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct KC2 {
+    static __host__ __device__ double kernel(double qx, double qy, double k, double a) { return a * qx * qy * k; }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct KC3 {
+    static __host__ __device__ double kernel(double qx, double qy, double qz, double k, double a)
+    {
+      return a * qx * qy * qz * k;
+    }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+
   // a wrapper of the shape the generated X_integrator classes have (templated request<NT>)
   template <class I> struct Wrapper {
     I &integ;
@@ -126,6 +148,21 @@ double instantiate_with_reference_json(const ReferenceJSONValue &json)
   Integrator4DTBB<double, K4D> dt(qp, {{32, 8, 8, 8}}, 20., json);
   IntegratorFiniteTq0TBB<4, double, KT> et(qp, {{32}}, 20., json);
   s += at.get(1., 2.) + bt.get(1., 2.) + ct.get(1., 2.) + dt.get(1., 2.) + et.get(1., 2.);
+  // Cartesian forms: JSON constructors (ranges from /discretization/integration/q?_{min,max}) and
+  // the range mutators
+  Integrator1DCartesianGPU<double, KC1> h1(qp, {{32}}, json);
+  Integrator1DCartesianGPU<double, KC1> h1x(qp, {{32}}, 0., json);
+  Integrator2DCartesianGPU<double, KC2> h2(qp, {{32, 16}}, 0., json);
+  Integrator3DCartesianGPU<double, KC3> h3(qp, {{16, 16, 8}}, 0., json);
+  Integrator1DCartesianTBB<double, KC1> h1t(qp, {{32}}, json);
+  Integrator2DCartesianTBB<double, KC2> h2t(qp, {{32, 16}}, 0., json);
+  Integrator3DCartesianTBB<double, KC3> h3t(qp, {{16, 16, 8}}, 0., json);
+  h1.set_qx_min(-1.);
+  h2.set_qy_max(1.);
+  h3.set_qz_min(-2.);
+  h3t.set_qz_max(2.);
+  s += h1.get(1., 2.) + h1x.request(1., 2.).get() + h2.get(1., 2.) + h3.request(1., 2.).get();
+  s += h1t.get(1., 2.) + h2t.request(1., 2.).get() + h3t.get(1., 2.);
   // request_data over a coordinate grid, through a generated-wrapper-shaped object
   LogarithmicCoordinates1D<double> grid(16, 1e-3, 10., 3.);
   IntegratorGPU<4, double, K1p> ap(qp, {{64}}, 20., json);

@@ -9,6 +9,12 @@
 #include <DiFfRG/common/math.hh>
 #include <DiFfRG/common/quadrature/quadrature_provider.hh>
 #include <DiFfRG/discretization/grid/coordinates.hh>
+#include <DiFfRG/physics/integration/integrator_1D_cartesian_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_1D_cartesian_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_2D_cartesian_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_2D_cartesian_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_3D_cartesian_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_3D_cartesian_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_2ang_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_2ang_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_cpu.hh>
@@ -296,6 +302,87 @@ namespace
                                                              const double, const double, const double, const double,
                                                              const double, const double, const double, const double,
                                                              const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+
+  // integrator_1D_cartesian_gpu.cu:16-31: a cubic in qx
+  __forceinline__ __host__ __device__ double cubic(double v, double c0, double c1, double c2, double c3)
+  {
+    return c0 + c1 * powr<1>(v) + c2 * powr<2>(v) + c3 * powr<3>(v);
+  }
+  struct PolyCart1 {
+    static __forceinline__ __host__ __device__ auto kernel(const double qx, const double, const double,
+                                                           const double x0, const double x1, const double x2,
+                                                           const double x3)
+    {
+      return cubic(qx, x0, x1, x2, x3);
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double)
+    {
+      return c;
+    }
+  };
+  // complex-valued and dual-valued variants: (1 + i/2) cubic and (cubic, cubic/4)
+  struct PolyCart1C {
+    static __forceinline__ __host__ __device__ complex<double> kernel(const double qx, const double, const double,
+                                                                      const double x0, const double x1,
+                                                                      const double x2, const double x3)
+    {
+      const double v = cubic(qx, x0, x1, x2, x3);
+      return complex<double>(v, 0.5 * v);
+    }
+    static __forceinline__ __host__ __device__ complex<double> constant(const double, const double c, const double,
+                                                                        const double, const double, const double)
+    {
+      return complex<double>(c, 0.0);
+    }
+  };
+  struct PolyCart1D {
+    static __forceinline__ __host__ __device__ Dual1 kernel(const double qx, const double, const double,
+                                                            const double x0, const double x1, const double x2,
+                                                            const double x3)
+    {
+      const double v = cubic(qx, x0, x1, x2, x3);
+      return Dual1(v, 0.25 * v);
+    }
+    static __forceinline__ __host__ __device__ Dual1 constant(const double, const double c, const double,
+                                                              const double, const double, const double)
+    {
+      return Dual1(c, 0.0);
+    }
+  };
+  // integrator_2D_cartesian_gpu.cu:14-31
+  struct PolyCart2 {
+    static __forceinline__ __host__ __device__ auto kernel(const double qx, const double qy, const double,
+                                                           const double, const double x0, const double x1,
+                                                           const double x2, const double x3, const double y0,
+                                                           const double y1, const double y2, const double y3)
+    {
+      return cubic(qx, x0, x1, x2, x3) * cubic(qy, y0, y1, y2, y3);
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double)
+    {
+      return c;
+    }
+  };
+  // integrator_3D_cartesian_gpu.cu:14-36
+  struct PolyCart3 {
+    static __forceinline__ __host__ __device__ auto
+    kernel(const double qx, const double qy, const double qz, const double, const double, const double x0,
+           const double x1, const double x2, const double x3, const double y0, const double y1, const double y2,
+           const double y3, const double z0, const double z1, const double z2, const double z3)
+    {
+      return cubic(qx, x0, x1, x2, x3) * cubic(qy, y0, y1, y2, y3) * cubic(qz, z0, z1, z2, z3);
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double)
     {
       return c;
     }
@@ -937,6 +1024,184 @@ int kat_angle(int device, int d, int nx, int na, double x_extent, double k, cons
     case 8: run(IntegratorAngleTBB<4, double, PolyAngle>(provider(), g, x_extent)); break;
     case 9: run(IntegratorAngleGPU<4, double, PolyAngle>(provider(), g, x_extent)); break;
     default: throw std::invalid_argument("unsupported dimension");
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cartesian integrators (integrator_{1,2,3}D_cartesian_{cpu,gpu}.hh)
+// ---------------------------------------------------------------------------------------------
+
+} // extern "C"
+
+namespace
+{
+  // I.get / I.request of the cubic product with coefficients c (4 per axis)
+  template <int D, class I> auto cart_call(I &integ, bool req, double k, double constant, const double *c)
+  {
+    if constexpr (D == 1) {
+      return req ? integ.request(k, constant, c[0], c[1], c[2], c[3]).get() : integ.get(k, constant, c[0], c[1], c[2], c[3]);
+    } else if constexpr (D == 2) {
+      return req ? integ.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]).get()
+                 : integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]);
+    } else {
+      return req ? integ.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9], c[10], c[11])
+                       .get()
+                 : integ.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9], c[10], c[11]);
+    }
+  }
+  template <int D, class I> auto cart_request(I &integ, double k, double constant, const double *c)
+  {
+    if constexpr (D == 1)
+      return integ.request(k, constant, c[0], c[1], c[2], c[3]);
+    else if constexpr (D == 2)
+      return integ.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]);
+    else
+      return integ.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9], c[10], c[11]);
+  }
+  // The range of every axis through the mutators: set_q?_min first (max_first: set_q?_max first).
+  template <int D, class I> void cart_set(I &integ, const double *lo, const double *hi, bool max_first)
+  {
+    auto one = [&](auto set_min, auto set_max, int i) {
+      if (max_first) {
+        set_max(hi[i]);
+        set_min(lo[i]);
+      } else {
+        set_min(lo[i]);
+        set_max(hi[i]);
+      }
+    };
+    one([&](double v) { integ.set_qx_min(v); }, [&](double v) { integ.set_qx_max(v); }, 0);
+    if constexpr (D > 1) one([&](double v) { integ.set_qy_min(v); }, [&](double v) { integ.set_qy_max(v); }, 1);
+    if constexpr (D > 2) one([&](double v) { integ.set_qz_min(v); }, [&](double v) { integ.set_qz_max(v); }, 2);
+  }
+  template <int D, template <class, class> class G, template <class, class> class H, class K>
+  struct CartForms {
+    using Gpu = G<double, K>;
+    using Host = H<double, K>;
+  };
+  template <int D> struct CartOf;
+  template <> struct CartOf<1> : CartForms<1, Integrator1DCartesianGPU, Integrator1DCartesianTBB, PolyCart1> {
+    static Gpu gpu(const int *n, const double *lo, const double *hi)
+    {
+      return Gpu(provider(), {{(uint)n[0]}}, 0., 256, lo[0], hi[0]);
+    }
+    static Host host(const int *n, const double *lo, const double *hi)
+    {
+      return Host(provider(), {{(uint)n[0]}}, 0., 0, lo[0], hi[0]);
+    }
+  };
+  template <> struct CartOf<2> : CartForms<2, Integrator2DCartesianGPU, Integrator2DCartesianTBB, PolyCart2> {
+    static Gpu gpu(const int *n, const double *lo, const double *hi)
+    {
+      return Gpu(provider(), {{(uint)n[0], (uint)n[1]}}, 0., 256, lo[0], lo[1], hi[0], hi[1]);
+    }
+    static Host host(const int *n, const double *lo, const double *hi)
+    {
+      return Host(provider(), {{(uint)n[0], (uint)n[1]}}, 0., 0, lo[0], lo[1], hi[0], hi[1]);
+    }
+  };
+  template <> struct CartOf<3> : CartForms<3, Integrator3DCartesianGPU, Integrator3DCartesianTBB, PolyCart3> {
+    static Gpu gpu(const int *n, const double *lo, const double *hi)
+    {
+      return Gpu(provider(), {{(uint)n[0], (uint)n[1], (uint)n[2]}}, 0., 256, lo[0], lo[1], lo[2], hi[0], hi[1],
+                 hi[2]);
+    }
+    static Host host(const int *n, const double *lo, const double *hi)
+    {
+      return Host(provider(), {{(uint)n[0], (uint)n[1], (uint)n[2]}}, 0., 0, lo[0], lo[1], lo[2], hi[0], hi[1],
+                  hi[2]);
+    }
+  };
+  template <class F> void cart_dispatch(int D, F &&f)
+  {
+    switch (D) {
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 3: f(std::integral_constant<int, 3>()); break;
+    default: throw std::invalid_argument("Cartesian integrators have one to three axes");
+    }
+  }
+} // namespace
+
+extern "C" {
+
+// get() (req = 0) or request().get() (req = 1) of the D-axis Cartesian integrator on the GPU
+// (device 1) or the host (device 0); n, lo, hi: D entries; c: 4 cubic coefficients per axis
+int kat_cartesian(int device, int req, int D, const int *n, const double *lo, const double *hi, double k,
+                  double constant, const double *c, double *out)
+{
+  return guard([&] {
+    cart_dispatch(D, [&](auto dc) {
+      constexpr int DD = decltype(dc)::value;
+      if (device) {
+        auto I = CartOf<DD>::gpu(n, lo, hi);
+        *out = cart_call<DD>(I, req != 0, k, constant, c);
+      } else {
+        auto I = CartOf<DD>::host(n, lo, hi);
+        *out = cart_call<DD>(I, req != 0, k, constant, c);
+      }
+    });
+  });
+}
+
+// Range changes on the GPU integrator built on (lo, hi):
+//   out[0] a request made before the change, read after it;  out[1] a request after the change;
+//   out[2] get() after the change;  out[3] get() of a new integrator on (lo2, hi2);
+//   out[4] get() of a copy made before the change;  out[5] get() of a copy set to (lo2, hi2) with
+//   set_q?_max before set_q?_min
+int kat_cartesian_ranges(int D, const int *n, const double *lo, const double *hi, const double *lo2,
+                         const double *hi2, double k, const double *c, double *out)
+{
+  return guard([&] {
+    cart_dispatch(D, [&](auto dc) {
+      constexpr int DD = decltype(dc)::value;
+      auto I = CartOf<DD>::gpu(n, lo, hi);
+      auto J = I, K = I;
+      auto f0 = cart_request<DD>(I, k, 0.0, c);
+      cart_set<DD>(I, lo2, hi2, false);
+      auto f1 = cart_request<DD>(I, k, 0.0, c);
+      out[2] = cart_call<DD>(I, false, k, 0.0, c);
+      auto F = CartOf<DD>::gpu(n, lo2, hi2);
+      out[3] = cart_call<DD>(F, false, k, 0.0, c);
+      out[0] = f0.get();
+      out[1] = f1.get();
+      out[4] = cart_call<DD>(J, false, k, 0.0, c);
+      cart_set<DD>(K, lo2, hi2, true);
+      out[5] = cart_call<DD>(K, false, k, 0.0, c);
+    });
+  });
+}
+
+// The 1D integrator over number types nt = 0 double, 1 complex<double>, 2 dual: out = (first,
+// second component) of get(), GPU (device 1) or host (device 0)
+int kat_cartesian1_nt(int device, int nt, int n, double lo, double hi, double k, double constant, const double *c,
+                      double *out)
+{
+  return guard([&] {
+    auto run = [&](auto I) {
+      const auto v = I.get(k, constant, c[0], c[1], c[2], c[3]);
+      using V = std::decay_t<decltype(v)>;
+      if constexpr (std::is_same_v<V, double>) {
+        out[0] = v;
+        out[1] = 0.0;
+      } else if constexpr (std::is_same_v<V, Dual1>) {
+        out[0] = v.v;
+        out[1] = v.d;
+      } else {
+        out[0] = v.real();
+        out[1] = v.imag();
+      }
+    };
+    const std::array<uint, 1> g{{(uint)n}};
+    switch (nt * 2 + device) {
+    case 0: run(Integrator1DCartesianTBB<double, PolyCart1>(provider(), g, 0., 0, lo, hi)); break;
+    case 1: run(Integrator1DCartesianGPU<double, PolyCart1>(provider(), g, 0., 256, lo, hi)); break;
+    case 2: run(Integrator1DCartesianTBB<complex<double>, PolyCart1C>(provider(), g, 0., 0, lo, hi)); break;
+    case 3: run(Integrator1DCartesianGPU<complex<double>, PolyCart1C>(provider(), g, 0., 256, lo, hi)); break;
+    case 4: run(Integrator1DCartesianTBB<Dual1, PolyCart1D>(provider(), g, 0., 0, lo, hi)); break;
+    case 5: run(Integrator1DCartesianGPU<Dual1, PolyCart1D>(provider(), g, 0., 256, lo, hi)); break;
+    default: throw std::invalid_argument("unsupported number type");
     }
   });
 }

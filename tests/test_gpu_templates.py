@@ -319,6 +319,93 @@ def test_angle_template(kat, qp, d):
         assert is_close(out.value, closed, 1e-6)
 
 
+_dint = C.POINTER(C.c_int)
+CART_SHAPES = [(32,), (7,), (300,), (32, 32), (20, 12), (16, 300), (32, 32, 32), (24, 20, 12)]
+
+
+def _cart_closed(c, lo, hi, const):
+    nd = len(lo)
+    return const + np.prod([poly_int(c[4 * i:4 * i + 4], lo[i], hi[i]) for i in range(nd)]) / (2 * math.pi) ** nd
+
+
+def _cart(kat, device, req, n, lo, hi, k, const, c):
+    out = D()
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    call(kat, "kat_cartesian", device, req, len(n), nn.ctypes.data_as(_dint), P(lo), P(hi), D(k), D(const), P(c),
+         C.byref(out))
+    return out.value
+
+
+@pytest.mark.parametrize("n", CART_SHAPES, ids=lambda n: "x".join(map(str, n)))
+def test_cartesian_template(kat, qp, n):
+    # Integrator{1,2,3}DCartesianGPU/TBB (integrator_{1,2,3}D_cartesian_gpu.cu): oracle parity, the
+    # closed form of the cubic products, and request() bitwise get().  The shapes cover the flat
+    # units (several when an axis has more than 256 nodes) and the row units (qy qz planes of 256
+    # and more nodes), with node counts and ranges differing per axis.
+    nd = len(n)
+    rng = np.random.default_rng(90 + sum(n))
+    rules = [(qp.get_points(m), qp.get_weights(m)) for m in n]
+    for device in (1, 0):
+        lo, hi = arr(rng.uniform(-2, -1, nd)), arr(rng.uniform(1, 2, nd))
+        c = arr(rng.uniform(-1, 1, 4 * nd))
+        const, k = rng.uniform(-1, 1), rng.uniform(0, 1)
+        print(f"[case] cartesian n={n} device={device}", flush=True)
+        got = _cart(kat, device, 0, n, lo, hi, k, const, c)
+        ref = O.kat_cartesian(rules, lo, hi, c, const)
+        assert got == pytest.approx(ref, rel=TOL, abs=TOL * abs(const))
+        assert is_close(got, _cart_closed(c, lo, hi, const), 1e-6)
+        one = np.zeros(4 * nd)
+        one[0::4] = 1.0
+        vol = _cart(kat, device, 0, n, lo, hi, k, 0.0, arr(one))
+        assert is_close(vol, np.prod(hi - lo) / (2 * math.pi) ** nd, 1e-6)
+        if device:
+            assert _cart(kat, device, 1, n, lo, hi, k, const, c) == got  # request: bitwise get()
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_cartesian_range_mutators(kat, qp, nd):
+    # set_q?_min / set_q?_max (integrator_2D_cartesian_gpu.hh:144-171): a request keeps the range it
+    # was made with, later calls and requests use the new one, copies made before keep theirs, and
+    # set_q?_max before set_q?_min gives the same box
+    rng = np.random.default_rng(120 + nd)
+    n = [32, 20, 12][:nd]
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    lo, hi = arr(rng.uniform(-2, -1, nd)), arr(rng.uniform(1, 2, nd))
+    lo2, hi2 = arr(rng.uniform(-3, -2, nd)), arr(rng.uniform(0.5, 1, nd))
+    c = arr(rng.uniform(-1, 1, 4 * nd))
+    k = 0.5
+    out = np.zeros(6)
+    print(f"[case] cartesian ranges nd={nd}", flush=True)
+    call(kat, "kat_cartesian_ranges", nd, nn.ctypes.data_as(_dint), P(lo), P(hi), P(lo2), P(hi2), D(k), P(c),
+         P(out))
+    first = _cart(kat, 1, 0, n, lo, hi, k, 0.0, c)
+    assert out[0] == first and out[4] == first
+    assert out[1] == out[2] == out[3]
+    assert out[5] == pytest.approx(out[3], rel=1e-12)
+    rules = [(qp.get_points(m), qp.get_weights(m)) for m in n]
+    assert out[3] == pytest.approx(O.kat_cartesian(rules, lo2, hi2, c, 0.0), rel=TOL, abs=1e-14)
+    assert out[3] != pytest.approx(first, rel=1e-3)
+
+
+@pytest.mark.parametrize("nt", [0, 1, 2], ids=["double", "complex", "dual"])
+def test_cartesian_number_types(kat, qp, nt):
+    # integrator_1D_cartesian_gpu.cu: double, complex<double> and a dual number, GPU and host
+    rng = np.random.default_rng(130 + nt)
+    x, w = qp.get_points(32), qp.get_weights(32)
+    lo, hi = rng.uniform(-2, -1), rng.uniform(1, 2)
+    c = arr(rng.uniform(-1, 1, 4))
+    const = rng.uniform(-1, 1)
+    integral = O.kat_cartesian([(x, w)], [lo], [hi], c, 0.0)
+    second = [0.0, 0.5, 0.25][nt] * integral
+    for device in (1, 0):
+        out = np.zeros(2)
+        print(f"[case] cartesian nt={nt} device={device}", flush=True)
+        call(kat, "kat_cartesian1_nt", device, nt, 32, D(lo), D(hi), D(0.5), D(const), P(c), P(out))
+        assert out[0] == pytest.approx(const + integral, rel=TOL, abs=TOL)
+        assert out[1] == pytest.approx(second, rel=TOL, abs=TOL)
+        assert is_close(out[0], _cart_closed(c, [lo], [hi], const), 1e-6)
+
+
 def _sum_closed(T, val):
     return 1 / T / math.tanh(0.5 / val) / 2 / val
 

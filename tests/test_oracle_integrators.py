@@ -132,6 +132,28 @@ def test_angle_known_answers(d):
             assert O.kat_angle(d, xq, wq, xa, wa, x_ext, k, ce, a, 0.0) == pytest.approx(ref, rel=1e-12)
 
 
+@pytest.mark.parametrize("D", [1, 2, 3])
+def test_cartesian_known_answers(D):
+    # integrator_{1,2,3}D_cartesian_cpu.cc: volume prod(hi - lo)/(2pi)^D and products of cubics over
+    # random boxes (Gauss-Legendre with 32 nodes integrates cubics exactly, so to rounding); uneven
+    # node counts and ranges per axis catch a swapped axis, range or weight
+    rng = np.random.default_rng(60 + D)
+    ns = [32, 20, 12][:D]
+    rules = [O.gl_nodes(n) for n in ns]
+    for _ in range(4):
+        lo, hi = rng.uniform(-2, -1, D), rng.uniform(1, 2, D)
+        one = np.zeros(4 * D)
+        one[0::4] = 1.0
+        vol = O.kat_cartesian(rules, lo, hi, one, 0.0)
+        assert vol == pytest.approx(np.prod(hi - lo) / (2 * math.pi) ** D, rel=1e-13)
+        c = rng.uniform(-1, 1, 4 * D)
+        const = rng.uniform(-1, 1)
+        ref = const + np.prod([poly_int(c[4 * i:4 * i + 4], lo[i], hi[i]) for i in range(D)]) / (2 * math.pi) ** D
+        got = O.kat_cartesian(rules, lo, hi, c, const)
+        assert is_close(got, ref, 1e-6)
+        assert got == pytest.approx(ref, rel=1e-12, abs=1e-14)
+
+
 def _sum_closed(T, val):
     # T sum_n 1/(T^2 + val^2 (2 pi n T)^2) = coth(1/(2 val)) / (2 val T)
     return 1 / T / math.tanh(0.5 / val) / 2 / val
