@@ -133,7 +133,7 @@ namespace dfrg
       const AngNode<ctype> *ang = nullptr;          // angular tables, one per angular digit
       const ctype *mx = nullptr, *mw = nullptr;     // Matsubara rule (positive nodes)
       int nx = 0, na = 0, nm = 0;
-      int nb = 0;                                   // Cartesian: nodes of the third axis
+      int nb = 0;                                   // an inner axis of its own order (Cartesian qz, q0)
       ctype x_extent = 0, m_T = 0;
       ctype q_min = 0, q_ext = 0;                   // Cartesian: range of the first axis
     };
@@ -214,6 +214,78 @@ namespace dfrg
       {
         const AngNode<C> a1 = gld(n.ang + g[1]), ap = gld(n.ang + n.na + g[2]), a2 = gld(n.ang + 2 * n.na + g[3]);
         return NT((rf * (a1.a * ap.a * a2.a)) * KERNEL::kernel(q, a1.c, a2.c, ap.c, k, t...));
+      }
+    };
+
+    // Integrator3DGPU (integrator_3D_gpu.hh:21-43): (q, cos, phi); tables cos, then phi
+    struct Angle1Phi {
+      static constexpr int ND = 3, NA = 2;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = r[2] = n.na;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        return (q / C(2) * powr<2>(k)) / powr<3>(two_pi<C>);
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> a = gld(n.ang + g[1]), ph = gld(n.ang + n.na + g[2]);
+        return NT((rf * (a.a * ph.a)) * KERNEL::kernel(q, a.c, ph.c, k, t...));
+      }
+    };
+
+    // Zero-temperature frequency integrals over q0 in [-q0_ext, q0_ext] as f(q0) + f(-q0) on a
+    // Gauss-Legendre rule on [0, 1] mapped to [0, q0_ext] (table {x q0_ext, w q0_ext}, nb nodes).
+    // PX0: the rule is in x0 = q0/k (table {x x0_ext, w x0_ext}), so q0 = k x0 and the measure
+    // gains a factor k.
+    // Integrator2Dpq0GPU (integrator_2Dpq0_gpu.hh:21-46), Integrator2Dpx0GPU (integrator_2Dpx0_gpu.hh:
+    // 28-54): (q, cos, q0) in 2+1 dimensions; tables cos, then q0
+    template <bool PX0> struct Angle1Q0 {
+      static constexpr int ND = 3, NA = 2;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = n.na;
+        r[2] = n.nb;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C, C k)
+      {
+        const C e = (powr<2>(k) / C(2)) * two_pi<C> * (C(1) / C(2)) / powr<3>(two_pi<C>);
+        return PX0 ? e * k : e;
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> a = gld(n.ang + g[1]), z = gld(n.ang + n.na + g[2]);
+        const C q0 = PX0 ? k * z.c : z.c;
+        return NT((rf * (a.a * z.a)) * (KERNEL::kernel(q, a.c, q0, k, t...) + KERNEL::kernel(q, a.c, -q0, k, t...)));
+      }
+    };
+    // Integrator3Dpq0GPU (integrator_3Dpq0_gpu.hh:21-46), Integrator3Dpx0GPU (integrator_3Dpx0_gpu.hh:
+    // 21-47): (q, cos, phi, q0) in 3+1 dimensions; tables cos, phi, then q0
+    template <bool PX0> struct Angle1PhiQ0 {
+      static constexpr int ND = 4, NA = 3;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        r[1] = r[2] = n.na;
+        r[3] = n.nb;
+      }
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        const C e = (q / C(2) * powr<2>(k)) / powr<4>(two_pi<C>);
+        return PX0 ? e * k : e;
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        const AngNode<C> a = gld(n.ang + g[1]), ph = gld(n.ang + n.na + g[2]), z = gld(n.ang + 2 * n.na + g[3]);
+        const C q0 = PX0 ? k * z.c : z.c;
+        return NT((rf * (a.a * ph.a * z.a)) *
+                  (KERNEL::kernel(q, a.c, ph.c, q0, k, t...) + KERNEL::kernel(q, a.c, ph.c, -q0, k, t...)));
       }
     };
 

@@ -621,6 +621,52 @@ double orc_kat_cartesian(int D, const double *const *x, const double *const *w, 
   return constant + value;
 }
 
+double orc_kat_freq(int dim, int q0_axis, int px0, const double *xq, const double *wq, int nq, const double *xa,
+                    const double *wa, int na, const double *xb, const double *wb, int nb, double x_extent, double ext,
+                    double k, const double *c6, const double *a4, const double *p4, const double *z4, double constant)
+{
+  /* integrator_3D_cpu.hh:41-67 (dim 3, no q0 axis): (q/2 k^2)/(2pi)^3, weight 2pi w_phi 2 w_cos w_x x_extent;
+   * integrator_2Dpq0_cpu.hh:77-106 (dim 2): (k^2/2) S_2 (1/2)/(2pi)^3, weight w_q0 q0_ext 2 w_cos w_x x_extent,
+   *   f(q0) + f(-q0) with q0 = x q0_ext;
+   * integrator_3Dpq0_cpu.hh:43-75 (dim 3): (q/2 k^2)/(2pi)^4, weight w_q0 q0_ext 2pi w_phi 2 w_cos w_x x_extent;
+   * px0 (integrator_2Dpx0_cpu.hh:78-108, integrator_3Dpx0_cpu.hh:43-75): q0 = k x x0_ext, measure times k.
+   * Integrand poly6(q) poly4(cos) [poly4(phi)] [poly4(q0)]. */
+  const int nphi = dim == 3 ? na : 1, nz = q0_axis ? nb : 1;
+  double value = 0.0;
+  for (int ix = 0; ix < nq; ++ix) {
+    const double q = k * sqrt(xq[ix] * x_extent);
+    double int_element;
+    if (dim == 2)
+      int_element = (0.5 * k * k) * (2.0 * M_PI) * 0.5 / ipow(2.0 * M_PI, 3);
+    else
+      int_element = (q / 2.0 * k * k) / ipow(2.0 * M_PI, q0_axis ? 4 : 3);
+    if (px0) int_element *= k;
+    double part = 0.0;
+    for (int ic = 0; ic < na; ++ic) {
+      const double cos1 = 2.0 * (xa[ic] - 0.5);
+      for (int ip = 0; ip < nphi; ++ip) {
+        const double phi = 2.0 * M_PI * xa[ip];
+        for (int iz = 0; iz < nz; ++iz) {
+          double weight = 2.0 * wa[ic] * wq[ix] * x_extent;
+          double f = poly6(c6, q) * poly4(a4, cos1);
+          if (dim == 3) {
+            weight *= 2.0 * M_PI * wa[ip];
+            f *= poly4(p4, phi);
+          }
+          if (q0_axis) {
+            const double q0 = (px0 ? k : 1.0) * xb[iz] * ext;
+            weight *= wb[iz] * ext;
+            f *= poly4(z4, q0) + poly4(z4, -q0);
+          }
+          part += int_element * weight * f;
+        }
+      }
+    }
+    value += part;
+  }
+  return constant + value;
+}
+
 static inline double poly5(const double *c, double v)
 {
   return c[0] + c[1] * v + c[2] * v * v + c[3] * v * v * v + c[4] * v * v * v * v;

@@ -118,6 +118,12 @@ double orc_kat_angle(int d, const double *xq, const double *wq, int nq, const do
  * rule of axis i on [0,1] with n[i] nodes; c: 4 coefficients per axis */
 double orc_kat_cartesian(int D, const double *const *x, const double *const *w, const int *n, const double *lo,
                          const double *hi, const double *c, double constant);
+/* zero-temperature (q, cos[, phi][, q0]) forms: dim 3 without q0 axis = Integrator3D; dim 2/3 with a q0 axis =
+ * Integrator{2,3}Dpq0 (px0: the axis in x0 = q0/k); integrand poly6(q) poly4(cos) [poly4(phi)]
+ * [poly4(q0) + poly4(-q0)] (integrator_3D_cpu.cc, integrator_{2,3}D{pq0,px0}_cpu.cc) */
+double orc_kat_freq(int dim, int q0_axis, int px0, const double *xq, const double *wq, int nq, const double *xa,
+                    const double *wa, int na, const double *xb, const double *wb, int nb, double x_extent, double ext,
+                    double k, const double *c6, const double *a4, const double *p4, const double *z4, double constant);
 /* 4D finite-T q0: poly(q) poly1(cos) poly2(phi) / poly5(q0) (integrator_4D_finiteTq0_cpu.cc:14-40,
  * integrator_4D_finiteTq0_cpu.hh:90-120); equal cos and phi orders */
 double orc_kat_4d_finiteT(const double *xq, const double *wq, int nq, const double *xa, const double *wa, int na,

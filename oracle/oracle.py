@@ -95,6 +95,9 @@ def lib():
         L.orc_kat_cartesian.restype = C.c_double
         L.orc_kat_cartesian.argtypes = [C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(C.c_int), _dp, _dp, _dp,
                                         C.c_double]
+        L.orc_kat_freq.restype = C.c_double
+        L.orc_kat_freq.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int,
+                                   C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, C.c_double]
         L.orc_kat_4d_finiteT.restype = C.c_double
         L.orc_kat_4d_finiteT.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp,
                                          C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_double]
@@ -303,6 +306,13 @@ def kat_cartesian(rules, lo, hi, c, constant):
     lo, hi, c = _f64(lo), _f64(hi), _f64(c)
     assert len(lo) == D and len(hi) == D and len(c) == 4 * D
     return lib().orc_kat_cartesian(D, xp, wp, n, _p(lo), _p(hi), _p(c), constant)
+
+
+def kat_freq(dim, q0_axis, px0, xq, wq, xa, wa, xb, wb, x_extent, ext, k, c6, a4, p4, z4, constant):
+    """Integrator3D (dim 3, no q0 axis) and Integrator{2,3}D{pq0,px0}: see orc_kat_freq."""
+    xq, wq, xa, wa, xb, wb, c6, a4, p4, z4 = (_f64(v) for v in (xq, wq, xa, wa, xb, wb, c6, a4, p4, z4))
+    return lib().orc_kat_freq(dim, int(q0_axis), int(px0), _p(xq), _p(wq), len(xq), _p(xa), _p(wa), len(xa), _p(xb),
+                              _p(wb), len(xb), x_extent, ext, k, _p(c6), _p(a4), _p(p4), _p(z4), constant)
 
 
 def kat_4d_finiteT(xq, wq, xa, wa, x_extent, k, mx, mw, T, c6, a4, p4, d5, constant):

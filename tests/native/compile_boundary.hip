@@ -23,6 +23,16 @@
 #include <DiFfRG/physics/integration/integrator_2D_cartesian_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_3D_cartesian_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_3D_cartesian_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_2Dpq0_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_2Dpq0_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_2Dpx0_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_2Dpx0_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_3D_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_3D_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_3Dpq0_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_3Dpq0_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_3Dpx0_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_3Dpx0_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_2ang_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_2ang_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_cpu.hh>
@@ -122,6 +132,22 @@ namespace
     static __host__ __device__ double constant(double, double) { return 0.; }
   };
 
+  struct K3D {
+    static __host__ __device__ double kernel(double q, double c, double ph, double k, double a) { return a * q * c * ph * k; }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct KQ0 {
+    static __host__ __device__ double kernel(double q, double c, double q0, double k, double a) { return a * q * c / (q0 * q0 + k); }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+  struct K3Q0 {
+    static __host__ __device__ double kernel(double q, double c, double ph, double q0, double k, double a)
+    {
+      return a * q * c * ph / (q0 * q0 + k);
+    }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
+
   // a wrapper of the shape the generated X_integrator classes have (templated request<NT>)
   template <class I> struct Wrapper {
     I &integ;
@@ -163,6 +189,19 @@ double instantiate_with_reference_json(const ReferenceJSONValue &json)
   h3t.set_qz_max(2.);
   s += h1.get(1., 2.) + h1x.request(1., 2.).get() + h2.get(1., 2.) + h3.request(1., 2.).get();
   s += h1t.get(1., 2.) + h2t.request(1., 2.).get() + h3t.get(1., 2.);
+  // zero-temperature (q, cos[, phi][, q0]) forms with JSON constructors
+  Integrator3DGPU<double, K3D> f3(qp, {{32, 8, 8}}, 20., json);
+  Integrator2Dpq0GPU<double, KQ0> f2q(qp, {{32, 8, 8}}, 20., 2., json);
+  Integrator2Dpx0GPU<double, KQ0> f2x(qp, {{32, 8, 8}}, 20., 2., json);
+  Integrator3Dpq0GPU<double, K3Q0> f3q(qp, {{32, 8, 8, 8}}, 20., 2., json);
+  Integrator3Dpx0GPU<double, K3Q0> f3x(qp, {{32, 8, 8, 8}}, 20., 2., json);
+  Integrator3DTBB<double, K3D> f3t(qp, {{32, 8, 8}}, 20., json);
+  Integrator2Dpq0TBB<double, KQ0> f2qt(qp, {{32, 8, 8}}, 20., 2., json);
+  Integrator2Dpx0TBB<double, KQ0> f2xt(qp, {{32, 8, 8}}, 20., 2., json);
+  Integrator3Dpq0TBB<double, K3Q0> f3qt(qp, {{32, 8, 8, 8}}, 20., 2., json);
+  Integrator3Dpx0TBB<double, K3Q0> f3xt(qp, {{32, 8, 8, 8}}, 20., 2., json);
+  s += f3.get(1., 2.) + f2q.request(1., 2.).get() + f2x.get(1., 2.) + f3q.request(1., 2.).get() + f3x.get(1., 2.);
+  s += f3t.get(1., 2.) + f2qt.get(1., 2.) + f2xt.request(1., 2.).get() + f3qt.get(1., 2.) + f3xt.get(1., 2.);
   // request_data over a coordinate grid, through a generated-wrapper-shaped object
   LogarithmicCoordinates1D<double> grid(16, 1e-3, 10., 3.);
   IntegratorGPU<4, double, K1p> ap(qp, {{64}}, 20., json);

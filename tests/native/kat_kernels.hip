@@ -15,6 +15,16 @@
 #include <DiFfRG/physics/integration/integrator_2D_cartesian_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_3D_cartesian_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_3D_cartesian_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_2Dpq0_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_2Dpq0_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_2Dpx0_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_2Dpx0_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_3D_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_3D_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_3Dpq0_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_3Dpq0_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_3Dpx0_cpu.hh>
+#include <DiFfRG/physics/integration/integrator_3Dpx0_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_2ang_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_2ang_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_4D_cpu.hh>
@@ -383,6 +393,67 @@ namespace
                                                              const double, const double, const double, const double,
                                                              const double, const double, const double, const double,
                                                              const double, const double)
+    {
+      return c;
+    }
+  };
+
+  __forceinline__ __host__ __device__ double quintic(double v, double x0, double x1, double x2, double x3, double x4,
+                                                     double x5)
+  {
+    return x0 + x1 * powr<1>(v) + x2 * powr<2>(v) + x3 * powr<3>(v) + x4 * powr<4>(v) + x5 * powr<5>(v);
+  }
+  // integrator_3D_gpu.cu:16-37: poly(q) cos_poly(cos) phi_poly(phi)
+  struct PolyFreq3D {
+    static __forceinline__ __host__ __device__ auto
+    kernel(const double q, const double cos, const double phi, const double, const double, const double x0,
+           const double x1, const double x2, const double x3, const double x4, const double x5, const double c0,
+           const double c1, const double c2, const double c3, const double p0, const double p1, const double p2,
+           const double p3)
+    {
+      return quintic(q, x0, x1, x2, x3, x4, x5) * cubic(cos, c0, c1, c2, c3) * cubic(phi, p0, p1, p2, p3);
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+  // integrator_2Dpq0_gpu.cu:16-37 and integrator_2Dpx0_gpu.cu: poly(q) cos_poly(cos) q0_poly(q0)
+  struct PolyFreq2Q0 {
+    static __forceinline__ __host__ __device__ auto
+    kernel(const double q, const double cos, const double q0, const double, const double, const double x0,
+           const double x1, const double x2, const double x3, const double x4, const double x5, const double c0,
+           const double c1, const double c2, const double c3, const double z0, const double z1, const double z2,
+           const double z3)
+    {
+      return quintic(q, x0, x1, x2, x3, x4, x5) * cubic(cos, c0, c1, c2, c3) * cubic(q0, z0, z1, z2, z3);
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+  // integrator_3Dpq0_gpu.cu:16-38 and integrator_3Dpx0_gpu.cu: poly(q) cos_poly phi_poly q0_poly
+  struct PolyFreq3Q0 {
+    static __forceinline__ __host__ __device__ auto
+    kernel(const double q, const double cos, const double phi, const double q0, const double, const double,
+           const double x0, const double x1, const double x2, const double x3, const double x4, const double x5,
+           const double c0, const double c1, const double c2, const double c3, const double p0, const double p1,
+           const double p2, const double p3, const double z0, const double z1, const double z2, const double z3)
+    {
+      return quintic(q, x0, x1, x2, x3, x4, x5) * cubic(cos, c0, c1, c2, c3) * cubic(phi, p0, p1, p2, p3) *
+             cubic(q0, z0, z1, z2, z3);
+    }
+    static __forceinline__ __host__ __device__ auto
+    constant(const double, const double c, const double, const double, const double, const double, const double,
+             const double, const double, const double, const double, const double, const double, const double,
+             const double, const double, const double, const double, const double, const double)
     {
       return c;
     }
@@ -1202,6 +1273,53 @@ int kat_cartesian1_nt(int device, int nt, int n, double lo, double hi, double k,
     case 4: run(Integrator1DCartesianTBB<Dual1, PolyCart1D>(provider(), g, 0., 0, lo, hi)); break;
     case 5: run(Integrator1DCartesianGPU<Dual1, PolyCart1D>(provider(), g, 0., 256, lo, hi)); break;
     default: throw std::invalid_argument("unsupported number type");
+    }
+  });
+}
+
+// get() (req = 0) or request().get() (req = 1) of the zero-temperature forms on the GPU (device 1)
+// or the host (device 0): form 0 Integrator3D {nx, na, nphi}, 1 Integrator2Dpq0 {nx, na, nb},
+// 2 Integrator2Dpx0, 3 Integrator3Dpq0 {nx, na, nphi, nb}, 4 Integrator3Dpx0; ext is q0_extent or
+// x0_extent; c: 6 q coefficients, a, p, z: 4 cos, phi and q0 coefficients
+int kat_freq(int device, int form, int req, int nx, int na, int nphi, int nb, double x_ext, double ext, double k,
+             const double *c, const double *a, const double *p, const double *z, double constant, double *out)
+{
+  return guard([&] {
+    auto run3 = [&](auto &&I) {
+      *out = req ? I.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], p[0], p[1], p[2],
+                             p[3])
+                       .get()
+                 : I.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], p[0], p[1], p[2],
+                         p[3]);
+    };
+    auto run2 = [&](auto &&I) {
+      *out = req ? I.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], z[0], z[1], z[2],
+                             z[3])
+                       .get()
+                 : I.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], z[0], z[1], z[2],
+                         z[3]);
+    };
+    auto run4 = [&](auto &&I) {
+      *out = req ? I.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], p[0], p[1], p[2],
+                             p[3], z[0], z[1], z[2], z[3])
+                       .get()
+                 : I.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], p[0], p[1], p[2],
+                         p[3], z[0], z[1], z[2], z[3]);
+    };
+    const std::array<uint, 3> g3{{(uint)nx, (uint)na, (uint)nphi}}, g2{{(uint)nx, (uint)na, (uint)nb}};
+    const std::array<uint, 4> g4{{(uint)nx, (uint)na, (uint)nphi, (uint)nb}};
+    switch (form * 2 + device) {
+    case 0: run3(Integrator3DTBB<double, PolyFreq3D>(provider(), g3, x_ext)); break;
+    case 1: run3(Integrator3DGPU<double, PolyFreq3D>(provider(), g3, x_ext)); break;
+    case 2: run2(Integrator2Dpq0TBB<double, PolyFreq2Q0>(provider(), g2, x_ext, ext)); break;
+    case 3: run2(Integrator2Dpq0GPU<double, PolyFreq2Q0>(provider(), g2, x_ext, ext)); break;
+    case 4: run2(Integrator2Dpx0TBB<double, PolyFreq2Q0>(provider(), g2, x_ext, ext)); break;
+    case 5: run2(Integrator2Dpx0GPU<double, PolyFreq2Q0>(provider(), g2, x_ext, ext)); break;
+    case 6: run4(Integrator3Dpq0TBB<double, PolyFreq3Q0>(provider(), g4, x_ext, ext)); break;
+    case 7: run4(Integrator3Dpq0GPU<double, PolyFreq3Q0>(provider(), g4, x_ext, ext)); break;
+    case 8: run4(Integrator3Dpx0TBB<double, PolyFreq3Q0>(provider(), g4, x_ext, ext)); break;
+    case 9: run4(Integrator3Dpx0GPU<double, PolyFreq3Q0>(provider(), g4, x_ext, ext)); break;
+    default: throw std::invalid_argument("unsupported form");
     }
   });
 }

@@ -406,6 +406,62 @@ def test_cartesian_number_types(kat, qp, nt):
         assert is_close(out[0], _cart_closed(c, [lo], [hi], const), 1e-6)
 
 
+FREQ = {"3D": (0, 3, 0, 0), "2Dpq0": (1, 2, 1, 0), "2Dpx0": (2, 2, 1, 1), "3Dpq0": (3, 3, 1, 0), "3Dpx0": (4, 3, 1, 1)}
+
+
+@pytest.mark.parametrize("name", list(FREQ))
+@pytest.mark.parametrize("sizes", [(64, 12, 10), (20, 8, 40)], ids=["64x12x10", "20x8x40"])
+def test_freq_template(kat, qp, name, sizes):
+    # Integrator3DGPU/TBB and Integrator{2,3}D{pq0,px0}GPU/TBB (integrator_3D_gpu.cu,
+    # integrator_{2,3}D{pq0,px0}_gpu.cu): oracle parity, the reference tests' closed forms, and
+    # request() bitwise get(); the frequency axis has its own order (10 or 40 nodes)
+    form, dim, q0_axis, px0 = FREQ[name]
+    nx, na, nb = sizes
+    rng = np.random.default_rng(140 + form + nx)
+    xq, wq = qp.get_points(nx), qp.get_weights(nx)
+    xa, wa = qp.get_points(na), qp.get_weights(na)
+    xb, wb = qp.get_points(nb), qp.get_weights(nb)
+    for device in (1, 0):
+        x_ext, ext, k = rng.uniform(1, 2), rng.uniform(1, 2), rng.uniform(0.1, 1)
+        c, a, p, z = arr(rng.uniform(-1, 1, 6)), arr(rng.uniform(-1, 1, 4)), arr(rng.uniform(-1, 1, 4)), arr(
+            rng.uniform(-1, 1, 4))
+        const = rng.uniform(-1, 1)
+        print(f"[case] {name} sizes={sizes} device={device}", flush=True)
+        out = D()
+        call(kat, "kat_freq", device, form, 0, nx, na, na, nb, D(x_ext), D(ext), D(k), P(c), P(a), P(p), P(z),
+             D(const), C.byref(out))
+        ref = O.kat_freq(dim, q0_axis, px0, xq, wq, xa, wa, xb, wb, x_ext, ext, k, c, a, p, z, const)
+        assert out.value == pytest.approx(ref, rel=TOL, abs=TOL * abs(const))
+        q_ext = math.sqrt(x_ext) * k
+        Q = k * ext if px0 else ext
+        if dim == 2:
+            closed = S_d(2) / 2 * poly_int(c, 0, q_ext, 1) * poly_int(a, -1, 1) * poly_int(z, -Q, Q) / (2 * math.pi) ** 3
+        else:
+            closed = poly_int(c, 0, q_ext, 2) * poly_int(a, -1, 1) * poly_int(p, 0, 2 * math.pi)
+            closed /= (2 * math.pi) ** (4 if q0_axis else 3)
+            if q0_axis:
+                closed *= poly_int(z, -Q, Q)
+        assert is_close(out.value, const + closed, 1e-6 if nx >= 64 else 1e-4)
+        if device:
+            req = D()
+            call(kat, "kat_freq", device, form, 1, nx, na, na, nb, D(x_ext), D(ext), D(k), P(c), P(a), P(p), P(z),
+                 D(const), C.byref(req))
+            assert req.value == out.value
+
+
+@pytest.mark.parametrize("name", ["3D", "3Dpq0", "3Dpx0"])
+def test_freq_template_unequal_angles(kat, name):
+    # integrator_3D_gpu.hh:64-65, integrator_3Dpq0_gpu.hh:69-70: cos and phi orders must be equal
+    form = FREQ[name][0]
+    z4 = arr(np.zeros(4))
+    for device in (1, 0):
+        out = D()
+        rc = kat.kat_freq(device, form, 0, 16, 8, 6, 8, D(1.0), D(1.0), D(0.5), P(arr(np.zeros(6))), P(z4), P(z4),
+                          P(z4), D(0.0), C.byref(out))
+        assert rc == -2
+        assert kat.kat_last_error() == b"Grid sizes must be currently equal for all angular dimensions!"
+
+
 def _sum_closed(T, val):
     return 1 / T / math.tanh(0.5 / val) / 2 / val
 

@@ -154,6 +154,50 @@ def test_cartesian_known_answers(D):
         assert got == pytest.approx(ref, rel=1e-12, abs=1e-14)
 
 
+FREQ_FORMS = [(3, 0, 0), (2, 1, 0), (2, 1, 1), (3, 1, 0), (3, 1, 1)]
+
+
+def freq_closed(dim, q0_axis, px0, q_ext, ext, k, c, a, p, z, const):
+    """integrator_3D_cpu.cc, integrator_{2,3}D{pq0,px0}_cpu.cc: the reference tests' closed forms."""
+    Q = k * ext if px0 else ext
+    if dim == 2:
+        val = S_d(2) / 2 * poly_int(c, 0, q_ext, 1) * poly_int(a, -1, 1) * poly_int(z, -Q, Q) / (2 * math.pi) ** 3
+    else:
+        val = poly_int(c, 0, q_ext, 2) * poly_int(a, -1, 1) * poly_int(p, 0, 2 * math.pi)
+        val /= (2 * math.pi) ** (4 if q0_axis else 3)
+        if q0_axis:
+            val *= poly_int(z, -Q, Q)
+    return const + val
+
+
+@pytest.mark.parametrize("form", FREQ_FORMS, ids=["3D", "2Dpq0", "2Dpx0", "3Dpq0", "3Dpx0"])
+def test_freq_known_answers(form):
+    # volumes V_{d}(q_ext) (times 2 Q with a frequency axis) / (2pi)^D and products of polynomials;
+    # exact to rounding when the q polynomial makes the radial integrand polynomial in x = q^2/k^2
+    # (odd powers for the q-measures of 3 spatial dimensions, even for 2); the frequency axis has
+    # its own order, so a swapped table or weight fails
+    dim, q0_axis, px0 = form
+    rng = np.random.default_rng(80 + 10 * dim + 2 * q0_axis + px0)
+    xq, wq = O.gl_nodes(64)
+    xa, wa = O.gl_nodes(12)
+    xb, wb = O.gl_nodes(10)
+    for _ in range(3):
+        x_ext, ext, k = rng.uniform(1, 2), rng.uniform(1, 2), rng.uniform(0.1, 1)
+        q_ext = math.sqrt(x_ext) * k
+        one6, one4 = np.array([1.0, 0, 0, 0, 0, 0]), np.array([1.0, 0, 0, 0])
+        vol = O.kat_freq(dim, q0_axis, px0, xq, wq, xa, wa, xb, wb, x_ext, ext, k, one6, one4, one4, one4, 0.0)
+        ref = freq_closed(dim, q0_axis, px0, q_ext, ext, k, one6, one4, one4, one4, 0.0)
+        assert is_close(vol, ref, 1e-6)
+        c, a, p, z = rng.uniform(-1, 1, 6), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4)
+        const = rng.uniform(-1, 1)
+        got = O.kat_freq(dim, q0_axis, px0, xq, wq, xa, wa, xb, wb, x_ext, ext, k, c, a, p, z, const)
+        assert is_close(got, freq_closed(dim, q0_axis, px0, q_ext, ext, k, c, a, p, z, const), 1e-6)
+        ce = c.copy()
+        ce[slice(0, None, 2) if dim == 3 else slice(1, None, 2)] = 0.0
+        got = O.kat_freq(dim, q0_axis, px0, xq, wq, xa, wa, xb, wb, x_ext, ext, k, ce, a, p, z, 0.0)
+        assert got == pytest.approx(freq_closed(dim, q0_axis, px0, q_ext, ext, k, ce, a, p, z, 0.0), rel=1e-12)
+
+
 def _sum_closed(T, val):
     # T sum_n 1/(T^2 + val^2 (2 pi n T)^2) = coth(1/(2 val)) / (2 val T)
     return 1 / T / math.tanh(0.5 / val) / 2 / val
