@@ -1,0 +1,37 @@
+// DiFfRG/physics/integration_finiteT/integrator_finiteTx0_gpu.hh -- IntegratorFiniteTx0GPU<d, NT,
+// KERNEL> (reference integrator_finiteTx0_gpu.hh:64-190) on the tile engine of dfrg/integrate.hh:
+// (d-1)-dimensional radial integral and the frequency axis in x0 = q0/k, the first x0_summands
+// Matsubara frequencies summed and the rest integrated up to x0_extent (dfrg/finiteTx0.hh);
+// grid_sizes = {q nodes, x0 integral nodes}.  get() is one launch; request()s are batched.
+#pragma once
+
+#include <array>
+
+#include "dfrg/finiteTx0.hh"
+
+namespace DiFfRG
+{
+  template <int d, typename NT, typename KERNEL>
+  class IntegratorFiniteTx0GPU : public dfrg::tmpl::FiniteTx0GPU<d, 0, NT, KERNEL>
+  {
+    using Base = dfrg::tmpl::FiniteTx0GPU<d, 0, NT, KERNEL>;
+
+  public:
+    using ctype = typename Base::ctype;
+
+    template <dfrg::JsonLike JSON>
+    IntegratorFiniteTx0GPU(QuadratureProvider &quadrature_provider, const std::array<unsigned, 2> grid_sizes,
+                           const ctype x_extent, const ctype x0_extent, const unsigned x0_summands, const JSON &json)
+        : IntegratorFiniteTx0GPU(quadrature_provider, grid_sizes, x_extent, x0_extent, x0_summands,
+                                 json.get_double("/physical/T"), json.get_uint("/integration/cudathreadsperblock", 256))
+    {
+    }
+    IntegratorFiniteTx0GPU(QuadratureProvider &quadrature_provider, const std::array<unsigned, 2> grid_sizes,
+                           const ctype x_extent, const ctype x0_extent, const unsigned x0_summands, const ctype T,
+                           const unsigned max_block_size = 256)
+        : Base(quadrature_provider, grid_sizes[0], 0, grid_sizes[1], x_extent, x0_extent, x0_summands, T)
+    {
+      (void)max_block_size;
+    }
+  };
+} // namespace DiFfRG

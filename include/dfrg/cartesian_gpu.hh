@@ -34,13 +34,6 @@ namespace dfrg
       using type = Cartesian3;
     };
 
-    // A new batcher state per range change (any integrator), so no batch mixes two ranges.
-    inline unsigned long long next_range_state()
-    {
-      static std::atomic<unsigned long long> s{1};
-      return s.fetch_add(1, std::memory_order_relaxed);
-    }
-
     template <int D, typename NT, typename KERNEL> class CartesianGPU
     {
       static_assert(D >= 1 && D <= 3, "Cartesian integrators have one to three axes");
@@ -106,7 +99,7 @@ namespace dfrg
           tab = upload_angles(t);
           nodes.ang = tab.get();
         }
-        state = next_range_state();
+        state = next_node_state();
       }
 
       std::array<ctype, D> lo{}, ext{};

@@ -140,6 +140,14 @@ namespace dfrg
 
     template <class ctype> constexpr ctype two_pi = ctype(2) * ctype(3.14159265358979323846);
 
+    // A new batcher state for an integrator whose nodes changed (any integrator, any change), so no
+    // batch of requests mixes two node sets.
+    inline unsigned long long next_node_state()
+    {
+      static std::atomic<unsigned long long> s{1};
+      return s.fetch_add(1, std::memory_order_relaxed);
+    }
+
     // Each shape: ND digits (radial first, the fastest last, as the reference's flat loops),
     // element(q, k) of the radial node, and the node term given the staged radial factor rf.
     // IntegratorGPU<d> (integrator_gpu.hh:34-60)
@@ -444,6 +452,74 @@ namespace dfrg
       for (size_t i = 0; i < x.size(); ++i) t[i] = {q_min + q_ext * x[i], q_ext * w[i]};
       return t;
     }
+
+    // Finite temperature in x0 = q0/k: the first nm Matsubara frequencies q0 = 2 pi n T as a sum
+    // (T f(0) + T sum_{0<n<nm} (f(q0_n) + f(-q0_n))), the rest of the frequency axis as an
+    // integral over x0 from 2 pi nm T / k to x0_extent on a Gauss-Legendre rule in log x0 (nb nodes
+    // in mx, mw), f(q0) + f(-q0).  At T = 0 (nm = 0) the log map starts at 1e-3, shifted back, with
+    // the reference's weight w log_ext q0 / k.  Digits (q, [cos, [phi,]] n), the frequency index
+    // fastest, n < nm the sum.  Nodes: m_T = T, nm summands, q_ext = x0_extent.
+    // KIND 0: IntegratorFiniteTx0GPU<d> (integrator_finiteTx0_gpu.hh:21-62), (q, q0);
+    // KIND 1: IntegratorAngleFiniteTx0GPU<d> (integrator_angle_finiteTx0_gpu.hh:21-69), (q, cos, q0);
+    // KIND 2: Integrator4DFiniteTx0GPU (integrator_4D_finiteTx0_gpu.hh:21-74), (q, cos, phi, q0).
+    template <int d, int KIND> struct FiniteTx0 {
+      static constexpr int ND = 2 + KIND, NA = KIND;
+      template <class C> static __host__ __device__ void radix(const Nodes<C> &n, int (&r)[ND])
+      {
+        r[0] = n.nx;
+        for (int i = 1; i <= KIND; ++i) r[i] = n.na;
+        r[ND - 1] = n.nm + n.nb;
+      }
+      // the part common to the sum and the integral: S_{d-1} q^{d-3}/2 k^2 (times 1/2 for the cos
+      // average of KIND 1; KIND 2 has its angles explicit)
+      template <class C> static __host__ __device__ __forceinline__ C element(C q, C k)
+      {
+        const C r = powr<d - 3>(q) / C(2) * powr<2>(k);
+        if constexpr (KIND == 2)
+          return r;
+        else if constexpr (KIND == 1)
+          return S_d_prec<C>(d - 1) * r * (C(1) / C(2));
+        else
+          return S_d_prec<C>(d - 1) * r;
+      }
+      template <class NT, class KERNEL, class C, class... T>
+      static __device__ __forceinline__ NT term(const Nodes<C> &n, C q, C rf, const int (&g)[ND], C k, const T &...t)
+      {
+        C af = C(1), c1 = C(0), ph = C(0);
+        if constexpr (KIND >= 1) {
+          const AngNode<C> a = gld(n.ang + g[1]);
+          af = a.a;
+          c1 = a.c;
+        }
+        if constexpr (KIND == 2) {
+          const AngNode<C> p = gld(n.ang + n.na + g[2]);
+          af *= p.a;
+          ph = p.c;
+        }
+        auto K = [&](C q0) {
+          if constexpr (KIND == 0)
+            return KERNEL::kernel(q, q0, k, t...);
+          else if constexpr (KIND == 1)
+            return KERNEL::kernel(q, c1, q0, k, t...);
+          else
+            return KERNEL::kernel(q, c1, ph, q0, k, t...);
+        };
+        const int y = g[ND - 1];
+        if (y < n.nm) {
+          const C f = (rf * af) * (n.m_T / powr<d - 1>(two_pi<C>));
+          if (y == 0) return NT(f * K(C(0)));
+          const C q0 = two_pi<C> * n.m_T * C(y);
+          return NT(f * (K(q0) + K(-q0)));
+        }
+        const C z = n.m_T == C(0) ? C(1e-3) : C(0);
+        const C start = (C(2) * C(n.nm) * C(3.14159265358979323846) * n.m_T) / k;
+        const C ls = log(start + z), le = log(n.q_ext / (start + z));
+        const int j = y - n.nm;
+        const C q0 = k * (exp(ls + le * gld(n.mx + j)) - z);
+        const C w = gld(n.mw + j) * le * q0 / k;
+        return NT(((rf * af) * (k / powr<d>(two_pi<C>)) * w) * (K(q0) + K(-q0)));
+      }
+    };
 
     // Angular tables of a shape from the Gauss-Legendre rule on [0,1] of order na (host nodes):
     //   cos = 2(x - 1/2), weight 2w; cos1 of the 4D forms also carries the Jacobian sqrt(1 - cos1^2);

@@ -667,6 +667,59 @@ double orc_kat_freq(int dim, int q0_axis, int px0, const double *xq, const doubl
   return constant + value;
 }
 
+double orc_kat_finiteTx0(int kind, int d, const double *xq, const double *wq, int nq, const double *xa,
+                         const double *wa, int na, const double *x0, const double *w0, int n0, double x_extent,
+                         double x0_extent, int summands, double T, double k, const double *c6, const double *a4,
+                         const double *p4, const double *z4, double constant)
+{
+  /* integrator_finiteTx0_cpu.hh:64-109 (kind 0), integrator_angle_finiteTx0_cpu.hh:69-122 (kind 1),
+   * integrator_4D_finiteTx0_cpu.hh:66-124 (kind 2, d = 4): sum over n < summands (summands = 0 at T = 0) with
+   * T S_{d-1} q^{d-3}/2 k^2 [1/2] / (2pi)^{d-1}, integral over x0 nodes log-mapped from 2 pi summands T / k
+   * (1e-3 at T = 0, shifted back) to x0_extent with S_{d-1} q^{d-3}/2 k^2 k [1/2] / (2pi)^d and weight
+   * w log_ext q0 / k; kind 2 has no S_{d-1} and no 1/2 and explicit cos and phi weights.  Integrand
+   * poly6(q) [poly4(cos)] [poly4(phi)] poly4(q0) + (q0 -> -q0). */
+  const double z = T == 0.0 ? 1e-3 : 0.0;
+  const int nm = T == 0.0 ? 0 : summands;
+  const double start = (2.0 * nm * M_PI * T) / k;
+  const double ls = log(start + z), le = log(x0_extent / (start + z));
+  const double S = kind == 2 ? 1.0 : S_d(d - 1);
+  const int nc = kind >= 1 ? na : 1, nphi = kind == 2 ? na : 1;
+  double value = 0.0;
+  for (int ix = 0; ix < nq; ++ix) {
+    const double q = k * sqrt(xq[ix] * x_extent);
+    const double base = S * (ipow(q, d - 3) / 2.0 * k * k) * (kind == 1 ? 0.5 : 1.0);
+    double part = 0.0;
+    for (int ic = 0; ic < nc; ++ic) {
+      const double cos1 = 2.0 * (xa[ic] - 0.5);
+      for (int ip = 0; ip < nphi; ++ip) {
+        const double phi = 2.0 * M_PI * xa[ip];
+        double weight = wq[ix] * x_extent;
+        double g = poly6(c6, q);
+        if (kind >= 1) {
+          weight *= 2.0 * wa[ic];
+          g *= poly4(a4, cos1);
+        }
+        if (kind == 2) {
+          weight *= 2.0 * M_PI * wa[ip];
+          g *= poly4(p4, phi);
+        }
+        for (int n = 0; n < nm; ++n) {
+          const double q0 = 2.0 * M_PI * T * n;
+          const double f = n == 0 ? poly4(z4, 0.0) : poly4(z4, q0) + poly4(z4, -q0);
+          part += T * base / ipow(2.0 * M_PI, d - 1) * weight * g * f;
+        }
+        for (int j = 0; j < n0; ++j) {
+          const double q0 = k * (exp(ls + le * x0[j]) - z);
+          const double wj = w0[j] * le * q0 / k;
+          part += base * k / ipow(2.0 * M_PI, d) * weight * wj * g * (poly4(z4, q0) + poly4(z4, -q0));
+        }
+      }
+    }
+    value += part;
+  }
+  return constant + value;
+}
+
 static inline double poly5(const double *c, double v)
 {
   return c[0] + c[1] * v + c[2] * v * v + c[3] * v * v * v + c[4] * v * v * v * v;

@@ -124,6 +124,13 @@ double orc_kat_cartesian(int D, const double *const *x, const double *const *w, 
 double orc_kat_freq(int dim, int q0_axis, int px0, const double *xq, const double *wq, int nq, const double *xa,
                     const double *wa, int na, const double *xb, const double *wb, int nb, double x_extent, double ext,
                     double k, const double *c6, const double *a4, const double *p4, const double *z4, double constant);
+/* finite-T in x0 = q0/k: kind 0 IntegratorFiniteTx0<d>, 1 IntegratorAngleFiniteTx0<d>, 2 Integrator4DFiniteTx0;
+ * x0, w0: the integral's rule on [0,1] (grid order + summands at T = 0); integrand poly6(q) [poly4(cos)]
+ * [poly4(phi)] poly4(q0) (integrator_{,angle_,4D_}finiteTx0_cpu.cc) */
+double orc_kat_finiteTx0(int kind, int d, const double *xq, const double *wq, int nq, const double *xa,
+                         const double *wa, int na, const double *x0, const double *w0, int n0, double x_extent,
+                         double x0_extent, int summands, double T, double k, const double *c6, const double *a4,
+                         const double *p4, const double *z4, double constant);
 /* 4D finite-T q0: poly(q) poly1(cos) poly2(phi) / poly5(q0) (integrator_4D_finiteTq0_cpu.cc:14-40,
  * integrator_4D_finiteTq0_cpu.hh:90-120); equal cos and phi orders */
 double orc_kat_4d_finiteT(const double *xq, const double *wq, int nq, const double *xa, const double *wa, int na,

@@ -98,6 +98,10 @@ def lib():
         L.orc_kat_freq.restype = C.c_double
         L.orc_kat_freq.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int,
                                    C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, C.c_double]
+        L.orc_kat_finiteTx0.restype = C.c_double
+        L.orc_kat_finiteTx0.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int,
+                                        C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp,
+                                        C.c_double]
         L.orc_kat_4d_finiteT.restype = C.c_double
         L.orc_kat_4d_finiteT.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp,
                                          C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_double]
@@ -313,6 +317,14 @@ def kat_freq(dim, q0_axis, px0, xq, wq, xa, wa, xb, wb, x_extent, ext, k, c6, a4
     xq, wq, xa, wa, xb, wb, c6, a4, p4, z4 = (_f64(v) for v in (xq, wq, xa, wa, xb, wb, c6, a4, p4, z4))
     return lib().orc_kat_freq(dim, int(q0_axis), int(px0), _p(xq), _p(wq), len(xq), _p(xa), _p(wa), len(xa), _p(xb),
                               _p(wb), len(xb), x_extent, ext, k, _p(c6), _p(a4), _p(p4), _p(z4), constant)
+
+
+def kat_finiteTx0(kind, d, xq, wq, xa, wa, x0, w0, x_extent, x0_extent, summands, T, k, c6, a4, p4, z4, constant):
+    """IntegratorFiniteTx0 (kind 0), IntegratorAngleFiniteTx0 (1), Integrator4DFiniteTx0 (2): see
+    orc_kat_finiteTx0; x0, w0 is the integral's rule (its order is the grid's plus the summands at T = 0)."""
+    xq, wq, xa, wa, x0, w0, c6, a4, p4, z4 = (_f64(v) for v in (xq, wq, xa, wa, x0, w0, c6, a4, p4, z4))
+    return lib().orc_kat_finiteTx0(kind, d, _p(xq), _p(wq), len(xq), _p(xa), _p(wa), len(xa), _p(x0), _p(w0), len(x0),
+                                   x_extent, x0_extent, summands, T, k, _p(c6), _p(a4), _p(p4), _p(z4), constant)
 
 
 def kat_4d_finiteT(xq, wq, xa, wa, x_extent, k, mx, mw, T, c6, a4, p4, d5, constant):

@@ -42,6 +42,12 @@
 #include <DiFfRG/physics/integration/integrator_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTx0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTx0_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTx0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTx0_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_finiteTx0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_finiteTx0_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_cpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_gpu.hh>
@@ -202,6 +208,18 @@ double instantiate_with_reference_json(const ReferenceJSONValue &json)
   Integrator3Dpx0TBB<double, K3Q0> f3xt(qp, {{32, 8, 8, 8}}, 20., 2., json);
   s += f3.get(1., 2.) + f2q.request(1., 2.).get() + f2x.get(1., 2.) + f3q.request(1., 2.).get() + f3x.get(1., 2.);
   s += f3t.get(1., 2.) + f2qt.get(1., 2.) + f2xt.request(1., 2.).get() + f3qt.get(1., 2.) + f3xt.get(1., 2.);
+  // finite-T forms in x0 = q0/k (JSON: /physical/T)
+  IntegratorFiniteTx0GPU<4, double, KT> x1(qp, {{32, 16}}, 20., 100., 8, json);
+  IntegratorAngleFiniteTx0GPU<4, double, KAngleT> x2(qp, {{32, 8, 16}}, 20., 100., 8, json);
+  Integrator4DFiniteTx0GPU<double, K3Q0> x3(qp, {{32, 8, 8, 16}}, 20., 100., 8, json);
+  IntegratorFiniteTx0TBB<4, double, KT> x1t(qp, {{32, 16}}, 20., 100., 8, json);
+  IntegratorAngleFiniteTx0TBB<4, double, KAngleT> x2t(qp, {{32, 8, 16}}, 20., 100., 8, json);
+  Integrator4DFiniteTx0TBB<double, K3Q0> x3t(qp, {{32, 8, 8, 16}}, 20., 100., 8, json);
+  x1.set_T(0.1);
+  x2.set_x0_extent(50.);
+  x3t.set_T(0.);
+  s += x1.get(1., 2.) + x2.request(1., 2.).get() + x3.get(1., 2.) + x1t.get(1., 2.) + x2t.request(1., 2.).get() +
+       x3t.get(1., 2.);
   // request_data over a coordinate grid, through a generated-wrapper-shaped object
   LogarithmicCoordinates1D<double> grid(16, 1e-3, 10., 3.);
   IntegratorGPU<4, double, K1p> ap(qp, {{64}}, 20., json);

@@ -34,6 +34,12 @@
 #include <DiFfRG/physics/integration/integrator_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTx0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTx0_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTx0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTx0_gpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_finiteTx0_cpu.hh>
+#include <DiFfRG/physics/integration_finiteT/integrator_finiteTx0_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_cpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_angle_finiteTq0_gpu.hh>
@@ -454,6 +460,42 @@ namespace
     constant(const double, const double c, const double, const double, const double, const double, const double,
              const double, const double, const double, const double, const double, const double, const double,
              const double, const double, const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+
+  // integrator_finiteTx0_gpu.cu:14-33: poly(q) q0_poly(q0)
+  struct PolyTx0 {
+    static __forceinline__ __host__ __device__ auto kernel(const double q, const double q0, const double, const double,
+                                                           const double x0, const double x1, const double x2,
+                                                           const double x3, const double x4, const double x5,
+                                                           const double z0, const double z1, const double z2,
+                                                           const double z3)
+    {
+      return quintic(q, x0, x1, x2, x3, x4, x5) * cubic(q0, z0, z1, z2, z3);
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
+    {
+      return c;
+    }
+  };
+  // integrator_angle_finiteTx0_gpu.cu: poly(q) cos_poly(cos) q0_poly(q0)
+  struct PolyTx0Angle {
+    static __forceinline__ __host__ __device__ auto
+    kernel(const double q, const double cos, const double q0, const double, const double, const double x0,
+           const double x1, const double x2, const double x3, const double x4, const double x5, const double c0,
+           const double c1, const double c2, const double c3, const double z0, const double z1, const double z2,
+           const double z3)
+    {
+      return quintic(q, x0, x1, x2, x3, x4, x5) * cubic(cos, c0, c1, c2, c3) * cubic(q0, z0, z1, z2, z3);
+    }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double,
+                                                             const double, const double, const double, const double)
     {
       return c;
     }
@@ -1319,6 +1361,59 @@ int kat_freq(int device, int form, int req, int nx, int na, int nphi, int nb, do
     case 7: run4(Integrator3Dpq0GPU<double, PolyFreq3Q0>(provider(), g4, x_ext, ext)); break;
     case 8: run4(Integrator3Dpx0TBB<double, PolyFreq3Q0>(provider(), g4, x_ext, ext)); break;
     case 9: run4(Integrator3Dpx0GPU<double, PolyFreq3Q0>(provider(), g4, x_ext, ext)); break;
+    default: throw std::invalid_argument("unsupported form");
+    }
+  });
+}
+
+// get() (req = 0) or request().get() (req = 1) of the finite-T x0 forms on the GPU (device 1) or
+// the host (device 0): kind 0 IntegratorFiniteTx0<d> {nx, nint}, 1 IntegratorAngleFiniteTx0<d>
+// {nx, na, nint}, 2 Integrator4DFiniteTx0 {nx, na, na, nint}; built at T_first, then set_T(T)
+// when T_first differs; c: 6 q coefficients, a, p, z: 4 cos, phi and q0 coefficients
+int kat_finiteTx0(int device, int kind, int d, int req, int nx, int na, int nint, int summands, double x_ext,
+                  double x0_ext, double T_first, double T, double k, const double *c, const double *a, const double *p,
+                  const double *z, double constant, double *out)
+{
+  return guard([&] {
+    auto go1 = [&](auto &&I) {
+      if (T_first != T) I.set_T(T);
+      *out = req ? I.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], z[0], z[1], z[2], z[3]).get()
+                 : I.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], z[0], z[1], z[2], z[3]);
+    };
+    auto go2 = [&](auto &&I) {
+      if (T_first != T) I.set_T(T);
+      *out = req ? I.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], z[0], z[1], z[2],
+                             z[3])
+                       .get()
+                 : I.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], z[0], z[1], z[2],
+                         z[3]);
+    };
+    auto go4 = [&](auto &&I) {
+      if (T_first != T) I.set_T(T);
+      *out = req ? I.request(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], p[0], p[1], p[2],
+                             p[3], z[0], z[1], z[2], z[3])
+                       .get()
+                 : I.get(k, constant, c[0], c[1], c[2], c[3], c[4], c[5], a[0], a[1], a[2], a[3], p[0], p[1], p[2],
+                         p[3], z[0], z[1], z[2], z[3]);
+    };
+    const unsigned S = (unsigned)summands;
+    const std::array<uint, 2> g1{{(uint)nx, (uint)nint}};
+    const std::array<uint, 3> g2{{(uint)nx, (uint)na, (uint)nint}};
+    const std::array<uint, 4> g4{{(uint)nx, (uint)na, (uint)na, (uint)nint}};
+    const int key = kind * 100 + d * 2 + device;
+    switch (key) {
+    case 4: go1(IntegratorFiniteTx0TBB<2, double, PolyTx0>(provider(), g1, x_ext, x0_ext, S, T_first)); break;
+    case 5: go1(IntegratorFiniteTx0GPU<2, double, PolyTx0>(provider(), g1, x_ext, x0_ext, S, T_first)); break;
+    case 6: go1(IntegratorFiniteTx0TBB<3, double, PolyTx0>(provider(), g1, x_ext, x0_ext, S, T_first)); break;
+    case 7: go1(IntegratorFiniteTx0GPU<3, double, PolyTx0>(provider(), g1, x_ext, x0_ext, S, T_first)); break;
+    case 8: go1(IntegratorFiniteTx0TBB<4, double, PolyTx0>(provider(), g1, x_ext, x0_ext, S, T_first)); break;
+    case 9: go1(IntegratorFiniteTx0GPU<4, double, PolyTx0>(provider(), g1, x_ext, x0_ext, S, T_first)); break;
+    case 106: go2(IntegratorAngleFiniteTx0TBB<3, double, PolyTx0Angle>(provider(), g2, x_ext, x0_ext, S, T_first)); break;
+    case 107: go2(IntegratorAngleFiniteTx0GPU<3, double, PolyTx0Angle>(provider(), g2, x_ext, x0_ext, S, T_first)); break;
+    case 108: go2(IntegratorAngleFiniteTx0TBB<4, double, PolyTx0Angle>(provider(), g2, x_ext, x0_ext, S, T_first)); break;
+    case 109: go2(IntegratorAngleFiniteTx0GPU<4, double, PolyTx0Angle>(provider(), g2, x_ext, x0_ext, S, T_first)); break;
+    case 208: go4(Integrator4DFiniteTx0TBB<double, PolyFreq3Q0>(provider(), g4, x_ext, x0_ext, S, T_first)); break;
+    case 209: go4(Integrator4DFiniteTx0GPU<double, PolyFreq3Q0>(provider(), g4, x_ext, x0_ext, S, T_first)); break;
     default: throw std::invalid_argument("unsupported form");
     }
   });

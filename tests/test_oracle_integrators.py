@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 from scipy import integrate, special
 
+from closed_forms import tx0_closed
 from oracle import oracle as O
 
 
@@ -196,6 +197,42 @@ def test_freq_known_answers(form):
         ce[slice(0, None, 2) if dim == 3 else slice(1, None, 2)] = 0.0
         got = O.kat_freq(dim, q0_axis, px0, xq, wq, xa, wa, xb, wb, x_ext, ext, k, ce, a, p, z, 0.0)
         assert got == pytest.approx(freq_closed(dim, q0_axis, px0, q_ext, ext, k, ce, a, p, z, 0.0), rel=1e-12)
+
+
+TX0_FORMS = [(0, 2), (0, 3), (0, 4), (1, 3), (1, 4), (2, 4)]
+
+
+@pytest.mark.parametrize("form", TX0_FORMS, ids=["1D-d2", "1D-d3", "1D-d4", "angle-d3", "angle-d4", "4D"])
+def test_finiteTx0_known_answers(form):
+    # integrator_{,angle_,4D_}finiteTx0_cpu.cc: at T > 0 the Matsubara sum is exact and the log-mapped
+    # frequency integral of a polynomial converges to rounding, so with radial powers that make the
+    # q measure polynomial in x = q^2/k^2 the closed form holds to ~1e-13; at T = 0 the reference
+    # test's volume formula at its tolerance (the map starts at 1e-3 and its weight omits the
+    # shift, a 1e-5-level effect there)
+    kind, d = form
+    rng = np.random.default_rng(200 + 10 * kind + d)
+    xq, wq = O.gl_nodes(64)
+    xa, wa = O.gl_nodes(8)
+    ns, nint = 16, 64
+    x0, w0 = O.gl_nodes(nint)
+    for T in (0.3, 0.05):
+        k, x_ext = rng.uniform(0.3, 1), rng.uniform(1, 2)
+        x0_ext = ns * 10 * 2 * math.pi * T / k * rng.uniform(1, 2) + 20 / k
+        c, a, p, z = (rng.uniform(-1, 1, m) for m in (6, 4, 4, 4))
+        c[(0 if (d - 3) % 2 else 1)::2] = 0.0  # powers m with d - 3 + m even
+        const = rng.uniform(-1, 1)
+        got = O.kat_finiteTx0(kind, d, xq, wq, xa, wa, x0, w0, x_ext, x0_ext, ns, T, k, c, a, p, z, const)
+        ref = const + tx0_closed(kind, d, math.sqrt(x_ext) * k, k, x0_ext, ns, T, c, a, p, z)
+        assert got == pytest.approx(ref, rel=1e-12, abs=1e-13)
+    # T = 0: integrator_finiteTx0_gpu.cu's volume integral, x0_extent = 1000 / k
+    if kind == 0 and d > 2:
+        k, x_ext = 0.5, 1.5
+        x0_ext = 1000 / k * 1.3
+        x0, w0 = O.gl_nodes(nint + ns)
+        one6, one4 = np.array([1.0, 0, 0, 0, 0, 0]), np.array([1.0, 0, 0, 0])
+        got = O.kat_finiteTx0(0, d, xq, wq, xa, wa, x0, w0, x_ext, x0_ext, ns, 0.0, k, one6, one4, one4, one4, 0.0)
+        ref = V_d(d - 1, math.sqrt(x_ext) * k) / (2 * math.pi) ** (d - 1) * (k * x0_ext * 2 / (2 * math.pi))
+        assert is_close(got, ref, 5e-5)
 
 
 def _sum_closed(T, val):
