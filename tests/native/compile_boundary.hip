@@ -39,6 +39,7 @@
 #include <DiFfRG/physics/integration/integrator_4D_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_angle_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_angle_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_constant.hh>
 #include <DiFfRG/physics/integration/integrator_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_cpu.hh>
@@ -122,6 +123,10 @@ namespace
     static __host__ __device__ double constant(double, double) { return 0.; }
   };
 
+  struct KC0 {
+    static __host__ __device__ double kernel(double k, double a) { return a * k; }
+    static __host__ __device__ double constant(double, double) { return 0.; }
+  };
   struct KC1 {
     static __host__ __device__ double kernel(double qx, double k, double a) { return a * qx * k; }
     static __host__ __device__ double constant(double, double) { return 0.; }
@@ -220,6 +225,8 @@ double instantiate_with_reference_json(const ReferenceJSONValue &json)
   x3t.set_T(0.);
   s += x1.get(1., 2.) + x2.request(1., 2.).get() + x3.get(1., 2.) + x1t.get(1., 2.) + x2t.request(1., 2.).get() +
        x3t.get(1., 2.);
+  IntegratorConstant<4, double, KC0> cst(qp, {{64}}, 20., json);
+  s += cst.get(1., 2.) + cst.request(1., 2.).get();
   // request_data over a coordinate grid, through a generated-wrapper-shaped object
   LogarithmicCoordinates1D<double> grid(16, 1e-3, 10., 3.);
   IntegratorGPU<4, double, K1p> ap(qp, {{64}}, 20., json);

@@ -31,6 +31,7 @@
 #include <DiFfRG/physics/integration/integrator_4D_gpu.hh>
 #include <DiFfRG/physics/integration/integrator_angle_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_angle_gpu.hh>
+#include <DiFfRG/physics/integration/integrator_constant.hh>
 #include <DiFfRG/physics/integration/integrator_cpu.hh>
 #include <DiFfRG/physics/integration/integrator_gpu.hh>
 #include <DiFfRG/physics/integration_finiteT/integrator_4D_finiteTq0_cpu.hh>
@@ -499,6 +500,12 @@ namespace
     {
       return c;
     }
+  };
+
+  // integrator_constant.cc:12-24
+  struct PolyConstant {
+    static __forceinline__ __host__ __device__ auto kernel(const double, const double, const double x0) { return x0; }
+    static __forceinline__ __host__ __device__ auto constant(const double, const double c, const double) { return c; }
   };
 
   struct B1 {
@@ -1415,6 +1422,24 @@ int kat_finiteTx0(int device, int kind, int d, int req, int nx, int na, int nint
     case 208: go4(Integrator4DFiniteTx0TBB<double, PolyFreq3Q0>(provider(), g4, x_ext, x0_ext, S, T_first)); break;
     case 209: go4(Integrator4DFiniteTx0GPU<double, PolyFreq3Q0>(provider(), g4, x_ext, x0_ext, S, T_first)); break;
     default: throw std::invalid_argument("unsupported form");
+    }
+  });
+}
+
+// IntegratorConstant<d> (integrator_constant.cc): out[0] = get(), out[1] = request().get()
+int kat_constant(int d, double k, double constant, double x0, double *out)
+{
+  return guard([&] {
+    auto run = [&](auto I) {
+      out[0] = I.get(k, constant, x0);
+      out[1] = I.request(k, constant, x0).get();
+    };
+    switch (d) {
+    case 1: run(IntegratorConstant<1, double, PolyConstant>(provider(), 64, 1.)); break;
+    case 2: run(IntegratorConstant<2, double, PolyConstant>(provider(), 64, 1.)); break;
+    case 3: run(IntegratorConstant<3, double, PolyConstant>(provider(), 64, 1.)); break;
+    case 4: run(IntegratorConstant<4, double, PolyConstant>(provider(), {{64}}, 1.)); break;
+    default: throw std::invalid_argument("unsupported dimension");
     }
   });
 }

@@ -504,6 +504,18 @@ def test_finiteTx0_template(kat, qp, name):
                 assert run(0, 0.7) == got
 
 
+@pytest.mark.parametrize("d", [1, 2, 3, 4])
+def test_constant_integrator(kat, d):
+    # IntegratorConstant<d> (integrator_constant.cc): constant + x0 S_d / (2pi)^d, get and request
+    rng = np.random.default_rng(400 + d)
+    for _ in range(3):
+        k, const, x0 = rng.uniform(0, 1), rng.uniform(-1, 1), rng.uniform(-1, 1)
+        out = np.zeros(2)
+        call(kat, "kat_constant", d, D(k), D(const), D(x0), P(out))
+        assert out[0] == pytest.approx(const + S_d(d) * x0 / (2 * math.pi) ** d, rel=1e-15, abs=1e-15)
+        assert out[1] == out[0]
+
+
 def _sum_closed(T, val):
     return 1 / T / math.tanh(0.5 / val) / 2 / val
 
