@@ -597,6 +597,7 @@ struct dfrg_se_batch {
   std::vector<double> p, coord; // coord: c, 1/b, (n-1)/bias
   unsigned long long evals = 0;
   // device data
+  void *d_inputs = nullptr; // one allocation: nodes, tab64, tabA, tabB, p (256-byte aligned sections)
   double *d_nodes = nullptr, *d_tab64 = nullptr, *d_p = nullptr;
   SECellY *d_tabA = nullptr;
   SECellZ *d_tabB = nullptr;
@@ -970,11 +971,19 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
       return fail(DFRG_ERR_HIP);                                                                               \
     }                                                                                                          \
   } while (0)
-  DFRG_HIP_B(hipMalloc((void **)&b->d_nodes, sizeof(double) * nodes.size()));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_tab64, sizeof(double) * tables_len));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_tabA, sizeof(SECellY) * tables_len));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_tabB, sizeof(SECellZ) * tables_len));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_p, sizeof(double) * n_int));
+  // the read-only inputs in one allocation, so a block's first loads stay within a few pages
+  {
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t o_tab64 = up(sizeof(double) * nodes.size()), o_tabA = o_tab64 + up(sizeof(double) * tables_len);
+    const size_t o_tabB = o_tabA + up(sizeof(SECellY) * tables_len), o_p = o_tabB + up(sizeof(SECellZ) * tables_len);
+    DFRG_HIP_B(hipMalloc(&b->d_inputs, o_p + up(sizeof(double) * n_int)));
+    char *base = (char *)b->d_inputs;
+    b->d_nodes = (double *)base;
+    b->d_tab64 = (double *)(base + o_tab64);
+    b->d_tabA = (SECellY *)(base + o_tabA);
+    b->d_tabB = (SECellZ *)(base + o_tabB);
+    b->d_p = (double *)(base + o_p);
+  }
   DFRG_HIP_B(hipMalloc((void **)&b->d_counters, sizeof(unsigned) * dev::kCounterStride * n_int));
   DFRG_HIP_B(hipMalloc((void **)&b->d_queue, sizeof(unsigned) * 2 * dev::kCounterStride));
   DFRG_HIP_B(hipMemcpy(b->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice));
@@ -1008,11 +1017,7 @@ int dfrg_se_batch_destroy(dfrg_se_batch *b)
   if (!b) return DFRG_OK;
   hipDeviceSynchronize();
   se_drop_graph(b);
-  hipFree(b->d_nodes);
-  hipFree(b->d_tab64);
-  hipFree(b->d_tabA);
-  hipFree(b->d_tabB);
-  hipFree(b->d_p);
+  hipFree(b->d_inputs);
   hipFree(b->d_tiles);
   hipFree(b->d_partials);
   hipFree(b->d_counters);
