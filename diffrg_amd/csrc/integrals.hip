@@ -597,7 +597,8 @@ struct dfrg_se_batch {
   std::vector<double> p, coord; // coord: c, 1/b, (n-1)/bias
   unsigned long long evals = 0;
   // device data
-  void *d_inputs = nullptr; // one allocation: nodes, tab64, tabA, tabB, p (256-byte aligned sections)
+  // one allocation: nodes, tab64, tabA, tabB, p, ticket counters, queue (256-byte aligned sections)
+  void *d_inputs = nullptr;
   double *d_nodes = nullptr, *d_tab64 = nullptr, *d_p = nullptr;
   SECellY *d_tabA = nullptr;
   SECellZ *d_tabB = nullptr;
@@ -971,21 +972,24 @@ int dfrg_se_batch_create(dfrg_se_batch **out, dfrg_qp *qp, int n_int, const doub
       return fail(DFRG_ERR_HIP);                                                                               \
     }                                                                                                          \
   } while (0)
-  // the read-only inputs in one allocation, so a block's first loads stay within a few pages
+  // the inputs and the ticket counters in one allocation, so a block's first loads and its
+  // cross-block finish stay within a few pages
   {
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t o_tab64 = up(sizeof(double) * nodes.size()), o_tabA = o_tab64 + up(sizeof(double) * tables_len);
     const size_t o_tabB = o_tabA + up(sizeof(SECellY) * tables_len), o_p = o_tabB + up(sizeof(SECellZ) * tables_len);
-    DFRG_HIP_B(hipMalloc(&b->d_inputs, o_p + up(sizeof(double) * n_int)));
+    const size_t o_cnt = o_p + up(sizeof(double) * n_int);
+    const size_t o_queue = o_cnt + up(sizeof(unsigned) * dev::kCounterStride * n_int);
+    DFRG_HIP_B(hipMalloc(&b->d_inputs, o_queue + up(sizeof(unsigned) * 2 * dev::kCounterStride)));
     char *base = (char *)b->d_inputs;
     b->d_nodes = (double *)base;
     b->d_tab64 = (double *)(base + o_tab64);
     b->d_tabA = (SECellY *)(base + o_tabA);
     b->d_tabB = (SECellZ *)(base + o_tabB);
     b->d_p = (double *)(base + o_p);
+    b->d_counters = (unsigned *)(base + o_cnt);
+    b->d_queue = (unsigned *)(base + o_queue);
   }
-  DFRG_HIP_B(hipMalloc((void **)&b->d_counters, sizeof(unsigned) * dev::kCounterStride * n_int));
-  DFRG_HIP_B(hipMalloc((void **)&b->d_queue, sizeof(unsigned) * 2 * dev::kCounterStride));
   DFRG_HIP_B(hipMemcpy(b->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tab64, tables, sizeof(double) * tables_len, hipMemcpyHostToDevice));
   DFRG_HIP_B(hipMemcpy(b->d_tabA, tabA.data(), sizeof(SECellY) * tables_len, hipMemcpyHostToDevice));
@@ -1020,8 +1024,6 @@ int dfrg_se_batch_destroy(dfrg_se_batch *b)
   hipFree(b->d_inputs);
   hipFree(b->d_tiles);
   hipFree(b->d_partials);
-  hipFree(b->d_counters);
-  hipFree(b->d_queue);
   delete b;
   return DFRG_OK;
 }
