@@ -1228,7 +1228,7 @@ int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cac
     int bl, te, gr;
   };
   std::vector<Cand> cands;
-  // round 1: every shape, `reps` launches each; a shape counts only when every launch and event
+  // first sweep: every shape, `reps` launches each; a shape counts only when every launch and event
   // call of its measurement succeeded
   for (int bl : blocks) {
     for (int te : tiles) {
@@ -1244,7 +1244,7 @@ int dfrg_se_batch_autotune(dfrg_se_batch *b, int mode, int reps, const char *cac
       }
     }
   }
-  // round 2: the four fastest again, 4 x reps launches each, in three interleaved passes (the
+  // second sweep: the four fastest again, 4 x reps launches each, in three interleaved passes (the
   // best pass counts), so one noisy measurement does not decide between shapes a few % apart
   std::stable_sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.t < y.t; });
   if (cands.size() > 4) cands.resize(4);
