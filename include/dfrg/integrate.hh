@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -934,6 +935,24 @@ namespace dfrg
       return &tag;
     }
 
+    // Launch mode of request().  Off (the default): requests collect into batches, launched when a
+    // future of the thread is read, when request_data() ends, or when the batch is full.  On: every
+    // request is launched at once as a batch of one, as the reference's request() launches its
+    // kernel at the call (integrator_4D_2ang_gpu.hh:170-197): the GPU starts with the first request
+    // and overlaps the caller's host work from there, at one launch per request.  The values are
+    // the same bit for bit either way.  Process-wide; starts from DFRG_LAUNCH_AT_REQUEST=1 in the
+    // environment.
+    inline std::atomic<bool> &launch_at_request_flag()
+    {
+      static std::atomic<bool> f{[] {
+        const char *e = std::getenv("DFRG_LAUNCH_AT_REQUEST");
+        return e != nullptr && e[0] == '1' && e[1] == '\0';
+      }()};
+      return f;
+    }
+    inline void set_launch_at_request(bool on) { launch_at_request_flag().store(on, std::memory_order_relaxed); }
+    inline bool launch_at_request() { return launch_at_request_flag().load(std::memory_order_relaxed); }
+
     // Batches opened on this thread that are not launched yet.
     inline std::vector<std::weak_ptr<BatchBase>> &thread_open()
     {
@@ -1087,7 +1106,7 @@ namespace dfrg
             break;
           }
         }
-        if (full) b->flush();
+        if (full || launch_at_request()) b->flush();
         return std::async(std::launch::deferred, [b, idx]() { return b->get(idx); });
       }
 

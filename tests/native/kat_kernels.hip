@@ -1444,6 +1444,13 @@ int kat_constant(int d, double k, double constant, double x0, double *out)
   });
 }
 
+// request() launch mode (dfrg::tmpl::set_launch_at_request): 1 launches every request at once
+int kat_set_launch_at_request(int on)
+{
+  dfrg::tmpl::set_launch_at_request(on != 0);
+  return dfrg::tmpl::launch_at_request() ? 1 : 0;
+}
+
 // *nm receives the Matsubara rule size (GPU template only)
 int kat_4d_finiteT(int device, int nx, int na, double x_extent, double T, double k, const double *c, const double *a,
                    const double *p, const double *den, double constant, double *out, int *nm)

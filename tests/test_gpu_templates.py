@@ -642,6 +642,23 @@ def test_variant_requests_match_get_bitwise(kat, which):
     assert np.array_equal(req, get)
 
 
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_launch_at_request_matches_get_bitwise(kat, which):
+    # set_launch_at_request(true): every request is launched at the call as a batch of one, as the
+    # reference's request() does; the values stay bitwise those of get() and of batched requests
+    ks = arr([1.0, 0.9, 0.7, 0.5, 0.3])
+    batched, get = np.empty(len(ks)), np.empty(len(ks))
+    call(kat, "kat_variant_requests", which, 32, 8, D(1.5), D(0.2), P(ks), len(ks), P(batched), P(get))
+    assert kat.kat_set_launch_at_request(1) == 1
+    try:
+        eager, get2 = np.empty(len(ks)), np.empty(len(ks))
+        print(f"[case] launch at request, variant {which}", flush=True)
+        call(kat, "kat_variant_requests", which, 32, 8, D(1.5), D(0.2), P(ks), len(ks), P(eager), P(get2))
+    finally:
+        assert kat.kat_set_launch_at_request(0) == 0
+    assert np.array_equal(eager, get) and np.array_equal(get2, get) and np.array_equal(batched, get)
+
+
 def test_request_batching_is_one_launch_per_batch(kat):
     c, p, Z = _se_case(n_p=64, nq=64, na=32)
     tg, tr = D(), D()
