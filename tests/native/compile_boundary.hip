@@ -227,6 +227,11 @@ double instantiate_with_reference_json(const ReferenceJSONValue &json)
        x3t.get(1., 2.);
   IntegratorConstant<4, double, KC0> cst(qp, {{64}}, 20., json);
   s += cst.get(1., 2.) + cst.request(1., 2.).get();
+  // FP32 node types (FP32 evaluation, FP64 accumulation) on the newer shapes
+  IntegratorFiniteTx0GPU<4, float, KT> xf(qp, {{32, 16}}, 20.f, 100.f, 8, 0.1f);
+  Integrator3Dpq0GPU<float, K3Q0> f3qf(qp, {{32, 8, 8, 8}}, 20.f, 2.f);
+  Integrator2DCartesianGPU<float, KC2> h2f(qp, {{32, 16}}, 0.f, 256, -1.f, -1.f, 1.f, 1.f);
+  s += xf.get(1.f, 2.) + f3qf.request(1.f, 2.).get() + h2f.get(1.f, 2.);
   // request_data over a coordinate grid, through a generated-wrapper-shaped object
   LogarithmicCoordinates1D<double> grid(16, 1e-3, 10., 3.);
   IntegratorGPU<4, double, K1p> ap(qp, {{64}}, 20., json);
